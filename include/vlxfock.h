@@ -1,0 +1,115 @@
+/*
+ * C ABI of the MI355X-native two-electron Fock build (libvlxfock.so).
+ *
+ * Plain pointers and sizes only.  Every function returns 0 on success and a
+ * negative code on failure; vlx_last_error() then describes the failure
+ * (thread-local).  Matrices are dense nao x nao, row-major, float64, in the
+ * reference AO order (L-major, then spherical component, then function;
+ * src/orbdata/MolecularBasis.cpp:400-410,452-470).
+ */
+#ifndef VLXFOCK_H
+#define VLXFOCK_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vlx_context vlx_context;
+typedef struct vlx_screener vlx_screener;
+
+enum {
+    VLX_OK = 0,
+    VLX_ERR_ARG = -1,      /* bad argument (shape, label, threshold, null pointer) */
+    VLX_ERR_UNSUPPORTED = -2, /* angular momentum or operator not built */
+    VLX_ERR_HIP = -3,      /* HIP runtime error */
+    VLX_ERR_NOMEM = -4
+};
+
+/* Last error message of the calling thread (never NULL). */
+const char* vlx_last_error(void);
+
+/* Library version string. */
+const char* vlx_version(void);
+
+/* Highest angular momentum per shell the built kernels cover. */
+int vlx_max_angular_momentum(void);
+
+/* Device context: HIP device, stream, Boys table in HBM.
+ * Replaces the implicit OpenMP state of CFockDriver
+ * (src/t4c_func/FockDriver.hpp:36-60). */
+int vlx_context_create(int device, vlx_context** out);
+void vlx_context_destroy(vlx_context* ctx);
+
+/* Block-size factor of the reference's task partitioning
+ * (src/t4c_func/FockDriver.cpp:356-376).  Accepted and stored for interface
+ * compatibility; the GPU schedule does not depend on it. */
+int vlx_set_block_size_factor(vlx_context* ctx, int factor);
+
+/*
+ * T4CScreener::partition(basis, molecule, "eri")
+ * (src/t4c_func/T4CScreener.cpp:63-118, src/orbdata/BlockedGtoPairBlock.cpp:22-152).
+ * Builds all shell pairs, computes max over components of (ab|ab) on the GPU,
+ * bins them by -floor(log10 sqrt(max)) and drops pairs with sqrt <= 1e-15.
+ *
+ * Shell table (one entry per contracted shell, atom order then file order):
+ *   l[s], nprim[s], prim_offset[s] into exps/norms (normalised contraction
+ *   coefficients), centers[3 s .. 3 s + 2] in bohr, ao_base[s] = AO index of
+ *   component 0, ao_stride[s] = AO distance between components (n_L).
+ */
+int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* nprim, const int* prim_offset,
+                        const double* exps, const double* norms, const double* centers, const int* ao_base,
+                        const int* ao_stride, int nao, vlx_screener** out);
+void vlx_screener_destroy(vlx_screener* scr);
+
+/* Pair maxima max_comp (ab|ab) as used for binning, nshells x nshells
+ * (symmetric), for comparison with CT4CScreener's values. */
+int vlx_screener_pair_maxima(const vlx_screener* scr, double* out);
+
+/* Number of kept (binned) shell pairs. */
+int vlx_screener_number_of_pairs(const vlx_screener* scr, long long* out);
+
+/*
+ * CFockDriver::compute(screener, D, label, exchange_factor, omega, ithreshold)
+ * (src/t4c_func/FockDriver.cpp:120-171; rank/nodes variant :234-289).
+ * label: "2jk" (2J - K), "2jkx" (2J - xf K), "j" (J), "k" (K), "kx" (xf K).
+ * ithreshold >= 0 applies the binned Schwarz selection bin_bra + bin_ket <=
+ * ithreshold; ithreshold < 0 computes every shell quartet (the reference's
+ * unscreened _compute_fock_omp(basis, molecule, ...) path).  omega must be 0
+ * (range-separated labels are not built).  D must be symmetric.
+ * Host buffers: density in, G out (both nao x nao).
+ */
+int vlx_fock_compute(vlx_context* ctx, vlx_screener* scr, const double* density, const char* label,
+                     double exchange_factor, double omega, int ithreshold, double* fock_out);
+
+/* Same as vlx_fock_compute on device buffers already resident in HBM; runs on
+ * the context's stream and returns without synchronising. */
+int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,
+                            double exchange_factor, double omega, int ithreshold, double* d_fock_out);
+
+/* Synchronise the context's stream. */
+int vlx_context_synchronize(vlx_context* ctx);
+
+/* Raw HIP stream of the context (hipStream_t), for event timing by callers. */
+void* vlx_context_stream(vlx_context* ctx);
+
+/*
+ * Work of one build at ithreshold: one record per angular class
+ * (la, lb, lc, ld) with the number of contracted and primitive shell quartets.
+ * Writes at most max_records; *nrecords receives the number available.
+ */
+int vlx_fock_work(vlx_screener* scr, int ithreshold, int max_records, int* nrecords, int* classes /* 4 per */,
+                  long long* quartets, long long* prim_quartets);
+
+/*
+ * Spherical integral blocks (ab|cd) for explicit shell quartets
+ * (a, b, c, d) = quartets[4 q .. 4 q + 3]; block q is written to out at the
+ * offset sum of the previous block sizes, index ((i nb + j) nc + k) nd + l.
+ * Used by compute_eri (src/t4c_func/FockDriver.cpp:76-117) and by tests.
+ */
+int vlx_eri_quartets(vlx_context* ctx, vlx_screener* scr, int nquartets, const int* quartets, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VLXFOCK_H */

@@ -1,0 +1,160 @@
+"""GPU parity of the HIP Fock build against the CPU oracle and the reference fixtures.
+
+FP64 bar: elementwise 1e-10 Hartree against the oracle on the same basis,
+geometry and density; the reference's own rule (rtol = atol = 1e-12,
+src/math/SubMatrix.cpp:105-115) against its fixtures.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.common import dimer, load, reference_equal, sto3g_water
+from veloxchem_amd import FockDriver, MolecularBasis, T4CScreener, make_matrix, mat_t
+from veloxchem_amd.workloads import pseudo_density, water, water_cluster
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _screened(mol, bas):
+    scr = T4CScreener()
+    scr.partition(bas, mol, 'eri')
+    return scr
+
+
+def test_screener_maxima_and_bins_match_oracle():
+    mol, bas = dimer('def2-svpd')
+    scr = _screened(mol, bas)
+    mx_gpu = scr.pair_maxima()
+    mx_ref, bins_ref = orc.pair_maxima(orc.basis_from(bas, mol))
+    iu = np.triu_indices(mx_ref.shape[0])
+    assert np.allclose(mx_gpu[iu], mx_ref[iu], rtol=1e-12, atol=1e-15)
+    assert int(np.sum(bins_ref[iu] >= 0)) == scr.number_of_pairs()
+
+
+def test_eri_blocks_match_oracle_all_classes():
+    mol = water()
+    bas = MolecularBasis.read(mol, 'def2-svp')
+    scr = _screened(mol, bas)
+    ob = orc.basis_from(bas, mol)
+    ns = bas.number_of_shells()
+    rng = np.random.default_rng(7)
+    quartets = [tuple(int(x) for x in rng.integers(0, ns, 4)) for _ in range(400)]
+    blocks = FockDriver().eri_quartets(scr, quartets)
+    worst = 0.0
+    for q, blk in zip(quartets, blocks):
+        ref = orc.eri_sph(ob, *q)
+        worst = max(worst, float(np.max(np.abs(blk - ref))))
+    assert worst < 1e-12, worst
+
+
+def test_compute_eri_sto3g_fixture():
+    # reference tests/test_fock_eri.py:26-36
+    mol, bas = sto3g_water()
+    eri = FockDriver().compute_eri(mol, bas)
+    assert np.max(np.abs(eri - load('h2o.sto3g.eri.npy'))) < 1e-12
+
+
+@pytest.mark.parametrize('label,xf', [('j', 0.0), ('k', 0.0), ('2jk', 0.0), ('2jkx', 0.68), ('kx', 0.73)])
+def test_sto3g_fixtures_all_labels(label, xf):
+    mol, bas = sto3g_water()
+    scr = _screened(mol, bas)
+    dm = make_matrix(bas, mat_t.symmetric)
+    dm.set_values(load('h2o.sto3g.density.npy'))
+    g = FockDriver()._compute_fock_omp(scr, dm, label, xf, 0.0, 15).to_numpy()
+    j, k = load('h2o.sto3g.j.npy'), load('h2o.sto3g.k.npy')
+    ref = {'j': j, 'k': k, '2jk': 2 * j - k, '2jkx': 2 * j - xf * k, 'kx': xf * k}[label]
+    assert reference_equal(g, ref)
+
+
+def test_sto3g_unscreened_path():
+    mol, bas = sto3g_water()
+    dm = make_matrix(bas, mat_t.symmetric)
+    dm.set_values(load('h2o.sto3g.density.npy'))
+    g = FockDriver()._compute_fock_omp(bas, mol, dm, '2jk', 0.0, 0.0).to_numpy()
+    assert reference_equal(g, 2 * load('h2o.sto3g.j.npy') - load('h2o.sto3g.k.npy'))
+
+
+@pytest.mark.parametrize('label', ['j', 'k', '2jk'])
+def test_dimer_svpd_fixtures(label):
+    mol, bas = dimer('def2-svpd')
+    scr = _screened(mol, bas)
+    dm = make_matrix(bas, mat_t.symmetric)
+    dm.set_values(load('h2o.dimer.svpd.density.npy'))
+    g = FockDriver().compute(scr, dm, label, 0.0, 0.0, 15).to_numpy()
+    j, k = load('h2o.dimer.svpd.j.npy'), load('h2o.dimer.svpd.k.npy')
+    ref = {'j': j, 'k': k, '2jk': 2 * j - k}[label]
+    assert reference_equal(g, ref)
+
+
+def test_dimer_tzvp_fixture_2jk():
+    from veloxchem_amd import _native
+    if _native.lib().vlx_max_angular_momentum() < 3:
+        pytest.skip('f-shell kernels not built')
+    mol, bas = dimer('def2-tzvp')
+    scr = _screened(mol, bas)
+    dm = load('h2o.dimer.tzvp.density.npy')
+    g = FockDriver().compute(scr, dm, '2jk', 0.0, 0.0, 15)
+    assert reference_equal(g, 2 * load('h2o.dimer.tzvp.j.npy') - load('h2o.dimer.tzvp.k.npy'))
+
+
+def test_water_svp_config_vs_oracle_1e10():
+    mol = water()
+    bas = MolecularBasis.read(mol, 'def2-svp')
+    scr = _screened(mol, bas)
+    dm = pseudo_density(24, 5, seed=1)
+    g = FockDriver().compute(scr, dm, '2jk', 1.0, 0.0, 12)
+    ref, _ = orc.fock(orc.basis_from(bas, mol), dm, '2jk', 1.0, 12)
+    assert np.max(np.abs(g - ref)) < TOL
+
+
+def test_four_water_cluster_vs_oracle_1e10():
+    mol = water_cluster(4, 6.0, seed=4)
+    bas = MolecularBasis.read(mol, 'def2-svp')
+    scr = _screened(mol, bas)
+    dm = pseudo_density(96, 20, seed=2)
+    g = FockDriver().compute(scr, dm, '2jk', 1.0, 0.0, 12)
+    ref, _ = orc.fock(orc.basis_from(bas, mol), dm, '2jk', 1.0, 12)
+    assert np.max(np.abs(g - ref)) < TOL
+
+
+def test_cluster32_properties():
+    # full-size workload: symmetry, linearity in D and label consistency
+    mol = water_cluster(32, 10.0, seed=32)
+    bas = MolecularBasis.read(mol, 'def2-svp')
+    scr = _screened(mol, bas)
+    n = bas.get_dimensions_of_basis()
+    assert n == 768
+    d1 = pseudo_density(n, 160, seed=3)
+    d2 = pseudo_density(n, 160, seed=4)
+    drv = FockDriver()
+    g1 = drv.compute(scr, d1, '2jk', 1.0, 0.0, 12)
+    g2 = drv.compute(scr, d2, '2jk', 1.0, 0.0, 12)
+    g12 = drv.compute(scr, d1 + d2, '2jk', 1.0, 0.0, 12)
+    assert np.max(np.abs(g1 - g1.T)) < 1e-12
+    assert np.max(np.abs(g12 - g1 - g2)) < 1e-10
+    j = drv.compute(scr, d1, 'j', 0.0, 0.0, 12)
+    k = drv.compute(scr, d1, 'k', 0.0, 0.0, 12)
+    assert np.max(np.abs(2 * j - k - g1)) < 1e-10
+    # Coulomb energy is positive for a positive semi-definite density
+    assert float(np.sum(d1 * j)) > 0.0
+
+
+def test_error_paths():
+    mol, bas = sto3g_water()
+    scr = _screened(mol, bas)
+    dm = load('h2o.sto3g.density.npy')
+    drv = FockDriver()
+    with pytest.raises(ValueError):
+        drv.compute(scr, dm, 'xyz', 0.0, 0.0, 12)
+    with pytest.raises(NotImplementedError):
+        drv.compute(scr, dm, '2jk', 1.0, 0.4, 12)
+    with pytest.raises(ValueError):
+        drv.compute(scr, dm[:5, :5], '2jk', 1.0, 0.0, 12)
+    with pytest.raises(ValueError):
+        drv.compute(scr, dm, '2jk', 1.0, 0.0, 31)
+    bad = dm.copy()
+    bad[0, 1] += 1.0
+    with pytest.raises(ValueError):
+        drv.compute(scr, bad, '2jk', 1.0, 0.0, 12)

@@ -1,0 +1,99 @@
+"""Loader of the native Fock library (libvlxfock.so, HIP for gfx950).
+
+The product path has no fallback: if the library is missing or cannot
+create a device context, every call raises.
+"""
+
+import ctypes
+import os
+import re
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / 'libvlxfock.so'
+HEADER_PATH = _PKG.parent / 'include' / 'vlxfock.h'
+
+_lib = None
+_contexts = {}
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_ll_p = ctypes.POINTER(ctypes.c_longlong)
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise NativeError(f'{LIB_PATH} is missing: build it with __graft_entry__.build() '
+                              '(the Fock build has no CPU fallback)')
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.vlx_last_error.restype = ctypes.c_char_p
+        L.vlx_version.restype = ctypes.c_char_p
+        L.vlx_context_stream.restype = ctypes.c_void_p
+        L.vlx_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.vlx_context_destroy.argtypes = [ctypes.c_void_p]
+        L.vlx_context_stream.argtypes = [ctypes.c_void_p]
+        L.vlx_context_synchronize.argtypes = [ctypes.c_void_p]
+        L.vlx_set_block_size_factor.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.vlx_screener_create.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_double_p,
+                                          _c_double_p, _c_double_p, _c_int_p, _c_int_p, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_void_p)]
+        L.vlx_screener_destroy.argtypes = [ctypes.c_void_p]
+        L.vlx_screener_pair_maxima.argtypes = [ctypes.c_void_p, _c_double_p]
+        L.vlx_screener_number_of_pairs.argtypes = [ctypes.c_void_p, _c_ll_p]
+        L.vlx_fock_compute.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _c_double_p, ctypes.c_char_p,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_double_p]
+        L.vlx_fock_compute_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+        L.vlx_fock_work.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_ll_p, _c_ll_p]
+        L.vlx_eri_quartets.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_double_p]
+        _lib = L
+    return _lib
+
+
+def header_functions():
+    """Names of the functions declared in include/vlxfock.h."""
+    text = HEADER_PATH.read_text()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(vlx_[a-z_]+)\s*\(', text)))
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().vlx_last_error().decode()
+        if rc == -1:
+            raise ValueError(msg)
+        if rc == -2:
+            raise NotImplementedError(msg)
+        raise NativeError(msg)
+
+
+def context(device=0):
+    """Per-device context (stream + Boys table), created on first use."""
+    if device not in _contexts:
+        h = ctypes.c_void_p()
+        check(lib().vlx_context_create(int(device), ctypes.byref(h)))
+        _contexts[device] = h
+    return _contexts[device]
+
+
+def current_device():
+    return int(os.environ.get('VLX_DEVICE', '0'))
+
+
+def as_int(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_c_int_p)
+
+
+def as_double(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_c_double_p)

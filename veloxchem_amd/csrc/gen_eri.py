@@ -1,0 +1,313 @@
+"""Generator of the per-class electron-repulsion device functions (gfx950).
+
+For every shell-quartet class (la lb | lc ld) with la >= lb, lc >= ld and
+pair class (la, lb) >= (lc, ld) it writes one HIP source file
+``gen/eri_<la><lb><lc><ld>.hip`` that specialises
+
+    eri_quartet<la, lb, lc, ld>(bra pair, ket pair, Boys table, out[])
+
+as straight-line code the compiler keeps in registers:
+
+1. per primitive quartet: Boys F_0..F_L from the Taylor table, then the
+   Obara-Saika vertical recursion on centre A and electron transfer to
+   centre C (Head-Gordon-Pople form): [e0|00]^(m) -> [e0|f0]^(m), every
+   intermediate emitted once (memoised DAG, unused branches never emitted);
+2. contraction: [e0|f0]^(0) summed over bra and ket primitive pairs;
+3. horizontal recursions on the contracted values, ket first then bra,
+   each followed by the Cartesian -> real-solid-harmonic transform with the
+   factors and component order of src/general/SphericalMomentum.hpp:18-134
+   (p = y,z,x; d = xy,yz,z2,xz,x2-y2; f as listed there).
+
+The semantics of the produced integrals are those of the reference kernels
+erirec::comp_electron_repulsion_<abcd>
+(src/t4c_electron_repulsion/ElectronRepulsionRec*.hpp) -- contracted,
+normalised spherical (ab|cd) -- but the code is generated from the
+recurrences themselves, not from those files.
+
+It also writes ``gen/dispatch.inc`` with the table of launchers.
+
+Usage: python gen_eri.py OUTDIR LMAX
+"""
+
+import math
+import sys
+from pathlib import Path
+
+
+def cart_list(lang):
+    return [(i, j, lang - i - j) for i in range(lang, -1, -1) for j in range(lang - i, -1, -1)]
+
+
+def ncart(lang):
+    return (lang + 1) * (lang + 2) // 2
+
+
+def nsph(lang):
+    return 2 * lang + 1
+
+
+# Cartesian -> spherical factors: sph[l][m] = [(cart_index, coefficient)].
+_F3 = 2.0 * math.sqrt(3.0)
+_G5 = math.sqrt(2.5)
+_G15 = 2.0 * math.sqrt(15.0)
+_G3 = math.sqrt(1.5)
+SPH = {
+    0: [[(0, 1.0)]],
+    1: [[(1, 1.0)], [(2, 1.0)], [(0, 1.0)]],
+    2: [[(1, _F3)], [(4, _F3)], [(0, -1.0), (3, -1.0), (5, 2.0)], [(2, _F3)],
+        [(0, 0.5 * _F3), (3, -0.5 * _F3)]],
+    3: [[(1, 3.0 * _G5), (6, -_G5)], [(4, _G15)], [(8, 4.0 * _G3), (1, -_G3), (6, -_G3)],
+        [(9, 2.0), (2, -3.0), (7, -3.0)], [(5, 4.0 * _G3), (0, -_G3), (3, -_G3)],
+        [(2, 0.5 * _G15), (7, -0.5 * _G15)], [(0, _G5), (3, -3.0 * _G5)]],
+}
+
+AX = 'xyz'
+
+
+def lit(x):
+    r = repr(float(x))
+    return r if ('e' in r or '.' in r) else r + '.0'
+
+
+def add(t, i, d):
+    t = list(t)
+    t[i] += d
+    return tuple(t)
+
+
+class Code:
+    def __init__(self, prefix):
+        self.lines = []
+        self.n = 0
+        self.prefix = prefix
+
+    def new(self, expr):
+        name = f'{self.prefix}{self.n}'
+        self.n += 1
+        self.lines.append(f'const double {name} = {expr};')
+        return name
+
+
+def gen_vrr(code, la, lb, lc, ld, L):
+    """Emit the primitive VRR; returns {(e, f): var} for the contracted targets."""
+    memo = {}
+
+    def node(a, c, m):
+        key = (a, c, m)
+        if key in memo:
+            return memo[key]
+        if sum(c) == 0:
+            if sum(a) == 0:
+                memo[key] = f'b{m}'
+                return memo[key]
+            # raise on A: pick the axis with the smallest non-zero exponent
+            cands = [i for i in range(3) if a[i] > 0]
+            i = min(cands, key=lambda k: (a[k], k))
+            am = add(a, i, -1)
+            terms = [f'PA{AX[i]} * {node(am, c, m)}', f'WP{AX[i]} * {node(am, c, m + 1)}']
+            if am[i] > 0:
+                amm = add(am, i, -1)
+                terms.append(f'{lit(am[i])} * oo2z * ({node(amm, c, m)} - rz * {node(amm, c, m + 1)})')
+            memo[key] = code.new(' + '.join(terms))
+            return memo[key]
+        cands = [i for i in range(3) if c[i] > 0]
+        i = min(cands, key=lambda k: (a[k] > 0, c[k], k))
+        cm = add(c, i, -1)
+        terms = [f'QC{AX[i]} * {node(a, cm, m)}', f'WQ{AX[i]} * {node(a, cm, m + 1)}']
+        if cm[i] > 0:
+            cmm = add(cm, i, -1)
+            terms.append(f'{lit(cm[i])} * oo2e * ({node(a, cmm, m)} - re * {node(a, cmm, m + 1)})')
+        if a[i] > 0:
+            terms.append(f'{lit(a[i])} * oo2ze * {node(add(a, i, -1), cm, m + 1)}')
+        memo[key] = code.new(' + '.join(terms))
+        return memo[key]
+
+    targets = {}
+    for le in range(la, la + lb + 1):
+        for e in cart_list(le):
+            for lf in range(lc, lc + ld + 1):
+                for f in cart_list(lf):
+                    targets[(e, f)] = node(e, f, 0)
+    return targets
+
+
+def gen_class(la, lb, lc, ld):
+    L = la + lb + lc + ld
+    tag = f'{la}{lb}{lc}{ld}'
+    e_list = [e for le in range(la, la + lb + 1) for e in cart_list(le)]
+    f_list = [f for lf in range(lc, lc + ld + 1) for f in cart_list(lf)]
+    acc_index = {}
+    for ie, e in enumerate(e_list):
+        for jf, f in enumerate(f_list):
+            acc_index[(e, f)] = ie * len(f_list) + jf
+    nacc = len(acc_index)
+
+    vrr = Code('v')
+    targets = gen_vrr(vrr, la, lb, lc, ld, L)
+
+    out = []
+    w = out.append
+    w(f'// class ({la}{lb}|{lc}{ld}), L = {L}: {nacc} contracted [e0|f0] values, '
+      f'{len(vrr.lines)} VRR intermediates per primitive quartet')
+    w('template <>')
+    w(f'__device__ __forceinline__ void eri_quartet<{la}, {lb}, {lc}, {ld}>(')
+    w('    const PrimPair* __restrict__ bra_prims, const int bra_np, const double ABx, const double ABy, const double ABz,')
+    w('    const PrimPair* __restrict__ ket_prims, const int ket_np, const double CDx, const double CDy, const double CDz,')
+    w('    const double* __restrict__ boys_tab, double* __restrict__ out)')
+    w('{')
+    w(f'    double acc[{nacc}];')
+    w('#pragma unroll')
+    w(f'    for (int i = 0; i < {nacc}; i++) acc[i] = 0.0;')
+    w('    for (int ib = 0; ib < bra_np; ib++) {')
+    w('        const PrimPair bp = bra_prims[ib];')
+    w('        const double zeta = bp.zeta;')
+    w('        const double oo2z = 0.5 / zeta;')
+    w('        const double PAx = bp.pax, PAy = bp.pay, PAz = bp.paz;')
+    w('        for (int ik = 0; ik < ket_np; ik++) {')
+    w('            const PrimPair kp = ket_prims[ik];')
+    w('            const double eta = kp.zeta;')
+    w('            const double inv_s = 1.0 / (zeta + eta);')
+    w('            const double rho = zeta * eta * inv_s;')
+    w('            const double PQx = bp.px - kp.px, PQy = bp.py - kp.py, PQz = bp.pz - kp.pz;')
+    w('            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);')
+    w('            const double pref = bp.k * kp.k * sqrt(inv_s);')
+    w(f'            double F[{L + 1}];')
+    w(f'            boys_eval<{L}>(T, boys_tab, F);')
+    for m in range(L + 1):
+        w(f'            const double b{m} = pref * F[{m}];')
+    if L > 0:
+        w('            const double re = zeta * inv_s, rz = eta * inv_s;')
+        # W - P = -(eta / (zeta + eta)) (P - Q),  W - Q = (zeta / (zeta + eta)) (P - Q)
+        w('            const double WPx = -rz * PQx, WPy = -rz * PQy, WPz = -rz * PQz;')
+        w('            (void)WPx; (void)WPy; (void)WPz; (void)rz; (void)re; (void)PAx; (void)PAy; (void)PAz; (void)oo2z;')
+        w('            const double WQx = re * PQx, WQy = re * PQy, WQz = re * PQz;')
+        w('            (void)WQx; (void)WQy; (void)WQz;')
+        w('            const double QCx = kp.pax, QCy = kp.pay, QCz = kp.paz;')
+        w('            (void)QCx; (void)QCy; (void)QCz;')
+        w('            const double oo2e = 0.5 / eta, oo2ze = 0.5 * inv_s;')
+        w('            (void)oo2e; (void)oo2ze;')
+    for ln in vrr.lines:
+        w('            ' + ln)
+    for (e, f), var in targets.items():
+        w(f'            acc[{acc_index[(e, f)]}] += {var};')
+    w('        }')
+    w('    }')
+
+    # ---- ket HRR + ket cart->sph, for every bra e component
+    hrr = Code('h')
+    ket_sph = {}  # (e, kc, kd) -> var
+    for e in e_list:
+        memo = {}
+
+        def knode(c, d):
+            key = (c, d)
+            if key in memo:
+                return memo[key]
+            if sum(d) == 0:
+                memo[key] = f'acc[{acc_index[(e, c)]}]'
+                return memo[key]
+            i = [k for k in range(3) if d[k] > 0][0]
+            memo[key] = hrr.new(f'{knode(add(c, i, 1), add(d, i, -1))} + CD{AX[i]} * {knode(c, add(d, i, -1))}')
+            return memo[key]
+
+        cc, dd = cart_list(lc), cart_list(ld)
+        for kc in range(nsph(lc)):
+            for kd in range(nsph(ld)):
+                terms = []
+                for ic, fc in SPH[lc][kc]:
+                    for idd, fd in SPH[ld][kd]:
+                        coef = fc * fd
+                        v = knode(cc[ic], dd[idd])
+                        terms.append(v if coef == 1.0 else f'{lit(coef)} * {v}')
+                ket_sph[(e, kc, kd)] = hrr.new(' + '.join(terms)) if len(terms) > 1 or '*' in terms[0] else terms[0]
+
+    # ---- bra HRR + bra cart->sph, for every ket spherical pair
+    for kc in range(nsph(lc)):
+        for kd in range(nsph(ld)):
+            memo = {}
+
+            def bnode(a, b):
+                key = (a, b)
+                if key in memo:
+                    return memo[key]
+                if sum(b) == 0:
+                    memo[key] = ket_sph[(a, kc, kd)]
+                    return memo[key]
+                i = [k for k in range(3) if b[k] > 0][0]
+                memo[key] = hrr.new(f'{bnode(add(a, i, 1), add(b, i, -1))} + AB{AX[i]} * {bnode(a, add(b, i, -1))}')
+                return memo[key]
+
+            ca, cb = cart_list(la), cart_list(lb)
+            for ka in range(nsph(la)):
+                for kb in range(nsph(lb)):
+                    terms = []
+                    for ia, fa in SPH[la][ka]:
+                        for ib, fb in SPH[lb][kb]:
+                            coef = fa * fb
+                            v = bnode(ca[ia], cb[ib])
+                            terms.append(v if coef == 1.0 else f'{lit(coef)} * {v}')
+                    idx = ((ka * nsph(lb) + kb) * nsph(lc) + kc) * nsph(ld) + kd
+                    hrr.lines.append(f'out[{idx}] = {" + ".join(terms)};')
+    w('    (void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz;')
+    for ln in hrr.lines:
+        w('    ' + ln)
+    w('}')
+    return tag, '\n'.join(out), len(vrr.lines), len(hrr.lines)
+
+
+def pair_classes(lmax):
+    return [(a, b) for a in range(lmax + 1) for b in range(a + 1)]
+
+
+def class_pairs(lmax):
+    pcs = pair_classes(lmax)
+    return [(pcs[x], pcs[y]) for x in range(len(pcs)) for y in range(x + 1)]
+
+
+def main(outdir, lmax):
+    outdir = Path(outdir)
+    outdir.mkdir(parents=True, exist_ok=True)
+    pcs = pair_classes(lmax)
+    written = []
+    for (la, lb), (lc, ld) in class_pairs(lmax):
+        tag, body, nv, nh = gen_class(la, lb, lc, ld)
+        src = [
+            '// generated by gen_eri.py -- do not edit',
+            '#include "../eri_kernels.h"',
+            '',
+            body,
+            '',
+            f'VLX_INSTANTIATE_CLASS({la}, {lb}, {lc}, {ld})',
+        ]
+        if (la, lb) == (lc, ld):
+            src.append(f'VLX_INSTANTIATE_DIAG({la}, {lb})')
+        path = outdir / f'eri_{tag}.hip'
+        text = '\n'.join(src) + '\n'
+        if not path.exists() or path.read_text() != text:
+            path.write_text(text)
+        written.append((la, lb, lc, ld))
+    # dispatch table
+    d = ['// generated by gen_eri.py -- do not edit', f'#define VLX_GEN_LMAX {lmax}']
+    for (la, lb, lc, ld) in written:
+        d.append(f'VLX_DECLARE_CLASS({la}, {lb}, {lc}, {ld})')
+        if (la, lb) == (lc, ld):
+            d.append(f'VLX_DECLARE_DIAG({la}, {lb})')
+    d.append('static const ClassLaunchers g_class_table[] = {')
+    for (la, lb, lc, ld) in written:
+        x = pcs.index((la, lb))
+        y = pcs.index((lc, ld))
+        d.append(f'    {{{x}, {y}, {la}, {lb}, {lc}, {ld}, &launch_fock_{la}{lb}{lc}{ld}, &launch_eri_{la}{lb}{lc}{ld}}},')
+    d.append('};')
+    d.append('static const DiagLauncher g_diag_table[] = {')
+    for (la, lb) in pcs:
+        d.append(f'    {{{pcs.index((la, lb))}, {la}, {lb}, &launch_diag_{la}{lb}}},')
+    d.append('};')
+    text = '\n'.join(d) + '\n'
+    path = outdir / 'dispatch.inc'
+    if not path.exists() or path.read_text() != text:
+        path.write_text(text)
+    print(f'generated {len(written)} classes up to L={lmax} in {outdir}')
+
+
+if __name__ == '__main__':
+    main(sys.argv[1], int(sys.argv[2]))

@@ -1,0 +1,680 @@
+// Host driver and C ABI of the MI355X two-electron Fock build.
+//
+// Replaces CFockDriver::compute (src/t4c_func/FockDriver.cpp:120-289),
+// CT4CScreener::partition (src/t4c_func/T4CScreener.cpp:63-118) and the pair
+// data model (src/orbdata/GtoPairBlock.cpp:49-305) with a GPU pipeline:
+//
+//   screener: all shell pairs -> PairInfo/PrimPair in HBM -> diag_kernel
+//             (max over components of (ab|ab)) -> bins -> kept pairs sorted by
+//             (bin, primitive count) per pair class;
+//   build:    F' = 0 -> one fock_kernel launch per class pair (bra class >=
+//             ket class) -> G = F' + F'^T.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/vlxfock.h"
+#include "vlx_types.h"
+
+#include "gen/dispatch.inc"
+
+namespace {
+
+thread_local std::string g_last_error = "";
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t err_ = (expr);                                                                \
+        if (err_ != hipSuccess) return fail(VLX_ERR_HIP, "%s: %s", #expr, hipGetErrorString(err_)); \
+    } while (0)
+
+constexpr int NBINS = 16;
+
+int npair_classes() { return (VLX_GEN_LMAX + 1) * (VLX_GEN_LMAX + 2) / 2; }
+
+int pair_class_index(int la, int lb)  // la >= lb; order (0,0),(1,0),(1,1),(2,0),...
+{
+    return la * (la + 1) / 2 + lb;
+}
+
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), n(o.n)
+    {
+        o.ptr = nullptr;
+        o.n = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            ptr = o.ptr;
+            n = o.n;
+            o.ptr = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count)
+    {
+        if (count <= n && ptr) return hipSuccess;
+        release();
+        n = count;
+        return hipMalloc(&ptr, std::max<size_t>(count, 1) * sizeof(T));
+    }
+    hipError_t upload(const std::vector<T>& v)
+    {
+        hipError_t e = alloc(v.size());
+        if (e != hipSuccess) return e;
+        if (v.empty()) return hipSuccess;
+        return hipMemcpy(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    }
+};
+
+}  // namespace
+
+struct PairSet {
+    int la = 0, lb = 0;
+    std::vector<PairInfo> pairs;
+    std::vector<PrimPair> prims;
+    std::vector<int> shell_a, shell_b;  // shells of each pair
+    std::vector<int> bins;
+    int cnt_le[NBINS] = {0};            // pairs with bin <= b
+    std::vector<long long> nprim_prefix;
+    DevBuf<PairInfo> d_pairs;
+    DevBuf<PrimPair> d_prims;
+    int size() const { return static_cast<int>(pairs.size()); }
+};
+
+struct ClassSchedule {
+    int entry = -1;  // index into g_class_table
+    std::vector<int> nket;
+    std::vector<long long> tile_prefix;
+    long long quartets = 0, prim_quartets = 0;
+    DevBuf<int> d_nket;
+    DevBuf<long long> d_tile_prefix;
+};
+
+struct vlx_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf<double> d_boys;
+    DevBuf<double> d_density, d_fock, d_faccum;
+    int block_size_factor = 8;
+};
+
+struct vlx_screener {
+    vlx_context* ctx = nullptr;
+    int nshells = 0, nao = 0;
+    std::vector<int> l, nprim, prim_offset, ao_base, ao_stride;
+    std::vector<double> exps, norms, centers;
+    std::vector<PairSet> all;       // every pair, per pair class
+    std::vector<PairSet> kept;      // binned pairs sorted by (bin, nprim)
+    std::vector<double> pair_max;   // nshells x nshells
+    struct PairRef {
+        int cls = -1, idx = -1, swapped = 0;
+    };
+    std::vector<PairRef> pair_ref;  // nshells x nshells -> position in all[]
+    std::map<int, std::vector<std::unique_ptr<ClassSchedule>>> schedules;  // by ithreshold
+};
+
+namespace {
+
+void build_prims(const vlx_screener& s, int sa, int sb, std::vector<PrimPair>& prims)
+{
+    const double* A = &s.centers[3 * sa];
+    const double* B = &s.centers[3 * sb];
+    const double ab2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
+    const double kfac = std::sqrt(2.0) * std::pow(M_PI, 1.25);
+    for (int i = 0; i < s.nprim[sa]; i++)
+        for (int j = 0; j < s.nprim[sb]; j++) {
+            const double a = s.exps[s.prim_offset[sa] + i], b = s.exps[s.prim_offset[sb] + j];
+            const double na = s.norms[s.prim_offset[sa] + i], nb = s.norms[s.prim_offset[sb] + j];
+            const double z = a + b;
+            PrimPair p;
+            p.zeta = z;
+            p.px = (a * A[0] + b * B[0]) / z;
+            p.py = (a * A[1] + b * B[1]) / z;
+            p.pz = (a * A[2] + b * B[2]) / z;
+            p.pax = p.px - A[0];
+            p.pay = p.py - A[1];
+            p.paz = p.pz - A[2];
+            p.k = kfac * na * nb * std::exp(-a * b / z * ab2) / z;
+            prims.push_back(p);
+        }
+}
+
+void add_pair(const vlx_screener& s, PairSet& ps, int sa, int sb)
+{
+    PairInfo pi{};
+    pi.ao_a = s.ao_base[sa];
+    pi.str_a = s.ao_stride[sa];
+    pi.ao_b = s.ao_base[sb];
+    pi.str_b = s.ao_stride[sb];
+    pi.prim_start = static_cast<int>(ps.prims.size());
+    pi.nprim = s.nprim[sa] * s.nprim[sb];
+    pi.same = (sa == sb) ? 1 : 0;
+    pi.bin = 0;
+    pi.abx = s.centers[3 * sa] - s.centers[3 * sb];
+    pi.aby = s.centers[3 * sa + 1] - s.centers[3 * sb + 1];
+    pi.abz = s.centers[3 * sa + 2] - s.centers[3 * sb + 2];
+    pi.pad = 0.0;
+    ps.pairs.push_back(pi);
+    ps.shell_a.push_back(sa);
+    ps.shell_b.push_back(sb);
+    build_prims(s, sa, sb, ps.prims);
+}
+
+int pair_bin(double maxval)
+{
+    // src/orbdata/BlockedGtoPairBlock.cpp:32-42
+    const double fact = std::sqrt(maxval);
+    if (fact > 1.0e-15) return (fact < 1.0) ? static_cast<int>(-std::floor(std::log10(fact))) : 0;
+    return -1;
+}
+
+int upload_set(PairSet& ps)
+{
+    HIP_TRY(ps.d_pairs.upload(ps.pairs));
+    HIP_TRY(ps.d_prims.upload(ps.prims));
+    return VLX_OK;
+}
+
+const DiagLauncher* find_diag(int x)
+{
+    for (const auto& d : g_diag_table)
+        if (d.x == x) return &d;
+    return nullptr;
+}
+
+int label_factors(const char* label, double xf, double* fj, double* fk)
+{
+    // src/t4c_func/T4CLocalDistributor.cpp: 2jk (4, -1), 2jkx (4, -xf),
+    // j (2, 0), k (0, 1), kx (0, xf); then G = F' + F'^T.
+    if (!label) return fail(VLX_ERR_ARG, "label is NULL");
+    const std::string s(label);
+    if (s == "2jk") { *fj = 4.0; *fk = -1.0; }
+    else if (s == "2jkx") { *fj = 4.0; *fk = -xf; }
+    else if (s == "j") { *fj = 2.0; *fk = 0.0; }
+    else if (s == "k") { *fj = 0.0; *fk = 1.0; }
+    else if (s == "kx") { *fj = 0.0; *fk = xf; }
+    else if (s == "j_rs" || s == "k_rs" || s == "kx_rs")
+        return fail(VLX_ERR_UNSUPPORTED, "range-separated label '%s' is not built", label);
+    else
+        return fail(VLX_ERR_ARG, "unknown Fock label '%s'", label);
+    return VLX_OK;
+}
+
+__global__ void zero_kernel(double* __restrict__ p, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// G = F' + F'^T
+__global__ void symmetrize_kernel(const double* __restrict__ f, double* __restrict__ g, int n)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (q < n) g[(size_t)p * n + q] = f[(size_t)p * n + q] + f[(size_t)q * n + p];
+}
+
+int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSchedule>>** out)
+{
+    auto it = s->schedules.find(ithr);
+    if (it != s->schedules.end()) {
+        *out = &it->second;
+        return VLX_OK;
+    }
+    std::vector<std::unique_ptr<ClassSchedule>> list;
+    const int nentries = static_cast<int>(sizeof(g_class_table) / sizeof(g_class_table[0]));
+    const bool unscreened = ithr < 0;
+    for (int e = 0; e < nentries; e++) {
+        const auto& ce = g_class_table[e];
+        const PairSet& bra = unscreened ? s->all[ce.x] : s->kept[ce.x];
+        const PairSet& ket = unscreened ? s->all[ce.y] : s->kept[ce.y];
+        if (bra.size() == 0 || ket.size() == 0) continue;
+        auto sch = std::make_unique<ClassSchedule>();
+        sch->entry = e;
+        const int nb = bra.size();
+        sch->nket.resize(nb);
+        sch->tile_prefix.resize(nb + 1);
+        sch->tile_prefix[0] = 0;
+        for (int i = 0; i < nb; i++) {
+            int nk;
+            if (unscreened) {
+                nk = ket.size();
+            } else {
+                const int lim = ithr - bra.bins[i];
+                nk = (lim < 0) ? 0 : ket.cnt_le[std::min(lim, NBINS - 1)];
+            }
+            if (ce.x == ce.y) nk = std::min(nk, i + 1);
+            sch->nket[i] = nk;
+            sch->tile_prefix[i + 1] = sch->tile_prefix[i] + (nk + 63) / 64;
+            sch->quartets += nk;
+            sch->prim_quartets += (long long)bra.pairs[i].nprim * ket.nprim_prefix[nk];
+        }
+        if (sch->tile_prefix[nb] == 0) continue;
+        HIP_TRY(sch->d_nket.upload(sch->nket));
+        HIP_TRY(sch->d_tile_prefix.upload(sch->tile_prefix));
+        list.push_back(std::move(sch));
+    }
+    auto res = s->schedules.emplace(ithr, std::move(list));
+    *out = &res.first->second;
+    return VLX_OK;
+}
+
+int check_symmetric_host(const double* d, int n)
+{
+    for (int p = 0; p < n; p++)
+        for (int q = p + 1; q < n; q++) {
+            const double a = d[(size_t)p * n + q], b = d[(size_t)q * n + p];
+            if (std::fabs(a - b) > 1.0e-12 * std::max(1.0, std::max(std::fabs(a), std::fabs(b))))
+                return fail(VLX_ERR_ARG, "density is not symmetric at (%d, %d): general densities are not built", p, q);
+        }
+    return VLX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vlx_last_error(void) { return g_last_error.c_str(); }
+
+const char* vlx_version(void) { return "veloxchem_amd 0.1 (gfx950)"; }
+
+int vlx_max_angular_momentum(void) { return VLX_GEN_LMAX; }
+
+int vlx_context_create(int device, vlx_context** out)
+{
+    if (!out) return fail(VLX_ERR_ARG, "out is NULL");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(VLX_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
+    auto ctx = std::make_unique<vlx_context>();
+    ctx->device = device;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    std::vector<double> tab((size_t)(VLX_MAX_LTOT + 1) * BOYS_TAB_SIZE);
+    vlx_build_boys_table(tab.data());
+    HIP_TRY(ctx->d_boys.upload(tab));
+    *out = ctx.release();
+    return VLX_OK;
+}
+
+void vlx_context_destroy(vlx_context* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int vlx_set_block_size_factor(vlx_context* ctx, int factor)
+{
+    if (!ctx) return fail(VLX_ERR_ARG, "context is NULL");
+    if (factor < 1 || factor > 128) return fail(VLX_ERR_ARG, "block size factor %d outside 1..128", factor);
+    ctx->block_size_factor = factor;
+    return VLX_OK;
+}
+
+int vlx_context_synchronize(vlx_context* ctx)
+{
+    if (!ctx) return fail(VLX_ERR_ARG, "context is NULL");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VLX_OK;
+}
+
+void* vlx_context_stream(vlx_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* nprim, const int* prim_offset,
+                        const double* exps, const double* norms, const double* centers, const int* ao_base,
+                        const int* ao_stride, int nao, vlx_screener** out)
+{
+    if (!ctx || !out || !l || !nprim || !prim_offset || !exps || !norms || !centers || !ao_base || !ao_stride)
+        return fail(VLX_ERR_ARG, "NULL argument to vlx_screener_create");
+    if (nshells <= 0 || nao <= 0) return fail(VLX_ERR_ARG, "empty basis");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto s = std::make_unique<vlx_screener>();
+    s->ctx = ctx;
+    s->nshells = nshells;
+    s->nao = nao;
+    int nprims_total = 0;
+    for (int i = 0; i < nshells; i++) {
+        if (l[i] < 0 || l[i] > VLX_GEN_LMAX)
+            return fail(VLX_ERR_UNSUPPORTED, "shell %d has L=%d; kernels are built up to L=%d", i, l[i], VLX_GEN_LMAX);
+        if (nprim[i] <= 0) return fail(VLX_ERR_ARG, "shell %d has no primitives", i);
+        const int stride = ao_stride[i];
+        if (ao_base[i] < 0 || stride <= 0 || ao_base[i] + 2 * l[i] * stride >= nao)
+            return fail(VLX_ERR_ARG, "shell %d AO indices outside 0..%d", i, nao - 1);
+        nprims_total = std::max(nprims_total, prim_offset[i] + nprim[i]);
+    }
+    s->l.assign(l, l + nshells);
+    s->nprim.assign(nprim, nprim + nshells);
+    s->prim_offset.assign(prim_offset, prim_offset + nshells);
+    s->ao_base.assign(ao_base, ao_base + nshells);
+    s->ao_stride.assign(ao_stride, ao_stride + nshells);
+    s->exps.assign(exps, exps + nprims_total);
+    s->norms.assign(norms, norms + nprims_total);
+    s->centers.assign(centers, centers + 3 * nshells);
+
+    const int npc = npair_classes();
+    s->all.resize(npc);
+    s->kept.resize(npc);
+    for (int la = 0; la <= VLX_GEN_LMAX; la++)
+        for (int lb = 0; lb <= la; lb++) {
+            s->all[pair_class_index(la, lb)].la = la;
+            s->all[pair_class_index(la, lb)].lb = lb;
+            s->kept[pair_class_index(la, lb)].la = la;
+            s->kept[pair_class_index(la, lb)].lb = lb;
+        }
+    s->pair_ref.assign((size_t)nshells * nshells, {});
+    // every unordered shell pair once, oriented l_A >= l_B (A >= B by index on ties)
+    for (int sa = 0; sa < nshells; sa++)
+        for (int sb = 0; sb < nshells; sb++) {
+            const bool keep = (l[sa] > l[sb]) || (l[sa] == l[sb] && sa >= sb);
+            if (!keep) continue;
+            const int x = pair_class_index(l[sa], l[sb]);
+            PairSet& ps = s->all[x];
+            const int idx = ps.size();
+            add_pair(*s, ps, sa, sb);
+            s->pair_ref[(size_t)sa * nshells + sb] = {x, idx, 0};
+            s->pair_ref[(size_t)sb * nshells + sa] = {x, idx, 1};
+        }
+    // diagonal integrals on the GPU
+    s->pair_max.assign((size_t)nshells * nshells, 0.0);
+    for (int x = 0; x < npc; x++) {
+        PairSet& ps = s->all[x];
+        if (ps.size() == 0) continue;
+        int rc = upload_set(ps);
+        if (rc) return rc;
+        const DiagLauncher* dl = find_diag(x);
+        if (!dl) return fail(VLX_ERR_UNSUPPORTED, "no diagonal kernel for pair class %d", x);
+        DevBuf<double> d_max;
+        HIP_TRY(d_max.alloc(ps.size()));
+        DiagArgs da{ps.d_pairs.ptr, ps.d_prims.ptr, ps.size(),
+                     ctx->d_boys.ptr + (size_t)(2 * (ps.la + ps.lb)) * BOYS_TAB_SIZE, d_max.ptr};
+        dl->diag(da, ctx->stream);
+        HIP_TRY(hipGetLastError());
+        std::vector<double> mx(ps.size());
+        HIP_TRY(hipMemcpyAsync(mx.data(), d_max.ptr, mx.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        ps.bins.resize(ps.size());
+        for (int i = 0; i < ps.size(); i++) {
+            const int sa = ps.shell_a[i], sb = ps.shell_b[i];
+            s->pair_max[(size_t)sa * nshells + sb] = mx[i];
+            s->pair_max[(size_t)sb * nshells + sa] = mx[i];
+            ps.bins[i] = pair_bin(mx[i]);
+            ps.pairs[i].bin = ps.bins[i];
+        }
+        // kept set: bin >= 0, sorted by (bin, nprim)
+        std::vector<int> order;
+        for (int i = 0; i < ps.size(); i++)
+            if (ps.bins[i] >= 0) order.push_back(i);
+        std::stable_sort(order.begin(), order.end(), [&](int i, int j) {
+            if (ps.bins[i] != ps.bins[j]) return ps.bins[i] < ps.bins[j];
+            return ps.pairs[i].nprim < ps.pairs[j].nprim;
+        });
+        PairSet& ks = s->kept[x];
+        for (int i : order) add_pair(*s, ks, ps.shell_a[i], ps.shell_b[i]);
+        ks.bins.resize(ks.size());
+        for (int k = 0; k < ks.size(); k++) {
+            ks.bins[k] = ps.bins[order[k]];
+            ks.pairs[k].bin = ks.bins[k];
+        }
+        for (int b = 0; b < NBINS; b++) ks.cnt_le[b] = 0;
+        for (int k = 0; k < ks.size(); k++)
+            for (int b = ks.bins[k]; b < NBINS; b++) ks.cnt_le[b]++;
+        ks.nprim_prefix.assign(ks.size() + 1, 0);
+        for (int k = 0; k < ks.size(); k++) ks.nprim_prefix[k + 1] = ks.nprim_prefix[k] + ks.pairs[k].nprim;
+        ps.nprim_prefix.assign(ps.size() + 1, 0);
+        for (int k = 0; k < ps.size(); k++) ps.nprim_prefix[k + 1] = ps.nprim_prefix[k] + ps.pairs[k].nprim;
+        for (int b = 0; b < NBINS; b++) ps.cnt_le[b] = ps.size();
+        rc = upload_set(ks);
+        if (rc) return rc;
+    }
+    *out = s.release();
+    return VLX_OK;
+}
+
+void vlx_screener_destroy(vlx_screener* scr)
+{
+    if (!scr) return;
+    (void)hipSetDevice(scr->ctx->device);
+    (void)hipStreamSynchronize(scr->ctx->stream);
+    delete scr;
+}
+
+int vlx_screener_pair_maxima(const vlx_screener* scr, double* out)
+{
+    if (!scr || !out) return fail(VLX_ERR_ARG, "NULL argument");
+    std::copy(scr->pair_max.begin(), scr->pair_max.end(), out);
+    return VLX_OK;
+}
+
+int vlx_screener_number_of_pairs(const vlx_screener* scr, long long* out)
+{
+    if (!scr || !out) return fail(VLX_ERR_ARG, "NULL argument");
+    long long n = 0;
+    for (const auto& ks : scr->kept) n += ks.size();
+    *out = n;
+    return VLX_OK;
+}
+
+int vlx_fock_work(vlx_screener* scr, int ithreshold, int max_records, int* nrecords, int* classes,
+                  long long* quartets, long long* prim_quartets)
+{
+    if (!scr || !nrecords) return fail(VLX_ERR_ARG, "NULL argument");
+    if (ithreshold > 30) return fail(VLX_ERR_ARG, "ithreshold %d outside the accepted range (< 0 or 0..30)", ithreshold);
+    std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
+    int rc = get_schedules(scr, ithreshold, &list);
+    if (rc) return rc;
+    *nrecords = static_cast<int>(list->size());
+    for (int r = 0; r < std::min(max_records, *nrecords); r++) {
+        const auto& sch = *(*list)[r];
+        const auto& ce = g_class_table[sch.entry];
+        if (classes) {
+            classes[4 * r] = ce.la;
+            classes[4 * r + 1] = ce.lb;
+            classes[4 * r + 2] = ce.lc;
+            classes[4 * r + 3] = ce.ld;
+        }
+        if (quartets) quartets[r] = sch.quartets;
+        if (prim_quartets) prim_quartets[r] = sch.prim_quartets;
+    }
+    return VLX_OK;
+}
+
+int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,
+                            double exchange_factor, double omega, int ithreshold, double* d_fock_out)
+{
+    if (!ctx || !scr || !d_density || !d_fock_out) return fail(VLX_ERR_ARG, "NULL argument to vlx_fock_compute_device");
+    if (scr->ctx != ctx) return fail(VLX_ERR_ARG, "screener belongs to another context");
+    if (omega != 0.0) return fail(VLX_ERR_UNSUPPORTED, "omega != 0 (erf-attenuated operator) is not built");
+    if (ithreshold > 30) return fail(VLX_ERR_ARG, "ithreshold %d outside the accepted range (< 0 or 0..30)", ithreshold);
+    double fj = 0.0, fk = 0.0;
+    int rc = label_factors(label, exchange_factor, &fj, &fk);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int n = scr->nao;
+    const size_t nn = (size_t)n * n;
+    HIP_TRY(ctx->d_faccum.alloc(nn));
+    zero_kernel<<<std::min<size_t>((nn + 255) / 256, 4096), 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, nn);
+    HIP_TRY(hipGetLastError());
+    std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
+    rc = get_schedules(scr, ithreshold, &list);
+    if (rc) return rc;
+    const bool unscreened = ithreshold < 0;
+    for (const auto& schp : *list) {
+        const ClassSchedule& sch = *schp;
+        const auto& ce = g_class_table[sch.entry];
+        const PairSet& bra = unscreened ? scr->all[ce.x] : scr->kept[ce.x];
+        const PairSet& ket = unscreened ? scr->all[ce.y] : scr->kept[ce.y];
+        FockArgs a{};
+        a.bra = bra.d_pairs.ptr;
+        a.bra_prims = bra.d_prims.ptr;
+        a.ket = ket.d_pairs.ptr;
+        a.ket_prims = ket.d_prims.ptr;
+        a.tile_prefix = sch.d_tile_prefix.ptr;
+        a.nket = sch.d_nket.ptr;
+        a.total_tiles = sch.tile_prefix.back();
+        a.nbra = bra.size();
+        a.same_class = (ce.x == ce.y) ? 1 : 0;
+        a.boys_tab = ctx->d_boys.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;
+        a.D = d_density;
+        a.F = ctx->d_faccum.ptr;
+        a.nao = n;
+        a.fj = fj;
+        a.fk = fk;
+        const long long waves = std::min<long long>(a.total_tiles, 8192);
+        const int grid = static_cast<int>((waves + 3) / 4);
+        ce.fock(a, grid, ctx->stream);
+        HIP_TRY(hipGetLastError());
+    }
+    dim3 sg((n + 255) / 256, n);
+    symmetrize_kernel<<<sg, 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, d_fock_out, n);
+    HIP_TRY(hipGetLastError());
+    return VLX_OK;
+}
+
+int vlx_fock_compute(vlx_context* ctx, vlx_screener* scr, const double* density, const char* label,
+                     double exchange_factor, double omega, int ithreshold, double* fock_out)
+{
+    if (!ctx || !scr || !density || !fock_out) return fail(VLX_ERR_ARG, "NULL argument to vlx_fock_compute");
+    const int n = scr->nao;
+    int rc = check_symmetric_host(density, n);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t nn = (size_t)n * n;
+    HIP_TRY(ctx->d_density.alloc(nn));
+    HIP_TRY(ctx->d_fock.alloc(nn));
+    HIP_TRY(hipMemcpyAsync(ctx->d_density.ptr, density, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    rc = vlx_fock_compute_device(ctx, scr, ctx->d_density.ptr, label, exchange_factor, omega, ithreshold,
+                                 ctx->d_fock.ptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(fock_out, ctx->d_fock.ptr, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VLX_OK;
+}
+
+int vlx_eri_quartets(vlx_context* ctx, vlx_screener* scr, int nquartets, const int* quartets, double* out)
+{
+    if (!ctx || !scr || !quartets || !out || nquartets < 0) return fail(VLX_ERR_ARG, "bad argument to vlx_eri_quartets");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int ns = scr->nshells;
+    struct Req {
+        int q;
+        int bra_x, bra_i, ket_x, ket_i;
+        int perm[4];  // requested position -> canonical position (0 A, 1 B, 2 C, 3 D)
+        size_t offset;
+    };
+    std::vector<Req> reqs(nquartets);
+    size_t off = 0;
+    for (int q = 0; q < nquartets; q++) {
+        const int* sh = &quartets[4 * q];
+        for (int k = 0; k < 4; k++)
+            if (sh[k] < 0 || sh[k] >= ns) return fail(VLX_ERR_ARG, "quartet %d: shell index out of range", q);
+        auto r1 = scr->pair_ref[(size_t)sh[0] * ns + sh[1]];
+        auto r2 = scr->pair_ref[(size_t)sh[2] * ns + sh[3]];
+        Req r{};
+        r.q = q;
+        r.offset = off;
+        int pos[4];  // canonical position of requested shells a, b, c, d
+        pos[0] = r1.swapped ? 1 : 0;
+        pos[1] = r1.swapped ? 0 : 1;
+        pos[2] = r2.swapped ? 3 : 2;
+        pos[3] = r2.swapped ? 2 : 3;
+        if (r1.cls >= r2.cls) {
+            r.bra_x = r1.cls; r.bra_i = r1.idx; r.ket_x = r2.cls; r.ket_i = r2.idx;
+        } else {
+            r.bra_x = r2.cls; r.bra_i = r2.idx; r.ket_x = r1.cls; r.ket_i = r1.idx;
+            for (int k = 0; k < 4; k++) pos[k] = (pos[k] + 2) % 4;
+        }
+        for (int k = 0; k < 4; k++) r.perm[k] = pos[k];
+        reqs[q] = r;
+        size_t sz = 1;
+        for (int k = 0; k < 4; k++) sz *= 2 * scr->l[sh[k]] + 1;
+        off += sz;
+    }
+    const int nentries = static_cast<int>(sizeof(g_class_table) / sizeof(g_class_table[0]));
+    for (int e = 0; e < nentries; e++) {
+        const auto& ce = g_class_table[e];
+        std::vector<int> sel;
+        std::vector<int2> list;
+        for (const auto& r : reqs)
+            if (r.bra_x == ce.x && r.ket_x == ce.y) {
+                sel.push_back(r.q);
+                list.push_back(make_int2(r.bra_i, r.ket_i));
+            }
+        if (list.empty()) continue;
+        const int nsph = (2 * ce.la + 1) * (2 * ce.lb + 1) * (2 * ce.lc + 1) * (2 * ce.ld + 1);
+        DevBuf<int2> d_list;
+        DevBuf<double> d_out;
+        HIP_TRY(d_list.upload(list));
+        HIP_TRY(d_out.alloc(list.size() * nsph));
+        const PairSet& bra = scr->all[ce.x];
+        const PairSet& ket = scr->all[ce.y];
+        EriArgs a{bra.d_pairs.ptr, bra.d_prims.ptr, ket.d_pairs.ptr, ket.d_prims.ptr, d_list.ptr,
+                  static_cast<int>(list.size()),
+                  ctx->d_boys.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE, d_out.ptr};
+        ce.eri(a, ctx->stream);
+        HIP_TRY(hipGetLastError());
+        std::vector<double> h(list.size() * nsph);
+        HIP_TRY(hipMemcpyAsync(h.data(), d_out.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        const int cdim[4] = {2 * ce.la + 1, 2 * ce.lb + 1, 2 * ce.lc + 1, 2 * ce.ld + 1};
+        for (size_t k = 0; k < sel.size(); k++) {
+            const Req& r = reqs[sel[k]];
+            int rdim[4];
+            for (int t = 0; t < 4; t++) rdim[t] = cdim[r.perm[t]];
+            const double* src = &h[k * nsph];
+            double* dst = out + r.offset;
+            int idx[4];
+            for (idx[0] = 0; idx[0] < rdim[0]; idx[0]++)
+                for (idx[1] = 0; idx[1] < rdim[1]; idx[1]++)
+                    for (idx[2] = 0; idx[2] < rdim[2]; idx[2]++)
+                        for (idx[3] = 0; idx[3] < rdim[3]; idx[3]++) {
+                            int c[4];
+                            for (int t = 0; t < 4; t++) c[r.perm[t]] = idx[t];
+                            dst[((idx[0] * rdim[1] + idx[1]) * rdim[2] + idx[2]) * rdim[3] + idx[3]] =
+                                src[((c[0] * cdim[1] + c[1]) * cdim[2] + c[2]) * cdim[3] + c[3]];
+                        }
+        }
+    }
+    return VLX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// Data layout shared by the host driver and the gfx950 kernels.
+//
+// Shell pairs of one pair class (la >= lb) live in HBM as an array of
+// PairInfo (48 B) plus an array of PrimPair (one 64 B line per primitive
+// pair, so a lane's primitive data is a single cache-line read).  Pairs of a
+// class are sorted by (Schwarz bin, primitive count) so that the kets a bra
+// pair meets under bin_bra + bin_ket <= ithreshold form a prefix.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "vlx_boys.h"
+
+// One primitive pair (a on centre A, b on centre B), A = first shell of the pair.
+struct alignas(64) PrimPair {
+    double zeta;          // a + b
+    double px, py, pz;    // P = (a A + b B) / zeta
+    double pax, pay, paz; // P - A
+    double k;             // sqrt(2) pi^(5/4) N_a N_b exp(-a b |AB|^2 / zeta) / zeta
+};
+
+// One contracted shell pair (A, B) with l_A >= l_B.
+struct alignas(16) PairInfo {
+    int ao_a, str_a;  // AO index of component 0 of A and stride between components
+    int ao_b, str_b;
+    int prim_start;   // first PrimPair of this pair
+    int nprim;        // number of primitive pairs
+    int same;         // 1 when A and B are the same shell
+    int bin;          // Schwarz bin (0..15)
+    double abx, aby, abz, pad;  // A - B
+};
+
+struct FockArgs {
+    const PairInfo* bra;
+    const PrimPair* bra_prims;
+    const PairInfo* ket;
+    const PrimPair* ket_prims;
+    const long long* tile_prefix;  // nbra + 1 entries
+    const int* nket;               // kets of each bra pair
+    long long total_tiles;
+    int nbra;
+    int same_class;                // bra and ket come from the same pair list
+    const double* boys_tab;        // BOYS_TAB_SIZE doubles for L = la+lb+lc+ld
+    const double* D;               // nao x nao, row-major
+    double* F;                     // nao x nao accumulator F' (G = F' + F'^T)
+    int nao;
+    double fj;                     // coefficient of the Coulomb terms (0: none)
+    double fk;                     // coefficient of the exchange terms (0: none)
+};
+
+struct EriArgs {
+    const PairInfo* bra;
+    const PrimPair* bra_prims;
+    const PairInfo* ket;
+    const PrimPair* ket_prims;
+    const int2* quartets;  // (bra pair, ket pair)
+    int nquartets;
+    const double* boys_tab;
+    double* out;           // nquartets x nsph
+};
+
+struct DiagArgs {
+    const PairInfo* pairs;
+    const PrimPair* prims;
+    int npairs;
+    const double* boys_tab;
+    double* maxval;
+};
+
+using FockLaunchFn = void (*)(const FockArgs&, int, hipStream_t);
+using EriLaunchFn = void (*)(const EriArgs&, hipStream_t);
+using DiagLaunchFn = void (*)(const DiagArgs&, hipStream_t);
+
+struct ClassLaunchers {
+    int x, y;             // bra / ket pair-class indices (x >= y)
+    int la, lb, lc, ld;
+    FockLaunchFn fock;
+    EriLaunchFn eri;
+};
+
+struct DiagLauncher {
+    int x;
+    int la, lb;
+    DiagLaunchFn diag;
+};
+
+#define VLX_DECLARE_CLASS(la, lb, lc, ld)                                        \
+    void launch_fock_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s); \
+    void launch_eri_##la##lb##lc##ld(const EriArgs& a, hipStream_t s);
+#define VLX_DECLARE_DIAG(la, lb) void launch_diag_##la##lb(const DiagArgs& a, hipStream_t s);

@@ -1,0 +1,94 @@
+"""T4CScreener: Schwarz partition of shell pairs, computed on the GPU.
+
+Mirrors ``CT4CScreener`` (src/t4c_func/T4CScreener.hpp, .cpp:63-118;
+exported at src/python/ExportT4CIntegrals.cpp:127-140).  ``partition`` builds
+every shell pair, evaluates max over components of (ab|ab) with the gfx950
+kernels, bins the pairs by -floor(log10 sqrt(max)) and drops those with
+sqrt <= 1e-15 (src/orbdata/BlockedGtoPairBlock.cpp:32-42).
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+
+class T4CScreener:
+
+    def __init__(self):
+        self._handle = None
+        self._nshells = 0
+        self._nao = 0
+        self._device = _native.current_device()
+        self._key = None
+
+    def partition(self, basis, molecule, label='eri'):
+        if label != 'eri':
+            raise ValueError(f"T4CScreener.partition: label {label!r} is not supported (only 'eri')")
+        self._release()
+        sh = basis.shell_arrays(molecule.get_coordinates_in_bohr())
+        self._keep = {}
+        ptrs = {}
+        for k in ('l', 'nprim', 'prim_offset', 'ao_base', 'ao_stride'):
+            self._keep[k], ptrs[k] = _native.as_int(sh[k])
+        for k in ('exps', 'norms', 'centers'):
+            self._keep[k], ptrs[k] = _native.as_double(sh[k])
+        ctx = _native.context(self._device)
+        h = ctypes.c_void_p()
+        nao = basis.get_dimensions_of_basis()
+        _native.check(_native.lib().vlx_screener_create(
+            ctx, len(sh['l']), ptrs['l'], ptrs['nprim'], ptrs['prim_offset'], ptrs['exps'], ptrs['norms'],
+            ptrs['centers'], ptrs['ao_base'], ptrs['ao_stride'], nao, ctypes.byref(h)))
+        self._handle = h
+        self._nshells = len(sh['l'])
+        self._nao = nao
+        self._key = (basis.get_label(), np.asarray(molecule.get_coordinates_in_bohr()).tobytes())
+        return self
+
+    def _release(self):
+        if self._handle is not None and _native._lib is not None:
+            _native.lib().vlx_screener_destroy(self._handle)
+        self._handle = None
+
+    def __del__(self):
+        try:
+            self._release()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if self._handle is None:
+            raise RuntimeError('T4CScreener: call partition(basis, molecule, "eri") first')
+        return self._handle
+
+    def number_of_ao(self):
+        return self._nao
+
+    def pair_maxima(self):
+        """nshells x nshells matrix of max over components of (ab|ab)."""
+        out = np.zeros((self._nshells, self._nshells))
+        _native.check(_native.lib().vlx_screener_pair_maxima(self.handle, out.ctypes.data_as(_native._c_double_p)))
+        return out
+
+    def number_of_pairs(self):
+        n = ctypes.c_longlong()
+        _native.check(_native.lib().vlx_screener_number_of_pairs(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def work(self, ithreshold):
+        """Per-class (la, lb, lc, ld) contracted and primitive quartet counts of one build."""
+        L = _native.lib()
+        n = ctypes.c_int()
+        _native.check(L.vlx_fock_work(self.handle, int(ithreshold), 0, ctypes.byref(n), None, None, None))
+        cls = np.zeros(4 * n.value, dtype=np.int32)
+        nq = np.zeros(n.value, dtype=np.int64)
+        npq = np.zeros(n.value, dtype=np.int64)
+        _native.check(L.vlx_fock_work(self.handle, int(ithreshold), n.value, ctypes.byref(n),
+                                      cls.ctypes.data_as(_native._c_int_p), nq.ctypes.data_as(_native._c_ll_p),
+                                      npq.ctypes.data_as(_native._c_ll_p)))
+        return [(tuple(int(x) for x in cls[4 * i:4 * i + 4]), int(nq[i]), int(npq[i])) for i in range(n.value)]
+
+    def __eq__(self, other):
+        return isinstance(other, T4CScreener) and self._key == other._key
