@@ -1,0 +1,171 @@
+"""Benchmark of the two-electron J+K Fock build on one MI355X.
+
+Workload (one "step" = one build): seeded 32-water cluster in a 10 Angstrom
+box (seed 32), def2-SVP (768 AOs), FP64 G = 2J - K ('2jk', exchange factor
+1, omega 0, ithreshold 12).  The density is a seeded symmetric positive
+semi-definite pseudo-density (n_occ = 160); with density-independent Schwarz
+screening the work of a build does not depend on D.  D and G stay resident
+in HBM; the timed region is K builds between stream synchronisations.
+
+Printed: one JSON line with value = builds/s, the roofline of the dominant
+class kernel (FP64 vector-ALU bound; per-launch algorithmic FLOPs over the
+HIP-event kernel time measured inside the timed region), and a CPU baseline
+(the project's C oracle on the host cores over a bounded strided sample of
+the same build, extrapolated to builds/s).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+FP64_VALU_PEAK_TFLOPS = 78.6
+
+
+def build_workload(nmol, box, seed, basis):
+    from veloxchem_amd import MolecularBasis, T4CScreener
+    from veloxchem_amd.workloads import pseudo_density, water_cluster
+    mol = water_cluster(nmol, box, seed)
+    bas = MolecularBasis.read(mol, basis)
+    t0 = time.perf_counter()
+    scr = T4CScreener()
+    scr.partition(bas, mol, 'eri')
+    t_screen = time.perf_counter() - t0
+    n = bas.get_dimensions_of_basis()
+    dm = pseudo_density(n, 5 * nmol, seed=3)
+    return mol, bas, scr, dm, t_screen
+
+
+def cpu_baseline(mol, bas, dm, ithr, target_s):
+    """Oracle on all host cores over bra pairs ip % stride == 0, extrapolated."""
+    from oracle import oracle as orc
+    ob = orc.basis_from(bas, mol)
+    cores = min(os.cpu_count() or 1, 16)
+    _, bins = orc.pair_maxima(ob)  # screening set-up, not timed
+    # calibrate with a coarse sample, then size the sample to ~target_s
+    stride = 1024
+    t0 = time.perf_counter()
+    nq, total = orc.fock_sampled(ob, dm, '2jk', 1.0, ithr, cores, stride, bins)
+    dt = time.perf_counter() - t0
+    if dt < target_s / 4 and nq > 0:
+        stride = max(1, int(stride * dt / target_s))
+        t0 = time.perf_counter()
+        nq, total = orc.fock_sampled(ob, dm, '2jk', 1.0, ithr, cores, stride, bins)
+        dt = time.perf_counter() - t0
+    builds_per_s = (nq / total) / dt if nq else 0.0
+    return {'value': builds_per_s, 'unit': 'builds/s', 'cores': cores, 'kind': 'port',
+            'sample': f'oracle 2jk over every {stride}-th bra shell pair ({nq} of {total} shell quartets, '
+                      f'{dt:.1f} s), extrapolated by quartet count'}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--nmol', type=int, default=32)
+    ap.add_argument('--box', type=float, default=10.0)
+    ap.add_argument('--seed', type=int, default=32)
+    ap.add_argument('--basis', default='def2-svp')
+    ap.add_argument('--ithreshold', type=int, default=12)
+    ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--label', default='2jk', choices=['2jk', 'j', 'k'])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py: this build runs on one GPU; the multi-GPU sharding is design only '
+              '(one process per GPU over RCCL is not built). Use --gpus 1.', file=sys.stderr)
+        sys.exit(2)
+
+    from veloxchem_amd import _native
+    lib = _native.lib()
+    ctx = _native.context(0)
+    mol, bas, scr, dm, t_screen = build_workload(args.nmol, args.box, args.seed, args.basis)
+    n = bas.get_dimensions_of_basis()
+    d_dm = _native.DeviceArray((n, n))
+    d_g = _native.DeviceArray((n, n))
+    d_dm.upload(dm)
+    label = args.label.encode()
+
+    def one_build():
+        _native.check(lib.vlx_fock_compute_device(ctx, scr.handle, d_dm.ptr, label, 1.0, 0.0, args.ithreshold,
+                                                  d_g.ptr))
+
+    for _ in range(args.warmup):
+        one_build()
+    _native.check(lib.vlx_context_synchronize(ctx))
+    _native.check(lib.vlx_context_set_timing(ctx, 1))
+    _native.check(lib.vlx_context_reset_times(ctx))
+    _native.check(lib.vlx_context_synchronize(ctx))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one_build()
+    _native.check(lib.vlx_context_synchronize(ctx))
+    t1 = time.perf_counter()
+    _native.check(lib.vlx_context_set_timing(ctx, 0))
+    times = _native.class_times(0)
+    g = d_g.download()
+
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    work = scr.work(args.ithreshold)
+    flops_build = sum(w[3] for w in work)
+    by_class = {w[0]: w for w in work}
+    dom = max(times, key=lambda c: times[c][0])
+    dom_ms = times[dom][0] / times[dom][1]
+    dom_flops = by_class[dom][3]
+    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    build_tflops = flops_build / (ms_per_step * 1e-3) / 1e12
+
+    result = {
+        'metric': 'J+K Fock builds/s (FP64, 2jk)',
+        'value': 1e3 / ms_per_step,
+        'unit': 'builds/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms_per_step,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic (seeded water cluster, seeded pseudo-density)',
+        'config': {
+            'workload': f'{args.nmol}-water cluster (seed {args.seed}, {args.box} A box), {args.basis}, '
+                        f'{n} AOs, 2jk FP64 build, ithreshold {args.ithreshold}',
+            'nao': n,
+            'shell_quartets': int(sum(w[1] for w in work)),
+            'primitive_quartets': int(sum(w[2] for w in work)),
+            'screener_setup_s': round(t_screen, 3),
+        },
+        'roofline': {
+            'bound': 'valu',
+            'kernel': 'fock_kernel<%d,%d,%d,%d>' % dom,
+            'achieved': achieved,
+            'peak': FP64_VALU_PEAK_TFLOPS,
+            'unit': 'TFLOP/s',
+            'frac': achieved / FP64_VALU_PEAK_TFLOPS,
+            'traffic': None,
+            'kernel_ms': dom_ms,
+            'kernel_share_of_build': times[dom][0] / sum(v[0] for v in times.values()),
+            'build_achieved': build_tflops,
+            'build_frac': build_tflops / FP64_VALU_PEAK_TFLOPS,
+        },
+        'class_ms': {'%d%d%d%d' % c: round(v[0] / v[1], 4) for c, v in sorted(times.items())},
+        'checksum': float(np.sum(g * dm)),
+    }
+    if not args.no_cpu_baseline:
+        result['cpu_baseline'] = cpu_baseline(mol, bas, dm, args.ithreshold, args.cpu_seconds)
+    d_dm.free()
+    d_g.free()
+    print(json.dumps(result))
+
+
+if __name__ == '__main__':
+    main()

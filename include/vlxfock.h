@@ -86,6 +86,13 @@ int vlx_fock_compute(vlx_context* ctx, vlx_screener* scr, const double* density,
 int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,
                             double exchange_factor, double omega, int ithreshold, double* d_fock_out);
 
+/* Device memory helpers (HBM of the context's device) for callers that keep
+ * D and G resident between builds. */
+int vlx_device_malloc(vlx_context* ctx, unsigned long long bytes, void** out);
+int vlx_device_free(vlx_context* ctx, void* ptr);
+int vlx_copy_to_device(vlx_context* ctx, void* dst, const void* src, unsigned long long bytes);
+int vlx_copy_to_host(vlx_context* ctx, void* dst, const void* src, unsigned long long bytes);
+
 /* Synchronise the context's stream. */
 int vlx_context_synchronize(vlx_context* ctx);
 
@@ -94,11 +101,24 @@ void* vlx_context_stream(vlx_context* ctx);
 
 /*
  * Work of one build at ithreshold: one record per angular class
- * (la, lb, lc, ld) with the number of contracted and primitive shell quartets.
+ * (la, lb, lc, ld) with the number of contracted and primitive shell quartets
+ * and the FP64 FLOPs of one 2jk build of that class:
+ *   prim_quartets (V + 53 + 4 L) + quartets (C + 12 n_sph),
+ * V = VRR + contraction FLOPs per primitive quartet and C = HRR + cart->sph
+ * FLOPs per contracted quartet of the generated recursion.
  * Writes at most max_records; *nrecords receives the number available.
  */
 int vlx_fock_work(vlx_screener* scr, int ithreshold, int max_records, int* nrecords, int* classes /* 4 per */,
-                  long long* quartets, long long* prim_quartets);
+                  long long* quartets, long long* prim_quartets, double* flops);
+
+/* Per-class kernel timing with HIP events on the context's stream
+ * (off by default).  vlx_context_class_times synchronises the stream and
+ * returns, per class (la, lb, lc, ld), the summed kernel time in ms and the
+ * number of launches since the last reset. */
+int vlx_context_set_timing(vlx_context* ctx, int enable);
+int vlx_context_reset_times(vlx_context* ctx);
+int vlx_context_class_times(vlx_context* ctx, int max_records, int* nrecords, int* classes /* 4 per */,
+                            double* total_ms, long long* launches);
 
 /*
  * Spherical integral blocks (ab|cd) for explicit shell quartets

@@ -422,6 +422,21 @@ static void digest(const oracle_basis_t *bs, int sa, int sb, int sc, int sd, con
     }
 }
 
+/* internal: build with an optional bra-pair stride (1 = everything) */
+static long fock_impl(const oracle_basis_t *bs, const double *dm, int label, double xf, int ithreshold, int nthreads,
+                      int stride, const int *bins_in, double *gout, long *total);
+
+/*
+ * Bounded CPU-baseline sample: only bra pairs ip with ip % stride == 0 are
+ * computed.  Returns the number of shell quartets computed; *total receives
+ * the number a full build computes.  gout holds the partial result.
+ */
+long oracle_fock_sampled(const oracle_basis_t *bs, const double *dm, int label, double xf, int ithreshold,
+                         int nthreads, int stride, const int *bins, double *gout, long *total)
+{
+    return fock_impl(bs, dm, label, xf, ithreshold, nthreads, stride, bins, gout, total);
+}
+
 /*
  * G = F' + F'^T for a symmetric density dm (nao x nao, reference AO order).
  * ithreshold < 0 computes every quartet (the reference's unscreened
@@ -432,11 +447,22 @@ static void digest(const oracle_basis_t *bs, int sa, int sb, int sc, int sd, con
 long oracle_fock(const oracle_basis_t *bs, const double *dm, int label, double xf, int ithreshold, int nthreads,
                  double *gout)
 {
+    long total = 0;
+    return fock_impl(bs, dm, label, xf, ithreshold, nthreads, 1, NULL, gout, &total);
+}
+
+/* bins_in: optional nshells x nshells pair bins from oracle_pair_bins */
+static long fock_impl(const oracle_basis_t *bs, const double *dm, int label, double xf, int ithreshold, int nthreads,
+                      int stride, const int *bins_in, double *gout, long *total)
+{
     const int ns = bs->nshells, n = bs->nao;
+    if (stride < 1) return -1;
     if (label < 0 || label > 4) return -1;
     double *maxv = malloc(sizeof(double) * ns * ns);
     int *bins = malloc(sizeof(int) * ns * ns);
-    if (ithreshold >= 0) {
+    if (ithreshold >= 0 && bins_in) {
+        memcpy(bins, bins_in, sizeof(int) * ns * ns);
+    } else if (ithreshold >= 0) {
         if (oracle_pair_bins(bs, maxv, bins)) { free(maxv); free(bins); return -1; }
     } else {
         for (int i = 0; i < ns * ns; i++) bins[i] = 0;
@@ -448,6 +474,13 @@ long oracle_fock(const oracle_basis_t *bs, const double *dm, int label, double x
         for (int sb = sa; sb < ns; sb++)
             if (bins[sa * ns + sb] >= 0) { pa[npairs] = sa; pb[npairs] = sb; npairs++; }
     long nquart = 0;
+    long ntotal = 0;
+    for (int ip = 0; ip < npairs; ip++) {
+        const int bi = bins[pa[ip] * ns + pb[ip]];
+        for (int jp = ip; jp < npairs; jp++)
+            if (ithreshold < 0 || bi + bins[pa[jp] * ns + pb[jp]] <= ithreshold) ntotal++;
+    }
+    *total = ntotal;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
@@ -458,6 +491,7 @@ long oracle_fock(const oracle_basis_t *bs, const double *dm, int label, double x
         double *buf = malloc(sizeof(double) * 7 * 7 * 7 * 7);
 #pragma omp for schedule(dynamic)
         for (int ip = 0; ip < npairs; ip++) {
+            if (ip % stride) continue;
             const int bi = bins[pa[ip] * ns + pb[ip]];
             for (int jp = ip; jp < npairs; jp++) {
                 const int bj = bins[pa[jp] * ns + pb[jp]];

@@ -122,5 +122,25 @@ def fock(ob, density, label='2jk', exchange_factor=1.0, ithreshold=15, nthreads=
     return g, int(nq)
 
 
+def fock_sampled(ob, density, label, exchange_factor, ithreshold, nthreads, stride, bins):
+    """Partial build over bra pairs ip % stride == 0 with precomputed pair bins
+    (from pair_maxima): (quartets computed, quartets in a full build)."""
+    n = ob.struct.nao
+    bins = np.ascontiguousarray(bins, dtype=np.int32)
+    dm = np.ascontiguousarray(density, dtype=np.float64)
+    g = np.zeros((n, n))
+    total = ctypes.c_long()
+    L = lib()
+    L.oracle_fock_sampled.restype = ctypes.c_long
+    nq = L.oracle_fock_sampled(ctypes.byref(ob.struct), dm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                               ctypes.c_int(LABELS[label]), ctypes.c_double(exchange_factor),
+                               ctypes.c_int(ithreshold), ctypes.c_int(nthreads), ctypes.c_int(stride),
+                               bins.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                               g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(total))
+    if nq < 0:
+        raise ValueError('oracle_fock_sampled: bad input')
+    return int(nq), int(total.value)
+
+
 if os.environ.get('VLX_ORACLE_BUILD') == '1':
     build()

@@ -29,7 +29,15 @@ def test_screener_maxima_and_bins_match_oracle():
     mx_gpu = scr.pair_maxima()
     mx_ref, bins_ref = orc.pair_maxima(orc.basis_from(bas, mol))
     iu = np.triu_indices(mx_ref.shape[0])
-    assert np.allclose(mx_gpu[iu], mx_ref[iu], rtol=1e-12, atol=1e-15)
+    a, b = mx_gpu[iu], mx_ref[iu]
+    # values agree to round-off of two different recursions (MD vs HGP) ...
+    assert np.all(np.abs(a - b) <= 1e-10 * np.abs(b) + 1e-300)
+    # ... and the Schwarz bins that select quartets are identical
+    def bins(v):
+        f = np.sqrt(v)
+        with np.errstate(divide='ignore'):
+            return np.where(f > 1e-15, np.where(f < 1.0, -np.floor(np.log10(np.maximum(f, 1e-300))), 0), -1)
+    assert np.array_equal(bins(a), bins_ref[iu])
     assert int(np.sum(bins_ref[iu] >= 0)) == scr.number_of_pairs()
 
 

@@ -53,7 +53,15 @@ def lib():
                                        ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_double_p]
         L.vlx_fock_compute_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
                                               ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
-        L.vlx_fock_work.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_ll_p, _c_ll_p]
+        L.vlx_fock_work.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_ll_p, _c_ll_p,
+                                    _c_double_p]
+        L.vlx_context_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.vlx_context_reset_times.argtypes = [ctypes.c_void_p]
+        L.vlx_context_class_times.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_int_p, _c_double_p, _c_ll_p]
+        L.vlx_device_malloc.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p)]
+        L.vlx_device_free.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.vlx_copy_to_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong]
+        L.vlx_copy_to_host.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong]
         L.vlx_eri_quartets.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_double_p]
         _lib = L
     return _lib
@@ -83,6 +91,48 @@ def context(device=0):
         check(lib().vlx_context_create(int(device), ctypes.byref(h)))
         _contexts[device] = h
     return _contexts[device]
+
+
+def class_times(device=0):
+    """{(la, lb, lc, ld): (total_ms, launches)} of the timed class kernels."""
+    L = lib()
+    ctx = context(device)
+    n = ctypes.c_int()
+    check(L.vlx_context_class_times(ctx, 0, ctypes.byref(n), None, None, None))
+    cls = np.zeros(4 * max(n.value, 1), dtype=np.int32)
+    ms = np.zeros(max(n.value, 1))
+    cnt = np.zeros(max(n.value, 1), dtype=np.int64)
+    check(L.vlx_context_class_times(ctx, n.value, ctypes.byref(n), cls.ctypes.data_as(_c_int_p),
+                                    ms.ctypes.data_as(_c_double_p), cnt.ctypes.data_as(_c_ll_p)))
+    return {tuple(int(x) for x in cls[4 * i:4 * i + 4]): (float(ms[i]), int(cnt[i])) for i in range(n.value)}
+
+
+class DeviceArray:
+    """float64 buffer in HBM of one device (no torch needed)."""
+
+    def __init__(self, shape, device=0):
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        self.device = device
+        self.ptr = ctypes.c_void_p()
+        check(lib().vlx_device_malloc(context(device), self.nbytes, ctypes.byref(self.ptr)))
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        assert host.nbytes == self.nbytes
+        check(lib().vlx_copy_to_device(context(self.device), self.ptr, host.ctypes.data_as(ctypes.c_void_p),
+                                       self.nbytes))
+
+    def download(self):
+        out = np.empty(self.shape)
+        check(lib().vlx_copy_to_host(context(self.device), out.ctypes.data_as(ctypes.c_void_p), self.ptr,
+                                     self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(lib().vlx_device_free(context(self.device), self.ptr))
+            self.ptr = ctypes.c_void_p()
 
 
 def current_device():

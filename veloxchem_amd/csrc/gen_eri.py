@@ -75,6 +75,11 @@ def add(t, i, d):
     return tuple(t)
 
 
+def count_flops(expr):
+    """Binary floating-point operations in one generated expression."""
+    return expr.count('*') + expr.count(' + ') + expr.count(' - ')
+
+
 class Code:
     def __init__(self, prefix):
         self.lines = []
@@ -252,7 +257,9 @@ def gen_class(la, lb, lc, ld):
     for ln in hrr.lines:
         w('    ' + ln)
     w('}')
-    return tag, '\n'.join(out), len(vrr.lines), len(hrr.lines)
+    vrr_flops = sum(count_flops(ln.split(' = ', 1)[1]) for ln in vrr.lines) + nacc
+    hrr_flops = sum(count_flops(ln.split(' = ', 1)[1]) for ln in hrr.lines)
+    return tag, '\n'.join(out), vrr_flops, hrr_flops
 
 
 def pair_classes(lmax):
@@ -269,8 +276,10 @@ def main(outdir, lmax):
     outdir.mkdir(parents=True, exist_ok=True)
     pcs = pair_classes(lmax)
     written = []
+    flops = {}
     for (la, lb), (lc, ld) in class_pairs(lmax):
         tag, body, nv, nh = gen_class(la, lb, lc, ld)
+        flops[(la, lb, lc, ld)] = (nv, nh)
         src = [
             '// generated by gen_eri.py -- do not edit',
             '#include "../eri_kernels.h"',
@@ -296,7 +305,9 @@ def main(outdir, lmax):
     for (la, lb, lc, ld) in written:
         x = pcs.index((la, lb))
         y = pcs.index((lc, ld))
-        d.append(f'    {{{x}, {y}, {la}, {lb}, {lc}, {ld}, &launch_fock_{la}{lb}{lc}{ld}, &launch_eri_{la}{lb}{lc}{ld}}},')
+        nv, nh = flops[(la, lb, lc, ld)]
+        d.append(f'    {{{x}, {y}, {la}, {lb}, {lc}, {ld}, {nv}, {nh}, &launch_fock_{la}{lb}{lc}{ld}, '
+                 f'&launch_eri_{la}{lb}{lc}{ld}}},')
     d.append('};')
     d.append('static const DiagLauncher g_diag_table[] = {')
     for (la, lb) in pcs:

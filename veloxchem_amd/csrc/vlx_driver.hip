@@ -134,6 +134,15 @@ struct vlx_context {
     DevBuf<double> d_boys;
     DevBuf<double> d_density, d_fock, d_faccum;
     int block_size_factor = 8;
+    // optional per-class timing
+    bool timing = false;
+    struct Pending {
+        int entry;
+        hipEvent_t start, stop;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<int, std::pair<double, long long>> class_ms;  // entry -> (ms, launches)
 };
 
 struct vlx_screener {
@@ -339,6 +348,11 @@ void vlx_context_destroy(vlx_context* ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& p : ctx->pending) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    for (auto ev : ctx->event_pool) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -359,6 +373,39 @@ int vlx_context_synchronize(vlx_context* ctx)
 }
 
 void* vlx_context_stream(vlx_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int vlx_device_malloc(vlx_context* ctx, unsigned long long bytes, void** out)
+{
+    if (!ctx || !out) return fail(VLX_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMalloc(out, std::max<unsigned long long>(bytes, 1)));
+    return VLX_OK;
+}
+
+int vlx_device_free(vlx_context* ctx, void* ptr)
+{
+    if (!ctx) return fail(VLX_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return VLX_OK;
+}
+
+int vlx_copy_to_device(vlx_context* ctx, void* dst, const void* src, unsigned long long bytes)
+{
+    if (!ctx || !dst || !src) return fail(VLX_ERR_ARG, "NULL argument");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VLX_OK;
+}
+
+int vlx_copy_to_host(vlx_context* ctx, void* dst, const void* src, unsigned long long bytes)
+{
+    if (!ctx || !dst || !src) return fail(VLX_ERR_ARG, "NULL argument");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VLX_OK;
+}
 
 int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* nprim, const int* prim_offset,
                         const double* exps, const double* norms, const double* centers, const int* ao_base,
@@ -494,8 +541,64 @@ int vlx_screener_number_of_pairs(const vlx_screener* scr, long long* out)
     return VLX_OK;
 }
 
+int vlx_context_set_timing(vlx_context* ctx, int enable)
+{
+    if (!ctx) return fail(VLX_ERR_ARG, "context is NULL");
+    ctx->timing = enable != 0;
+    return VLX_OK;
+}
+
+static int drain_timing(vlx_context* ctx)
+{
+    if (ctx->pending.empty()) return VLX_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (auto& p : ctx->pending) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, p.start, p.stop));
+        auto& acc = ctx->class_ms[p.entry];
+        acc.first += ms;
+        acc.second += 1;
+        ctx->event_pool.push_back(p.start);
+        ctx->event_pool.push_back(p.stop);
+    }
+    ctx->pending.clear();
+    return VLX_OK;
+}
+
+int vlx_context_reset_times(vlx_context* ctx)
+{
+    if (!ctx) return fail(VLX_ERR_ARG, "context is NULL");
+    int rc = drain_timing(ctx);
+    ctx->class_ms.clear();
+    return rc;
+}
+
+int vlx_context_class_times(vlx_context* ctx, int max_records, int* nrecords, int* classes, double* total_ms,
+                            long long* launches)
+{
+    if (!ctx || !nrecords) return fail(VLX_ERR_ARG, "NULL argument");
+    int rc = drain_timing(ctx);
+    if (rc) return rc;
+    *nrecords = static_cast<int>(ctx->class_ms.size());
+    int r = 0;
+    for (const auto& kv : ctx->class_ms) {
+        if (r >= max_records) break;
+        const auto& ce = g_class_table[kv.first];
+        if (classes) {
+            classes[4 * r] = ce.la;
+            classes[4 * r + 1] = ce.lb;
+            classes[4 * r + 2] = ce.lc;
+            classes[4 * r + 3] = ce.ld;
+        }
+        if (total_ms) total_ms[r] = kv.second.first;
+        if (launches) launches[r] = kv.second.second;
+        r++;
+    }
+    return VLX_OK;
+}
+
 int vlx_fock_work(vlx_screener* scr, int ithreshold, int max_records, int* nrecords, int* classes,
-                  long long* quartets, long long* prim_quartets)
+                  long long* quartets, long long* prim_quartets, double* flops)
 {
     if (!scr || !nrecords) return fail(VLX_ERR_ARG, "NULL argument");
     if (ithreshold > 30) return fail(VLX_ERR_ARG, "ithreshold %d outside the accepted range (< 0 or 0..30)", ithreshold);
@@ -514,6 +617,12 @@ int vlx_fock_work(vlx_screener* scr, int ithreshold, int max_records, int* nreco
         }
         if (quartets) quartets[r] = sch.quartets;
         if (prim_quartets) prim_quartets[r] = sch.prim_quartets;
+        if (flops) {
+            const int L = ce.la + ce.lb + ce.lc + ce.ld;
+            const int nsph = (2 * ce.la + 1) * (2 * ce.lb + 1) * (2 * ce.lc + 1) * (2 * ce.ld + 1);
+            flops[r] = (double)sch.prim_quartets * (ce.vrr_flops + 53 + 4 * L) +
+                       (double)sch.quartets * (ce.hrr_flops + 12.0 * nsph);
+        }
     }
     return VLX_OK;
 }
@@ -561,8 +670,24 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
         a.fk = fk;
         const long long waves = std::min<long long>(a.total_tiles, 8192);
         const int grid = static_cast<int>((waves + 3) / 4);
+        vlx_context::Pending pend{sch.entry, nullptr, nullptr};
+        if (ctx->timing) {
+            for (hipEvent_t* ev : {&pend.start, &pend.stop}) {
+                if (ctx->event_pool.empty()) {
+                    HIP_TRY(hipEventCreate(ev));
+                } else {
+                    *ev = ctx->event_pool.back();
+                    ctx->event_pool.pop_back();
+                }
+            }
+            HIP_TRY(hipEventRecord(pend.start, ctx->stream));
+        }
         ce.fock(a, grid, ctx->stream);
         HIP_TRY(hipGetLastError());
+        if (ctx->timing) {
+            HIP_TRY(hipEventRecord(pend.stop, ctx->stream));
+            ctx->pending.push_back(pend);
+        }
     }
     dim3 sg((n + 255) / 256, n);
     symmetrize_kernel<<<sg, 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, d_fock_out, n);

@@ -74,6 +74,8 @@ using DiagLaunchFn = void (*)(const DiagArgs&, hipStream_t);
 struct ClassLaunchers {
     int x, y;             // bra / ket pair-class indices (x >= y)
     int la, lb, lc, ld;
+    int vrr_flops;        // generated VRR + contraction FLOPs per primitive quartet
+    int hrr_flops;        // generated HRR + cart->sph FLOPs per contracted quartet
     FockLaunchFn fock;
     EriLaunchFn eri;
 };

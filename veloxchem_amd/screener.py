@@ -81,14 +81,16 @@ class T4CScreener:
         """Per-class (la, lb, lc, ld) contracted and primitive quartet counts of one build."""
         L = _native.lib()
         n = ctypes.c_int()
-        _native.check(L.vlx_fock_work(self.handle, int(ithreshold), 0, ctypes.byref(n), None, None, None))
-        cls = np.zeros(4 * n.value, dtype=np.int32)
-        nq = np.zeros(n.value, dtype=np.int64)
-        npq = np.zeros(n.value, dtype=np.int64)
+        _native.check(L.vlx_fock_work(self.handle, int(ithreshold), 0, ctypes.byref(n), None, None, None, None))
+        cls = np.zeros(4 * max(n.value, 1), dtype=np.int32)
+        nq = np.zeros(max(n.value, 1), dtype=np.int64)
+        npq = np.zeros(max(n.value, 1), dtype=np.int64)
+        fl = np.zeros(max(n.value, 1))
         _native.check(L.vlx_fock_work(self.handle, int(ithreshold), n.value, ctypes.byref(n),
                                       cls.ctypes.data_as(_native._c_int_p), nq.ctypes.data_as(_native._c_ll_p),
-                                      npq.ctypes.data_as(_native._c_ll_p)))
-        return [(tuple(int(x) for x in cls[4 * i:4 * i + 4]), int(nq[i]), int(npq[i])) for i in range(n.value)]
+                                      npq.ctypes.data_as(_native._c_ll_p), fl.ctypes.data_as(_native._c_double_p)))
+        return [(tuple(int(x) for x in cls[4 * i:4 * i + 4]), int(nq[i]), int(npq[i]), float(fl[i]))
+                for i in range(n.value)]
 
     def __eq__(self, other):
         return isinstance(other, T4CScreener) and self._key == other._key
