@@ -70,186 +70,197 @@ __device__ __forceinline__ void stage_boys(double* lds, const double* __restrict
 }
 
 // ---------------------------------------------------------------- Fock
+//
+// Ket-stationary digestion.  Lane j owns ket pair (c, d) for a whole work
+// item and walks the item's bra pairs (a, b) in wave-uniform order.  With
+// G = F' + F'^T and the shell-level symmetry factor s (1/2 for a == b, for
+// c == d and for the diagonal pair-pair quartet), each quartet adds
+//   F'_ab += fj s (ab|cd) D_cd      (wave-reduced, one atomic per component)
+//   F'_cd += fj s (ab|cd) D_ab      (register accumulator, flushed per item)
+//   F'_ac += fk s (ab|cd) D_bd,  F'_ad += fk s (ab|cd) D_bc
+//                                   (register accumulators while shell a is
+//                                    unchanged; bras are sorted by a)
+//   F'_bc += fk s (ab|cd) D_ad,  F'_bd += fk s (ab|cd) D_ac   (atomics)
+// which, summed over the unique quartets, gives the reference's digestion
+// (src/t4c_func/T4CLocalDistributor.cpp:130-280 and siblings).
 
+template <int na, int nc>
+__device__ __forceinline__ void flush_block(double* __restrict__ F, const int n, const int* __restrict__ row,
+                                            const int* __restrict__ col, double (&acc)[na][nc], const double f)
+{
+#pragma unroll
+    for (int i = 0; i < na; i++)
+#pragma unroll
+        for (int k = 0; k < nc; k++) {
+            unsafeAtomicAdd(&F[(size_t)row[i] * n + col[k]], f * acc[i][k]);
+            acc[i][k] = 0.0;
+        }
+}
+
+// Pointers are separate __restrict__ kernel arguments so that wave-uniform
+// loads (work items, bra pairs and primitives, D_ab) become scalar loads.
 template <int la, int lb, int lc, int ld>
-__device__ __forceinline__ void digest_quartet(const FockArgs& a, const PairInfo& bp, const PairInfo& kp, const double s,
-                                               const bool active, const double* __restrict__ I)
+__global__ void __launch_bounds__(VLX_BLOCK)
+    fock_kernel(const PairInfo* __restrict__ bra, const PrimPair* __restrict__ bra_prims,
+                const PairInfo* __restrict__ ket, const PrimPair* __restrict__ ket_prims,
+                const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
+                const double* __restrict__ D, double* __restrict__ F, const FockScalars a)
 {
     constexpr int na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
-    const int n = a.nao;
-    const double* __restrict__ D = a.D;
-    double* __restrict__ F = a.F;
-    int p[na], q[nb], r[nc], t[nd];
-#pragma unroll
-    for (int i = 0; i < na; i++) p[i] = bp.ao_a + i * bp.str_a;
-#pragma unroll
-    for (int i = 0; i < nb; i++) q[i] = bp.ao_b + i * bp.str_b;
-#pragma unroll
-    for (int i = 0; i < nc; i++) r[i] = kp.ao_a + i * kp.str_a;
-#pragma unroll
-    for (int i = 0; i < nd; i++) t[i] = kp.ao_b + i * kp.str_b;
+    constexpr int nsph = na * nb * nc * nd;
+    __shared__ double tab[BOYS_TAB_SIZE];
+    stage_boys(tab, boys_tab);
 
-    if (a.fj != 0.0) {
-        const double fj = a.fj * s;
-        double dcd[nc][nd];
+    const int n = a.nao;
+    const bool do_j = a.fj != 0.0, do_k = a.fk != 0.0;
+    const int lane = threadIdx.x & (VLX_WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / VLX_WAVE);
+    const long long gw = (long long)blockIdx.x * VLX_WAVES_PER_BLOCK + wave;
+    const long long nw = (long long)gridDim.x * VLX_WAVES_PER_BLOCK;
+    const long long w0 = a.nitems * gw / nw, w1 = a.nitems * (gw + 1) / nw;
+
+    for (long long it = w0; it < w1; it++) {
+        const WorkItem wi = items[it];
+        const int j = wi.j0 + lane;
+        const bool lane_ok = j < a.nket;
+        const PairInfo kp = ket[lane_ok ? j : wi.j0];
+        int r[nc], t[nd];
+#pragma unroll
+        for (int k = 0; k < nc; k++) r[k] = kp.ao_a + k * kp.str_a;
+#pragma unroll
+        for (int l = 0; l < nd; l++) t[l] = kp.ao_b + l * kp.str_b;
+        double dcd[nc][nd], jcd[nc][nd];
 #pragma unroll
         for (int k = 0; k < nc; k++)
 #pragma unroll
-            for (int l = 0; l < nd; l++) dcd[k][l] = D[(size_t)r[k] * n + t[l]];
-        // bra side F_pq += fj s sum_rs I D_rs: reduce over the wavefront first
-#pragma unroll
-        for (int i = 0; i < na; i++)
-#pragma unroll
-            for (int j = 0; j < nb; j++) {
-                double v = 0.0;
-#pragma unroll
-                for (int k = 0; k < nc; k++)
-#pragma unroll
-                    for (int l = 0; l < nd; l++) v = fma(I[((i * nb + j) * nc + k) * nd + l], dcd[k][l], v);
-                v = wave_sum(v * fj);
-                if ((threadIdx.x & (VLX_WAVE - 1)) == 0) unsafeAtomicAdd(&F[(size_t)p[i] * n + q[j]], v);
+            for (int l = 0; l < nd; l++) {
+                dcd[k][l] = D[(size_t)r[k] * n + t[l]];
+                jcd[k][l] = 0.0;
             }
-        // ket side F_rs += fj s sum_pq I D_pq
-        double dab[na][nb];
+        double kac[na][nc], kad[na][nd];
 #pragma unroll
-        for (int i = 0; i < na; i++)
+        for (int i = 0; i < na; i++) {
 #pragma unroll
-            for (int j = 0; j < nb; j++) dab[i][j] = D[(size_t)p[i] * n + q[j]];
-        if (active) {
+            for (int k = 0; k < nc; k++) kac[i][k] = 0.0;
 #pragma unroll
-            for (int k = 0; k < nc; k++)
+            for (int l = 0; l < nd; l++) kad[i][l] = 0.0;
+        }
+        int p[na];
+        int cur_a = -1;
+        bool pending = false;  // this lane holds kac/kad contributions for cur_a
+
+        for (int ib = wi.i0; ib < wi.i1; ib++) {
+            const PairInfo bp = bra[ib];
+            if (bp.shell_a != cur_a) {
+                if (do_k && pending) {
+                    flush_block<na, nc>(F, n, p, r, kac, a.fk);
+                    flush_block<na, nd>(F, n, p, t, kad, a.fk);
+                }
+                pending = false;
+                cur_a = bp.shell_a;
 #pragma unroll
-                for (int l = 0; l < nd; l++) {
-                    double v = 0.0;
+                for (int i = 0; i < na; i++) p[i] = bp.ao_a + i * bp.str_a;
+            }
+            bool active = lane_ok;
+            if (a.ithreshold >= 0) active = active && (bp.bin + kp.bin <= a.ithreshold);
+            if (a.same_class) active = active && (bp.rank >= j);
+            double s = active ? 1.0 : 0.0;
+            if (bp.same) s *= 0.5;
+            if (kp.same) s *= 0.5;
+            if (a.same_class && bp.rank == j) s *= 0.5;
+            double I[nsph];
+            eri_quartet<la, lb, lc, ld>(bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
+                                        ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz,
+                                        tab, I);
+#pragma unroll
+            for (int x = 0; x < nsph; x++) I[x] *= s;
+            int q[nb];
+#pragma unroll
+            for (int jb = 0; jb < nb; jb++) q[jb] = bp.ao_b + jb * bp.str_b;
+            if (do_j) {
+                // bra side: F'_ab, reduced over the wavefront
+#pragma unroll
+                for (int i = 0; i < na; i++)
+#pragma unroll
+                    for (int jb = 0; jb < nb; jb++) {
+                        double v = 0.0;
+#pragma unroll
+                        for (int k = 0; k < nc; k++)
+#pragma unroll
+                            for (int l = 0; l < nd; l++) v = fma(I[((i * nb + jb) * nc + k) * nd + l], dcd[k][l], v);
+                        v = wave_sum(v);
+                        if (lane == 0 && v != 0.0) unsafeAtomicAdd(&F[(size_t)p[i] * n + q[jb]], a.fj * v);
+                    }
+                // ket side: F'_cd accumulated in registers
+#pragma unroll
+                for (int i = 0; i < na; i++)
+#pragma unroll
+                    for (int jb = 0; jb < nb; jb++) {
+                        const double dab = D[(size_t)p[i] * n + q[jb]];
+#pragma unroll
+                        for (int k = 0; k < nc; k++)
+#pragma unroll
+                            for (int l = 0; l < nd; l++) jcd[k][l] = fma(I[((i * nb + jb) * nc + k) * nd + l], dab, jcd[k][l]);
+                    }
+            }
+            if (do_k && active) {
+                pending = true;
+                // F'_ac += I D_bd and F'_ad += I D_bc (sticky while a is unchanged)
+#pragma unroll
+                for (int jb = 0; jb < nb; jb++) {
+                    double dbd[nd], dbc[nc];
+#pragma unroll
+                    for (int l = 0; l < nd; l++) dbd[l] = D[(size_t)q[jb] * n + t[l]];
+#pragma unroll
+                    for (int k = 0; k < nc; k++) dbc[k] = D[(size_t)q[jb] * n + r[k]];
 #pragma unroll
                     for (int i = 0; i < na; i++)
 #pragma unroll
-                        for (int j = 0; j < nb; j++) v = fma(I[((i * nb + j) * nc + k) * nd + l], dab[i][j], v);
-                    unsafeAtomicAdd(&F[(size_t)r[k] * n + t[l]], fj * v);
+                        for (int k = 0; k < nc; k++)
+#pragma unroll
+                            for (int l = 0; l < nd; l++) {
+                                const double v = I[((i * nb + jb) * nc + k) * nd + l];
+                                kac[i][k] = fma(v, dbd[l], kac[i][k]);
+                                kad[i][l] = fma(v, dbc[k], kad[i][l]);
+                            }
                 }
-        }
-    }
-    if (a.fk != 0.0 && active) {
-        const double fk = a.fk * s;
-        // F_pr += fk s sum_qs I D_qs
-        {
-            double dqs[nb][nd];
+                // F'_bc += I D_ad and F'_bd += I D_ac (b changes every bra)
+                double dad[na][nd], dac[na][nc];
 #pragma unroll
-            for (int j = 0; j < nb; j++)
+                for (int i = 0; i < na; i++) {
 #pragma unroll
-                for (int l = 0; l < nd; l++) dqs[j][l] = D[(size_t)q[j] * n + t[l]];
+                    for (int l = 0; l < nd; l++) dad[i][l] = D[(size_t)p[i] * n + t[l]];
 #pragma unroll
-            for (int i = 0; i < na; i++)
-#pragma unroll
-                for (int k = 0; k < nc; k++) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int j = 0; j < nb; j++)
-#pragma unroll
-                        for (int l = 0; l < nd; l++) v = fma(I[((i * nb + j) * nc + k) * nd + l], dqs[j][l], v);
-                    unsafeAtomicAdd(&F[(size_t)p[i] * n + r[k]], fk * v);
+                    for (int k = 0; k < nc; k++) dac[i][k] = D[(size_t)p[i] * n + r[k]];
                 }
-        }
-        // F_ps += fk s sum_qr I D_qr
-        {
-            double dqr[nb][nc];
 #pragma unroll
-            for (int j = 0; j < nb; j++)
+                for (int jb = 0; jb < nb; jb++) {
 #pragma unroll
-                for (int k = 0; k < nc; k++) dqr[j][k] = D[(size_t)q[j] * n + r[k]];
+                    for (int k = 0; k < nc; k++) {
+                        double v = 0.0;
 #pragma unroll
-            for (int i = 0; i < na; i++)
+                        for (int i = 0; i < na; i++)
 #pragma unroll
-                for (int l = 0; l < nd; l++) {
-                    double v = 0.0;
+                            for (int l = 0; l < nd; l++) v = fma(I[((i * nb + jb) * nc + k) * nd + l], dad[i][l], v);
+                        unsafeAtomicAdd(&F[(size_t)q[jb] * n + r[k]], a.fk * v);
+                    }
 #pragma unroll
-                    for (int j = 0; j < nb; j++)
+                    for (int l = 0; l < nd; l++) {
+                        double v = 0.0;
 #pragma unroll
-                        for (int k = 0; k < nc; k++) v = fma(I[((i * nb + j) * nc + k) * nd + l], dqr[j][k], v);
-                    unsafeAtomicAdd(&F[(size_t)p[i] * n + t[l]], fk * v);
+                        for (int i = 0; i < na; i++)
+#pragma unroll
+                            for (int k = 0; k < nc; k++) v = fma(I[((i * nb + jb) * nc + k) * nd + l], dac[i][k], v);
+                        unsafeAtomicAdd(&F[(size_t)q[jb] * n + t[l]], a.fk * v);
+                    }
                 }
+            }
         }
-        // F_qr += fk s sum_ps I D_ps
-        {
-            double dps[na][nd];
-#pragma unroll
-            for (int i = 0; i < na; i++)
-#pragma unroll
-                for (int l = 0; l < nd; l++) dps[i][l] = D[(size_t)p[i] * n + t[l]];
-#pragma unroll
-            for (int j = 0; j < nb; j++)
-#pragma unroll
-                for (int k = 0; k < nc; k++) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int i = 0; i < na; i++)
-#pragma unroll
-                        for (int l = 0; l < nd; l++) v = fma(I[((i * nb + j) * nc + k) * nd + l], dps[i][l], v);
-                    unsafeAtomicAdd(&F[(size_t)q[j] * n + r[k]], fk * v);
-                }
+        if (do_k && pending) {
+            flush_block<na, nc>(F, n, p, r, kac, a.fk);
+            flush_block<na, nd>(F, n, p, t, kad, a.fk);
         }
-        // F_qs += fk s sum_pr I D_pr
-        {
-            double dpr[na][nc];
-#pragma unroll
-            for (int i = 0; i < na; i++)
-#pragma unroll
-                for (int k = 0; k < nc; k++) dpr[i][k] = D[(size_t)p[i] * n + r[k]];
-#pragma unroll
-            for (int j = 0; j < nb; j++)
-#pragma unroll
-                for (int l = 0; l < nd; l++) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int i = 0; i < na; i++)
-#pragma unroll
-                        for (int k = 0; k < nc; k++) v = fma(I[((i * nb + j) * nc + k) * nd + l], dpr[i][k], v);
-                    unsafeAtomicAdd(&F[(size_t)q[j] * n + t[l]], fk * v);
-                }
-        }
-    }
-}
-
-// Each wavefront takes a contiguous range of 64-ket tiles; a tile is
-// (bra pair i, kets [64 w, 64 w + 64)).  tile_prefix[i] is the first tile of
-// bra pair i and nket[i] the length of its ket prefix.
-template <int la, int lb, int lc, int ld>
-__global__ void __launch_bounds__(VLX_BLOCK) fock_kernel(const FockArgs a)
-{
-    constexpr int nsph = (2 * la + 1) * (2 * lb + 1) * (2 * lc + 1) * (2 * ld + 1);
-    __shared__ double tab[BOYS_TAB_SIZE];
-    stage_boys(tab, a.boys_tab);
-
-    const int lane = threadIdx.x & (VLX_WAVE - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / VLX_WAVE);
-    const long gw = (long)blockIdx.x * VLX_WAVES_PER_BLOCK + wave;
-    const long nw = (long)gridDim.x * VLX_WAVES_PER_BLOCK;
-    const long long t0 = a.total_tiles * gw / nw;
-    const long long t1 = a.total_tiles * (gw + 1) / nw;
-    if (t0 >= t1) return;
-
-    int lo = 0, hi = a.nbra;  // tile_prefix[lo] <= t0 < tile_prefix[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (a.tile_prefix[mid] <= t0) lo = mid;
-        else hi = mid;
-    }
-    int i = lo;
-    for (long long tile = t0; tile < t1; tile++) {
-        while (a.tile_prefix[i + 1] <= tile) i++;
-        const PairInfo bp = a.bra[i];
-        const int nk = a.nket[i];
-        const int j = static_cast<int>(tile - a.tile_prefix[i]) * VLX_WAVE + lane;
-        const bool active = j < nk;
-        const PairInfo kp = a.ket[active ? j : 0];
-        double I[nsph];
-        eri_quartet<la, lb, lc, ld>(a.bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
-                                    a.ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz, tab, I);
-        double s = active ? 1.0 : 0.0;
-        if (bp.same) s *= 0.5;
-        if (kp.same) s *= 0.5;
-        if (a.same_class && i == j) s *= 0.5;
-        digest_quartet<la, lb, lc, ld>(a, bp, kp, s, active, I);
+        if (do_j && lane_ok) flush_block<nc, nd>(F, n, r, t, jcd, a.fj);
     }
 }
 
@@ -299,7 +310,8 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
 #define VLX_INSTANTIATE_CLASS(la, lb, lc, ld)                                                           \
     void launch_fock_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                       \
     {                                                                                                   \
-        hipLaunchKernelGGL((fock_kernel<la, lb, lc, ld>), dim3(grid), dim3(VLX_BLOCK), 0, s, a);        \
+        hipLaunchKernelGGL((fock_kernel<la, lb, lc, ld>), dim3(grid), dim3(VLX_BLOCK), 0, s, a.bra,     \
+                           a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F, a.sc);       \
     }                                                                                                   \
     void launch_eri_##la##lb##lc##ld(const EriArgs& a, hipStream_t s)                                   \
     {                                                                                                   \

@@ -163,14 +163,14 @@ def gen_class(la, lb, lc, ld):
     w(f'    double acc[{nacc}];')
     w('#pragma unroll')
     w(f'    for (int i = 0; i < {nacc}; i++) acc[i] = 0.0;')
-    w('    for (int ib = 0; ib < bra_np; ib++) {')
-    w('        const PrimPair bp = bra_prims[ib];')
-    w('        const double zeta = bp.zeta;')
-    w('        const double oo2z = 0.5 / zeta;')
-    w('        const double PAx = bp.pax, PAy = bp.pay, PAz = bp.paz;')
-    w('        for (int ik = 0; ik < ket_np; ik++) {')
-    w('            const PrimPair kp = ket_prims[ik];')
-    w('            const double eta = kp.zeta;')
+    w('    for (int ik = 0; ik < ket_np; ik++) {')
+    w('        const PrimPair kp = ket_prims[ik];')
+    w('        const double eta = kp.zeta;')
+    w('        for (int ib = 0; ib < bra_np; ib++) {')
+    w('            const PrimPair bp = bra_prims[ib];')
+    w('            const double zeta = bp.zeta;')
+    w('            const double oo2z = 0.5 / zeta;')
+    w('            const double PAx = bp.pax, PAy = bp.pay, PAz = bp.paz;')
     w('            const double inv_s = 1.0 / (zeta + eta);')
     w('            const double rho = zeta * eta * inv_s;')
     w('            const double PQx = bp.px - kp.px, PQy = bp.py - kp.py, PQz = bp.pz - kp.pz;')

@@ -121,16 +121,22 @@ struct PairSet {
 
 struct ClassSchedule {
     int entry = -1;  // index into g_class_table
-    std::vector<int> nket;
-    std::vector<long long> tile_prefix;
+    int stream = 0;  // side stream the class runs on
+    std::vector<WorkItem> items;
     long long quartets = 0, prim_quartets = 0;
-    DevBuf<int> d_nket;
-    DevBuf<long long> d_tile_prefix;
+    DevBuf<WorkItem> d_items;
 };
+
+constexpr int BRA_CHUNK = 128;  // bras per work item
+
+constexpr int NSIDE = 4;  // side streams that run class kernels concurrently
 
 struct vlx_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side[NSIDE] = {};
+    hipEvent_t ev_start = nullptr;
+    hipEvent_t ev_done[NSIDE] = {};
     DevBuf<double> d_boys;
     DevBuf<double> d_density, d_fock, d_faccum;
     int block_size_factor = 8;
@@ -150,8 +156,10 @@ struct vlx_screener {
     int nshells = 0, nao = 0;
     std::vector<int> l, nprim, prim_offset, ao_base, ao_stride;
     std::vector<double> exps, norms, centers;
-    std::vector<PairSet> all;       // every pair, per pair class
-    std::vector<PairSet> kept;      // binned pairs sorted by (bin, nprim)
+    std::vector<PairSet> all;       // every pair, per pair class (ket order = construction order)
+    std::vector<PairSet> all_bra;   // the same pairs sorted by shell A (bra order)
+    std::vector<PairSet> kept;      // binned pairs sorted by (bin, nprim): ket order
+    std::vector<PairSet> kept_bra;  // binned pairs sorted by (bin, shell A, nprim): bra order
     std::vector<double> pair_max;   // nshells x nshells
     struct PairRef {
         int cls = -1, idx = -1, swapped = 0;
@@ -197,10 +205,11 @@ void add_pair(const vlx_screener& s, PairSet& ps, int sa, int sb)
     pi.nprim = s.nprim[sa] * s.nprim[sb];
     pi.same = (sa == sb) ? 1 : 0;
     pi.bin = 0;
+    pi.rank = static_cast<int>(ps.pairs.size());
+    pi.shell_a = sa;
     pi.abx = s.centers[3 * sa] - s.centers[3 * sb];
     pi.aby = s.centers[3 * sa + 1] - s.centers[3 * sb + 1];
     pi.abz = s.centers[3 * sa + 2] - s.centers[3 * sb + 2];
-    pi.pad = 0.0;
     ps.pairs.push_back(pi);
     ps.shell_a.push_back(sa);
     ps.shell_b.push_back(sb);
@@ -213,6 +222,32 @@ int pair_bin(double maxval)
     const double fact = std::sqrt(maxval);
     if (fact > 1.0e-15) return (fact < 1.0) ? static_cast<int>(-std::floor(std::log10(fact))) : 0;
     return -1;
+}
+
+void finalize_set(PairSet& ps)
+{
+    for (int b = 0; b < NBINS; b++) ps.cnt_le[b] = 0;
+    for (int k = 0; k < ps.size(); k++)
+        for (int b = std::max(ps.pairs[k].bin, 0); b < NBINS; b++) ps.cnt_le[b]++;
+    ps.nprim_prefix.assign(ps.size() + 1, 0);
+    for (int k = 0; k < ps.size(); k++) ps.nprim_prefix[k + 1] = ps.nprim_prefix[k] + ps.pairs[k].nprim;
+}
+
+// Copy of `src` reordered by `order`; PairInfo.rank keeps the position in src.
+void reorder_set(const vlx_screener& s, const PairSet& src, const std::vector<int>& order, PairSet& dst)
+{
+    dst.pairs.clear();
+    dst.prims.clear();
+    dst.shell_a.clear();
+    dst.shell_b.clear();
+    dst.bins.clear();
+    for (int k : order) {
+        add_pair(s, dst, src.shell_a[k], src.shell_b[k]);
+        dst.pairs.back().bin = src.pairs[k].bin;
+        dst.pairs.back().rank = src.pairs[k].rank;
+        dst.bins.push_back(src.pairs[k].bin);
+    }
+    finalize_set(dst);
 }
 
 int upload_set(PairSet& ps)
@@ -272,33 +307,65 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
     const bool unscreened = ithr < 0;
     for (int e = 0; e < nentries; e++) {
         const auto& ce = g_class_table[e];
-        const PairSet& bra = unscreened ? s->all[ce.x] : s->kept[ce.x];
+        const PairSet& bra = unscreened ? s->all_bra[ce.x] : s->kept_bra[ce.x];
         const PairSet& ket = unscreened ? s->all[ce.y] : s->kept[ce.y];
         if (bra.size() == 0 || ket.size() == 0) continue;
+        const bool same = ce.x == ce.y;
         auto sch = std::make_unique<ClassSchedule>();
         sch->entry = e;
-        const int nb = bra.size();
-        sch->nket.resize(nb);
-        sch->tile_prefix.resize(nb + 1);
-        sch->tile_prefix[0] = 0;
-        for (int i = 0; i < nb; i++) {
-            int nk;
+        // exact quartet counts, from the ket side (ket order is sorted by bin)
+        for (int j = 0; j < ket.size(); j++) {
+            long long lo = 0, hi;
             if (unscreened) {
-                nk = ket.size();
+                hi = bra.size();
             } else {
-                const int lim = ithr - bra.bins[i];
-                nk = (lim < 0) ? 0 : ket.cnt_le[std::min(lim, NBINS - 1)];
+                const int lim = ithr - ket.pairs[j].bin;
+                hi = (lim < 0) ? 0 : bra.cnt_le[std::min(lim, NBINS - 1)];
             }
-            if (ce.x == ce.y) nk = std::min(nk, i + 1);
-            sch->nket[i] = nk;
-            sch->tile_prefix[i + 1] = sch->tile_prefix[i] + (nk + 63) / 64;
-            sch->quartets += nk;
-            sch->prim_quartets += (long long)bra.pairs[i].nprim * ket.nprim_prefix[nk];
+            if (same) lo = j;  // ranks of the same-class bras run over the ket order
+            if (hi <= lo) continue;
+            const PairSet& rk = same ? ket : bra;  // prefix sums in an order where [lo, hi) is contiguous
+            sch->quartets += hi - lo;
+            sch->prim_quartets += (long long)ket.pairs[j].nprim * (rk.nprim_prefix[hi] - rk.nprim_prefix[lo]);
         }
-        if (sch->tile_prefix[nb] == 0) continue;
-        HIP_TRY(sch->d_nket.upload(sch->nket));
-        HIP_TRY(sch->d_tile_prefix.upload(sch->tile_prefix));
+        // work items: 64 consecutive kets x a chunk of bras; the chunk shrinks
+        // for classes with little work so that the launch still has >= ~16k items
+        std::vector<std::pair<int, int>> ranges;
+        long long span = 0;
+        for (int j0 = 0; j0 < ket.size(); j0 += 64) {
+            const int bin_lo = unscreened ? 0 : ket.pairs[j0].bin;
+            int hi;
+            if (unscreened) {
+                hi = bra.size();
+            } else {
+                const int lim = ithr - bin_lo;
+                hi = (lim < 0) ? 0 : bra.cnt_le[std::min(lim, NBINS - 1)];
+            }
+            int lo = 0;
+            if (same && !unscreened) lo = (bin_lo > 0) ? bra.cnt_le[bin_lo - 1] : 0;
+            ranges.push_back({lo, hi});
+            span += std::max(0, hi - lo);
+        }
+        const int chunk = static_cast<int>(std::max<long long>(1, std::min<long long>(BRA_CHUNK, span / 16384)));
+        for (size_t t = 0; t < ranges.size(); t++)
+            for (int i0 = ranges[t].first; i0 < ranges[t].second; i0 += chunk)
+                sch->items.push_back({static_cast<int>(64 * t), i0, std::min(ranges[t].second, i0 + chunk), 0});
+        if (sch->items.empty()) continue;
+        HIP_TRY(sch->d_items.upload(sch->items));
         list.push_back(std::move(sch));
+    }
+    // spread classes over the side streams, most expensive first (estimated FLOPs)
+    std::vector<std::pair<double, ClassSchedule*>> cost;
+    for (auto& sp : list) {
+        const auto& ce = g_class_table[sp->entry];
+        cost.push_back({(double)sp->prim_quartets * (ce.vrr_flops + 60) + (double)sp->quartets * ce.hrr_flops, sp.get()});
+    }
+    std::sort(cost.begin(), cost.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    double load[NSIDE] = {0.0};
+    for (auto& c : cost) {
+        const int k = static_cast<int>(std::min_element(load, load + NSIDE) - load);
+        c.second->stream = k;
+        load[k] += c.first;
     }
     auto res = s->schedules.emplace(ithr, std::move(list));
     *out = &res.first->second;
@@ -336,6 +403,11 @@ int vlx_context_create(int device, vlx_context** out)
     ctx->device = device;
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (int k = 0; k < NSIDE; k++) {
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&ctx->ev_done[k], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventCreateWithFlags(&ctx->ev_start, hipEventDisableTiming));
     std::vector<double> tab((size_t)(VLX_MAX_LTOT + 1) * BOYS_TAB_SIZE);
     vlx_build_boys_table(tab.data());
     HIP_TRY(ctx->d_boys.upload(tab));
@@ -353,6 +425,12 @@ void vlx_context_destroy(vlx_context* ctx)
         (void)hipEventDestroy(p.stop);
     }
     for (auto ev : ctx->event_pool) (void)hipEventDestroy(ev);
+    for (int k = 0; k < NSIDE; k++) {
+        if (ctx->side[k]) (void)hipStreamSynchronize(ctx->side[k]);
+        if (ctx->side[k]) (void)hipStreamDestroy(ctx->side[k]);
+        if (ctx->ev_done[k]) (void)hipEventDestroy(ctx->ev_done[k]);
+    }
+    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -440,13 +518,17 @@ int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* 
 
     const int npc = npair_classes();
     s->all.resize(npc);
+    s->all_bra.resize(npc);
     s->kept.resize(npc);
+    s->kept_bra.resize(npc);
     for (int la = 0; la <= VLX_GEN_LMAX; la++)
         for (int lb = 0; lb <= la; lb++) {
             s->all[pair_class_index(la, lb)].la = la;
             s->all[pair_class_index(la, lb)].lb = lb;
-            s->kept[pair_class_index(la, lb)].la = la;
-            s->kept[pair_class_index(la, lb)].lb = lb;
+            for (auto* v : {&s->all_bra, &s->kept, &s->kept_bra}) {
+                (*v)[pair_class_index(la, lb)].la = la;
+                (*v)[pair_class_index(la, lb)].lb = lb;
+            }
         }
     s->pair_ref.assign((size_t)nshells * nshells, {});
     // every unordered shell pair once, oriented l_A >= l_B (A >= B by index on ties)
@@ -487,7 +569,7 @@ int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* 
             ps.bins[i] = pair_bin(mx[i]);
             ps.pairs[i].bin = ps.bins[i];
         }
-        // kept set: bin >= 0, sorted by (bin, nprim)
+        // kept set (ket order): bin >= 0, sorted by (bin, nprim)
         std::vector<int> order;
         for (int i = 0; i < ps.size(); i++)
             if (ps.bins[i] >= 0) order.push_back(i);
@@ -496,22 +578,28 @@ int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* 
             return ps.pairs[i].nprim < ps.pairs[j].nprim;
         });
         PairSet& ks = s->kept[x];
-        for (int i : order) add_pair(*s, ks, ps.shell_a[i], ps.shell_b[i]);
-        ks.bins.resize(ks.size());
-        for (int k = 0; k < ks.size(); k++) {
-            ks.bins[k] = ps.bins[order[k]];
-            ks.pairs[k].bin = ks.bins[k];
+        reorder_set(*s, ps, order, ks);
+        for (int k = 0; k < ks.size(); k++) ks.pairs[k].rank = k;
+        // bra order: (bin, shell A, nprim) over the kept set, ranks from the ket order
+        std::vector<int> border(ks.size());
+        std::iota(border.begin(), border.end(), 0);
+        std::stable_sort(border.begin(), border.end(), [&](int i, int j) {
+            if (ks.pairs[i].bin != ks.pairs[j].bin) return ks.pairs[i].bin < ks.pairs[j].bin;
+            if (ks.shell_a[i] != ks.shell_a[j]) return ks.shell_a[i] < ks.shell_a[j];
+            return ks.pairs[i].nprim < ks.pairs[j].nprim;
+        });
+        reorder_set(*s, ks, border, s->kept_bra[x]);
+        // unscreened lists: every pair, bins ignored
+        for (int k = 0; k < ps.size(); k++) ps.pairs[k].rank = k;
+        finalize_set(ps);
+        std::vector<int> aorder(ps.size());
+        std::iota(aorder.begin(), aorder.end(), 0);
+        std::stable_sort(aorder.begin(), aorder.end(), [&](int i, int j) { return ps.shell_a[i] < ps.shell_a[j]; });
+        reorder_set(*s, ps, aorder, s->all_bra[x]);
+        for (PairSet* up : {&ps, &ks, &s->kept_bra[x], &s->all_bra[x]}) {
+            rc = upload_set(*up);
+            if (rc) return rc;
         }
-        for (int b = 0; b < NBINS; b++) ks.cnt_le[b] = 0;
-        for (int k = 0; k < ks.size(); k++)
-            for (int b = ks.bins[k]; b < NBINS; b++) ks.cnt_le[b]++;
-        ks.nprim_prefix.assign(ks.size() + 1, 0);
-        for (int k = 0; k < ks.size(); k++) ks.nprim_prefix[k + 1] = ks.nprim_prefix[k] + ks.pairs[k].nprim;
-        ps.nprim_prefix.assign(ps.size() + 1, 0);
-        for (int k = 0; k < ps.size(); k++) ps.nprim_prefix[k + 1] = ps.nprim_prefix[k] + ps.pairs[k].nprim;
-        for (int b = 0; b < NBINS; b++) ps.cnt_le[b] = ps.size();
-        rc = upload_set(ks);
-        if (rc) return rc;
     }
     *out = s.release();
     return VLX_OK;
@@ -552,6 +640,7 @@ static int drain_timing(vlx_context* ctx)
 {
     if (ctx->pending.empty()) return VLX_OK;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamSynchronize(ctx->side[k]));
     for (auto& p : ctx->pending) {
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, p.start, p.stop));
@@ -647,28 +736,30 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
     rc = get_schedules(scr, ithreshold, &list);
     if (rc) return rc;
     const bool unscreened = ithreshold < 0;
+    HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+    for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamWaitEvent(ctx->side[k], ctx->ev_start, 0));
     for (const auto& schp : *list) {
         const ClassSchedule& sch = *schp;
         const auto& ce = g_class_table[sch.entry];
-        const PairSet& bra = unscreened ? scr->all[ce.x] : scr->kept[ce.x];
+        const PairSet& bra = unscreened ? scr->all_bra[ce.x] : scr->kept_bra[ce.x];
         const PairSet& ket = unscreened ? scr->all[ce.y] : scr->kept[ce.y];
         FockArgs a{};
         a.bra = bra.d_pairs.ptr;
         a.bra_prims = bra.d_prims.ptr;
         a.ket = ket.d_pairs.ptr;
         a.ket_prims = ket.d_prims.ptr;
-        a.tile_prefix = sch.d_tile_prefix.ptr;
-        a.nket = sch.d_nket.ptr;
-        a.total_tiles = sch.tile_prefix.back();
-        a.nbra = bra.size();
-        a.same_class = (ce.x == ce.y) ? 1 : 0;
+        a.items = sch.d_items.ptr;
         a.boys_tab = ctx->d_boys.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;
         a.D = d_density;
         a.F = ctx->d_faccum.ptr;
-        a.nao = n;
-        a.fj = fj;
-        a.fk = fk;
-        const long long waves = std::min<long long>(a.total_tiles, 8192);
+        a.sc.nitems = static_cast<long long>(sch.items.size());
+        a.sc.nket = ket.size();
+        a.sc.ithreshold = ithreshold;
+        a.sc.same_class = (ce.x == ce.y) ? 1 : 0;
+        a.sc.nao = n;
+        a.sc.fj = fj;
+        a.sc.fk = fk;
+        const long long waves = std::min<long long>(a.sc.nitems, 8192);
         const int grid = static_cast<int>((waves + 3) / 4);
         vlx_context::Pending pend{sch.entry, nullptr, nullptr};
         if (ctx->timing) {
@@ -680,14 +771,18 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
                     ctx->event_pool.pop_back();
                 }
             }
-            HIP_TRY(hipEventRecord(pend.start, ctx->stream));
+            HIP_TRY(hipEventRecord(pend.start, ctx->side[sch.stream]));
         }
-        ce.fock(a, grid, ctx->stream);
+        ce.fock(a, grid, ctx->side[sch.stream]);
         HIP_TRY(hipGetLastError());
         if (ctx->timing) {
-            HIP_TRY(hipEventRecord(pend.stop, ctx->stream));
+            HIP_TRY(hipEventRecord(pend.stop, ctx->side[sch.stream]));
             ctx->pending.push_back(pend);
         }
+    }
+    for (int k = 0; k < NSIDE; k++) {
+        HIP_TRY(hipEventRecord(ctx->ev_done[k], ctx->side[k]));
+        HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_done[k], 0));
     }
     dim3 sg((n + 255) / 256, n);
     symmetrize_kernel<<<sg, 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, d_fock_out, n);

@@ -19,7 +19,7 @@ struct alignas(64) PrimPair {
     double k;             // sqrt(2) pi^(5/4) N_a N_b exp(-a b |AB|^2 / zeta) / zeta
 };
 
-// One contracted shell pair (A, B) with l_A >= l_B.
+// One contracted shell pair (A, B) with l_A >= l_B (64 B).
 struct alignas(16) PairInfo {
     int ao_a, str_a;  // AO index of component 0 of A and stride between components
     int ao_b, str_b;
@@ -27,25 +27,38 @@ struct alignas(16) PairInfo {
     int nprim;        // number of primitive pairs
     int same;         // 1 when A and B are the same shell
     int bin;          // Schwarz bin (0..15)
-    double abx, aby, abz, pad;  // A - B
+    int rank;         // position in the ket ordering of its pair class
+    int shell_a;      // shell index of A
+    double abx, aby, abz;  // A - B
 };
 
-struct FockArgs {
-    const PairInfo* bra;
-    const PrimPair* bra_prims;
-    const PairInfo* ket;
-    const PrimPair* ket_prims;
-    const long long* tile_prefix;  // nbra + 1 entries
-    const int* nket;               // kets of each bra pair
-    long long total_tiles;
-    int nbra;
-    int same_class;                // bra and ket come from the same pair list
-    const double* boys_tab;        // BOYS_TAB_SIZE doubles for L = la+lb+lc+ld
-    const double* D;               // nao x nao, row-major
-    double* F;                     // nao x nao accumulator F' (G = F' + F'^T)
+// One unit of Fock work: kets [j0, j0 + 64) of the ket list (one per lane)
+// against bras [i0, i1) of the bra list.
+struct WorkItem {
+    int j0, i0, i1, pad;
+};
+
+// Scalar parameters of one Fock class launch.
+struct FockScalars {
+    long long nitems;
+    int nket;
+    int ithreshold;                // < 0: no Schwarz selection
+    int same_class;                // bra and ket lists hold the same pairs (quartets with rank_bra >= rank_ket)
     int nao;
     double fj;                     // coefficient of the Coulomb terms (0: none)
     double fk;                     // coefficient of the exchange terms (0: none)
+};
+
+struct FockArgs {
+    const PairInfo* bra;           // bra list, sorted by (bin, shell A, primitive count)
+    const PrimPair* bra_prims;
+    const PairInfo* ket;           // ket list, sorted by (bin, primitive count)
+    const PrimPair* ket_prims;
+    const WorkItem* items;
+    const double* boys_tab;        // BOYS_TAB_SIZE doubles for L = la+lb+lc+ld
+    const double* D;               // nao x nao, row-major
+    double* F;                     // nao x nao accumulator F' (G = F' + F'^T)
+    FockScalars sc;
 };
 
 struct EriArgs {
