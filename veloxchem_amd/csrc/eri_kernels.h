@@ -264,6 +264,137 @@ __global__ void __launch_bounds__(VLX_BLOCK)
     }
 }
 
+// Bra-stationary digestion, for classes whose integrals need most of the
+// register file.  A wavefront takes one bra pair (wave-uniform) and 64 kets
+// (one per lane) from the prefix of the ket list its bins allow; work item
+// {j0, i0, i1}: kets [j0, j0 + 64) of bra i0, i1 = number of kets of bra i0.
+// F'_ab is reduced over the wavefront; F'_cd, F'_ac, F'_ad, F'_bc and F'_bd
+// are per-lane atomics.
+template <int la, int lb, int lc, int ld>
+__global__ void __launch_bounds__(VLX_BLOCK)
+    fock_kernel_bs(const PairInfo* __restrict__ bra, const PrimPair* __restrict__ bra_prims,
+                   const PairInfo* __restrict__ ket, const PrimPair* __restrict__ ket_prims,
+                   const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
+                   const double* __restrict__ D, double* __restrict__ F, const FockScalars a)
+{
+    constexpr int na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
+    constexpr int nsph = na * nb * nc * nd;
+    __shared__ double tab[BOYS_TAB_SIZE];
+    stage_boys(tab, boys_tab);
+
+    const int n = a.nao;
+    const int lane = threadIdx.x & (VLX_WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / VLX_WAVE);
+    const long long gw = (long long)blockIdx.x * VLX_WAVES_PER_BLOCK + wave;
+    const long long nw = (long long)gridDim.x * VLX_WAVES_PER_BLOCK;
+    const long long w0 = a.nitems * gw / nw, w1 = a.nitems * (gw + 1) / nw;
+
+    for (long long it = w0; it < w1; it++) {
+        const WorkItem wi = items[it];
+        const PairInfo bp = bra[wi.i0];
+        const int j = wi.j0 + lane;
+        const bool active = j < wi.i1;
+        const PairInfo kp = ket[active ? j : wi.j0];
+        double s = active ? 1.0 : 0.0;
+        if (bp.same) s *= 0.5;
+        if (kp.same) s *= 0.5;
+        if (a.same_class && bp.rank == j) s *= 0.5;
+        double I[nsph];
+        eri_quartet<la, lb, lc, ld>(bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
+                                    ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz, tab, I);
+#pragma unroll
+        for (int x = 0; x < nsph; x++) I[x] *= s;
+        int p[na], q[nb], r[nc], t[nd];
+#pragma unroll
+        for (int i = 0; i < na; i++) p[i] = bp.ao_a + i * bp.str_a;
+#pragma unroll
+        for (int i = 0; i < nb; i++) q[i] = bp.ao_b + i * bp.str_b;
+#pragma unroll
+        for (int i = 0; i < nc; i++) r[i] = kp.ao_a + i * kp.str_a;
+#pragma unroll
+        for (int i = 0; i < nd; i++) t[i] = kp.ao_b + i * kp.str_b;
+        if (a.fj != 0.0) {
+#pragma unroll
+            for (int i = 0; i < na; i++)
+#pragma unroll
+                for (int jb = 0; jb < nb; jb++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int k = 0; k < nc; k++)
+#pragma unroll
+                        for (int l = 0; l < nd; l++)
+                            v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)r[k] * n + t[l]], v);
+                    v = wave_sum(v);
+                    if (lane == 0 && v != 0.0) unsafeAtomicAdd(&F[(size_t)p[i] * n + q[jb]], a.fj * v);
+                }
+            if (active) {
+#pragma unroll
+                for (int k = 0; k < nc; k++)
+#pragma unroll
+                    for (int l = 0; l < nd; l++) {
+                        double v = 0.0;
+#pragma unroll
+                        for (int i = 0; i < na; i++)
+#pragma unroll
+                            for (int jb = 0; jb < nb; jb++)
+                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)p[i] * n + q[jb]], v);
+                        unsafeAtomicAdd(&F[(size_t)r[k] * n + t[l]], a.fj * v);
+                    }
+            }
+        }
+        if (a.fk != 0.0 && active) {
+#pragma unroll
+            for (int i = 0; i < na; i++)
+#pragma unroll
+                for (int k = 0; k < nc; k++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int jb = 0; jb < nb; jb++)
+#pragma unroll
+                        for (int l = 0; l < nd; l++)
+                            v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)q[jb] * n + t[l]], v);
+                    unsafeAtomicAdd(&F[(size_t)p[i] * n + r[k]], a.fk * v);
+                }
+#pragma unroll
+            for (int i = 0; i < na; i++)
+#pragma unroll
+                for (int l = 0; l < nd; l++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int jb = 0; jb < nb; jb++)
+#pragma unroll
+                        for (int k = 0; k < nc; k++)
+                            v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)q[jb] * n + r[k]], v);
+                    unsafeAtomicAdd(&F[(size_t)p[i] * n + t[l]], a.fk * v);
+                }
+#pragma unroll
+            for (int jb = 0; jb < nb; jb++)
+#pragma unroll
+                for (int k = 0; k < nc; k++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int i = 0; i < na; i++)
+#pragma unroll
+                        for (int l = 0; l < nd; l++)
+                            v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)p[i] * n + t[l]], v);
+                    unsafeAtomicAdd(&F[(size_t)q[jb] * n + r[k]], a.fk * v);
+                }
+#pragma unroll
+            for (int jb = 0; jb < nb; jb++)
+#pragma unroll
+                for (int l = 0; l < nd; l++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int i = 0; i < na; i++)
+#pragma unroll
+                        for (int k = 0; k < nc; k++)
+                            v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)p[i] * n + r[k]], v);
+                    unsafeAtomicAdd(&F[(size_t)q[jb] * n + t[l]], a.fk * v);
+                }
+        }
+    }
+}
+
 // ------------------------------------------------- explicit quartet lists
 
 template <int la, int lb, int lc, int ld>
@@ -307,10 +438,11 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
     if (active) a.maxval[pi] = mx;
 }
 
-#define VLX_INSTANTIATE_CLASS(la, lb, lc, ld)                                                           \
+// kernel: fock_kernel (ket-stationary) or fock_kernel_bs (bra-stationary)
+#define VLX_INSTANTIATE_CLASS(la, lb, lc, ld, kernel)                                                   \
     void launch_fock_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                       \
     {                                                                                                   \
-        hipLaunchKernelGGL((fock_kernel<la, lb, lc, ld>), dim3(grid), dim3(VLX_BLOCK), 0, s, a.bra,     \
+        hipLaunchKernelGGL((kernel<la, lb, lc, ld>), dim3(grid), dim3(VLX_BLOCK), 0, s, a.bra,          \
                            a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F, a.sc);       \
     }                                                                                                   \
     void launch_eri_##la##lb##lc##ld(const EriArgs& a, hipStream_t s)                                   \

@@ -262,6 +262,13 @@ def gen_class(la, lb, lc, ld):
     return tag, '\n'.join(out), vrr_flops, hrr_flops
 
 
+def ket_stationary(la, lb, lc, ld):
+    """Digestion variant of a class: ket-stationary (sticky exchange
+    accumulators, few atomics) where the integrals are cheap and atomics
+    dominate; bra-stationary where the integrals need the register file."""
+    return lb == 0 and la + lb + lc + ld <= 4
+
+
 def pair_classes(lmax):
     return [(a, b) for a in range(lmax + 1) for b in range(a + 1)]
 
@@ -286,7 +293,8 @@ def main(outdir, lmax):
             '',
             body,
             '',
-            f'VLX_INSTANTIATE_CLASS({la}, {lb}, {lc}, {ld})',
+            f'VLX_INSTANTIATE_CLASS({la}, {lb}, {lc}, {ld}, '
+            f'{"fock_kernel" if ket_stationary(la, lb, lc, ld) else "fock_kernel_bs"})',
         ]
         if (la, lb) == (lc, ld):
             src.append(f'VLX_INSTANTIATE_DIAG({la}, {lb})')
@@ -306,7 +314,8 @@ def main(outdir, lmax):
         x = pcs.index((la, lb))
         y = pcs.index((lc, ld))
         nv, nh = flops[(la, lb, lc, ld)]
-        d.append(f'    {{{x}, {y}, {la}, {lb}, {lc}, {ld}, {nv}, {nh}, &launch_fock_{la}{lb}{lc}{ld}, '
+        ks = 1 if ket_stationary(la, lb, lc, ld) else 0
+        d.append(f'    {{{x}, {y}, {la}, {lb}, {lc}, {ld}, {nv}, {nh}, {ks}, &launch_fock_{la}{lb}{lc}{ld}, '
                  f'&launch_eri_{la}{lb}{lc}{ld}}},')
     d.append('};')
     d.append('static const DiagLauncher g_diag_table[] = {')

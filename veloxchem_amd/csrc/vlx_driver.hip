@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -127,12 +128,15 @@ struct ClassSchedule {
     DevBuf<WorkItem> d_items;
 };
 
-constexpr int BRA_CHUNK = 128;  // bras per work item
+constexpr int BRA_CHUNK = 128;  // default upper bound of bras per work item
 
 constexpr int NSIDE = 4;  // side streams that run class kernels concurrently
 
 struct vlx_context {
     int device = 0;
+    int nstreams = NSIDE;           // side streams in use (VLX_STREAMS, 1..NSIDE)
+    int max_chunk = BRA_CHUNK;      // bras per work item upper bound (VLX_BRA_CHUNK)
+    int min_items = 16384;          // work items a class launch aims for (VLX_MIN_ITEMS)
     hipStream_t stream = nullptr;
     hipStream_t side[NSIDE] = {};
     hipEvent_t ev_start = nullptr;
@@ -328,28 +332,44 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
             sch->quartets += hi - lo;
             sch->prim_quartets += (long long)ket.pairs[j].nprim * (rk.nprim_prefix[hi] - rk.nprim_prefix[lo]);
         }
-        // work items: 64 consecutive kets x a chunk of bras; the chunk shrinks
-        // for classes with little work so that the launch still has >= ~16k items
-        std::vector<std::pair<int, int>> ranges;
-        long long span = 0;
-        for (int j0 = 0; j0 < ket.size(); j0 += 64) {
-            const int bin_lo = unscreened ? 0 : ket.pairs[j0].bin;
-            int hi;
-            if (unscreened) {
-                hi = bra.size();
-            } else {
-                const int lim = ithr - bin_lo;
-                hi = (lim < 0) ? 0 : bra.cnt_le[std::min(lim, NBINS - 1)];
+        if (ce.ket_stationary) {
+            // work items: 64 consecutive kets x a chunk of bras; the chunk shrinks
+            // for classes with little work so that the launch still has enough items
+            std::vector<std::pair<int, int>> ranges;
+            long long span = 0;
+            for (int j0 = 0; j0 < ket.size(); j0 += 64) {
+                const int bin_lo = unscreened ? 0 : ket.pairs[j0].bin;
+                int hi;
+                if (unscreened) {
+                    hi = bra.size();
+                } else {
+                    const int lim = ithr - bin_lo;
+                    hi = (lim < 0) ? 0 : bra.cnt_le[std::min(lim, NBINS - 1)];
+                }
+                int lo = 0;
+                if (same && !unscreened) lo = (bin_lo > 0) ? bra.cnt_le[bin_lo - 1] : 0;
+                ranges.push_back({lo, hi});
+                span += std::max(0, hi - lo);
             }
-            int lo = 0;
-            if (same && !unscreened) lo = (bin_lo > 0) ? bra.cnt_le[bin_lo - 1] : 0;
-            ranges.push_back({lo, hi});
-            span += std::max(0, hi - lo);
+            const int chunk = static_cast<int>(
+                std::max<long long>(1, std::min<long long>(s->ctx->max_chunk, span / s->ctx->min_items)));
+            for (size_t t = 0; t < ranges.size(); t++)
+                for (int i0 = ranges[t].first; i0 < ranges[t].second; i0 += chunk)
+                    sch->items.push_back({static_cast<int>(64 * t), i0, std::min(ranges[t].second, i0 + chunk), 0});
+        } else {
+            // work items: one bra x 64 consecutive kets of the prefix its bins allow
+            for (int i = 0; i < bra.size(); i++) {
+                int nk;
+                if (unscreened) {
+                    nk = ket.size();
+                } else {
+                    const int lim = ithr - bra.pairs[i].bin;
+                    nk = (lim < 0) ? 0 : ket.cnt_le[std::min(lim, NBINS - 1)];
+                }
+                if (same) nk = std::min(nk, bra.pairs[i].rank + 1);
+                for (int j0 = 0; j0 < nk; j0 += 64) sch->items.push_back({j0, i, nk, 0});
+            }
         }
-        const int chunk = static_cast<int>(std::max<long long>(1, std::min<long long>(BRA_CHUNK, span / 16384)));
-        for (size_t t = 0; t < ranges.size(); t++)
-            for (int i0 = ranges[t].first; i0 < ranges[t].second; i0 += chunk)
-                sch->items.push_back({static_cast<int>(64 * t), i0, std::min(ranges[t].second, i0 + chunk), 0});
         if (sch->items.empty()) continue;
         HIP_TRY(sch->d_items.upload(sch->items));
         list.push_back(std::move(sch));
@@ -362,8 +382,9 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
     }
     std::sort(cost.begin(), cost.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
     double load[NSIDE] = {0.0};
+    const int nst = s->ctx->nstreams;
     for (auto& c : cost) {
-        const int k = static_cast<int>(std::min_element(load, load + NSIDE) - load);
+        const int k = static_cast<int>(std::min_element(load, load + nst) - load);
         c.second->stream = k;
         load[k] += c.first;
     }
@@ -401,6 +422,9 @@ int vlx_context_create(int device, vlx_context** out)
     if (device < 0 || device >= ndev) return fail(VLX_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
     auto ctx = std::make_unique<vlx_context>();
     ctx->device = device;
+    if (const char* e = std::getenv("VLX_STREAMS")) ctx->nstreams = std::max(1, std::min(NSIDE, std::atoi(e)));
+    if (const char* e = std::getenv("VLX_BRA_CHUNK")) ctx->max_chunk = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("VLX_MIN_ITEMS")) ctx->min_items = std::max(1, std::atoi(e));
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (int k = 0; k < NSIDE; k++) {

@@ -89,6 +89,7 @@ struct ClassLaunchers {
     int la, lb, lc, ld;
     int vrr_flops;        // generated VRR + contraction FLOPs per primitive quartet
     int hrr_flops;        // generated HRR + cart->sph FLOPs per contracted quartet
+    int ket_stationary;   // 1: fock_kernel (work items = ket tile x bra chunk); 0: fock_kernel_bs
     FockLaunchFn fock;
     EriLaunchFn eri;
 };
