@@ -57,6 +57,33 @@ def test_eri_blocks_match_oracle_all_classes():
     assert worst < 1e-12, worst
 
 
+def test_eri_blocks_tzvp_all_f_classes():
+    from veloxchem_amd import _native
+    if _native.lib().vlx_max_angular_momentum() < 3:
+        pytest.skip('f-shell kernels not built')
+    mol, bas = dimer('def2-tzvp')
+    scr = _screened(mol, bas)
+    ob = orc.basis_from(bas, mol)
+    ls = bas.shell_arrays(mol.get_coordinates_in_bohr())['l']
+    by_l = {l: [i for i, x in enumerate(ls) if x == l] for l in range(4)}
+    rng = np.random.default_rng(11)
+    quartets = []
+    for la in range(4):
+        for lb in range(4):
+            for lc in range(4):
+                for ld in range(4):
+                    if 3 not in (la, lb, lc, ld):
+                        continue
+                    for _ in range(2):
+                        quartets.append(tuple(int(rng.choice(by_l[x])) for x in (la, lb, lc, ld)))
+    blocks = FockDriver().eri_quartets(scr, quartets)
+    worst = 0.0
+    for q, blk in zip(quartets, blocks):
+        ref = orc.eri_sph(ob, *q)
+        worst = max(worst, float(np.max(np.abs(blk - ref))))
+    assert worst < 1e-12, worst
+
+
 def test_compute_eri_sto3g_fixture():
     # reference tests/test_fock_eri.py:26-36
     mol, bas = sto3g_water()

@@ -262,11 +262,21 @@ def gen_class(la, lb, lc, ld):
     return tag, '\n'.join(out), vrr_flops, hrr_flops
 
 
-def ket_stationary(la, lb, lc, ld):
-    """Digestion variant of a class: ket-stationary (sticky exchange
-    accumulators, few atomics) where the integrals are cheap and atomics
-    dominate; bra-stationary where the integrals need the register file."""
-    return lb == 0 and la + lb + lc + ld <= 4
+# classes whose straight-line recursion exceeds this many generated lines use
+# the wavefront-cooperative McMurchie-Davidson kernel (md_kernels.h) instead
+MD_LINES = 5500
+
+
+def variant(la, lb, lc, ld, nlines):
+    """Kernel of a class: 'md' (wavefront-cooperative, LDS) for the largest
+    classes; 'ks' (ket-stationary, sticky exchange accumulators, few atomics)
+    where the integrals are cheap and atomics dominate; 'bs'
+    (bra-stationary) where the integrals need most of the register file."""
+    if nlines > MD_LINES:
+        return 'md'
+    if lb == 0 and la + lb + lc + ld <= 4:
+        return 'ks'
+    return 'bs'
 
 
 def pair_classes(lmax):
@@ -284,20 +294,23 @@ def main(outdir, lmax):
     pcs = pair_classes(lmax)
     written = []
     flops = {}
+    variants = {}
     for (la, lb), (lc, ld) in class_pairs(lmax):
         tag, body, nv, nh = gen_class(la, lb, lc, ld)
         flops[(la, lb, lc, ld)] = (nv, nh)
-        src = [
-            '// generated by gen_eri.py -- do not edit',
-            '#include "../eri_kernels.h"',
-            '',
-            body,
-            '',
-            f'VLX_INSTANTIATE_CLASS({la}, {lb}, {lc}, {ld}, '
-            f'{"fock_kernel" if ket_stationary(la, lb, lc, ld) else "fock_kernel_bs"})',
-        ]
-        if (la, lb) == (lc, ld):
-            src.append(f'VLX_INSTANTIATE_DIAG({la}, {lb})')
+        var = variant(la, lb, lc, ld, body.count('\n'))
+        variants[(la, lb, lc, ld)] = var
+        if var == 'md':
+            src = ['// generated by gen_eri.py -- do not edit', '#include "../md_kernels.h"', '',
+                   f'VLX_INSTANTIATE_MD_CLASS({la}, {lb}, {lc}, {ld})']
+            if (la, lb) == (lc, ld):
+                src.append(f'VLX_INSTANTIATE_MD_DIAG({la}, {lb})')
+        else:
+            kern = 'fock_kernel' if var == 'ks' else 'fock_kernel_bs'
+            src = ['// generated by gen_eri.py -- do not edit', '#include "../eri_kernels.h"', '', body, '',
+                   f'VLX_INSTANTIATE_CLASS({la}, {lb}, {lc}, {ld}, {kern})']
+            if (la, lb) == (lc, ld):
+                src.append(f'VLX_INSTANTIATE_DIAG({la}, {lb})')
         path = outdir / f'eri_{tag}.hip'
         text = '\n'.join(src) + '\n'
         if not path.exists() or path.read_text() != text:
@@ -314,7 +327,7 @@ def main(outdir, lmax):
         x = pcs.index((la, lb))
         y = pcs.index((lc, ld))
         nv, nh = flops[(la, lb, lc, ld)]
-        ks = 1 if ket_stationary(la, lb, lc, ld) else 0
+        ks = {'bs': 'VLX_BRA_STATIONARY', 'ks': 'VLX_KET_STATIONARY', 'md': 'VLX_WAVE_MD'}[variants[(la, lb, lc, ld)]]
         d.append(f'    {{{x}, {y}, {la}, {lb}, {lc}, {ld}, {nv}, {nh}, {ks}, &launch_fock_{la}{lb}{lc}{ld}, '
                  f'&launch_eri_{la}{lb}{lc}{ld}}},')
     d.append('};')

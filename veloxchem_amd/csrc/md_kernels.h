@@ -1,0 +1,433 @@
+// Wavefront-cooperative electron-repulsion kernels for the largest classes
+// (f shells), where straight-line recursions would neither fit the register
+// file nor compile in reasonable time.
+//
+// One shell quartet at a time per wavefront (work-group = one wavefront),
+// McMurchie-Davidson in matrix form, everything staged in LDS:
+//   U_AB[ab][p]  = k_ab sum_{cart a,b} S_a S_b E^{ab,x}_t E^{ab,y}_u E^{ab,z}_v
+//                  (Hermite expansion of a bra primitive pair projected on the
+//                   real solid harmonics of src/general/SphericalMomentum.hpp)
+//   R[n][tuv]    = Hermite Coulomb integrals, by the recursion
+//                  R^n_{t+1,u,v} = t R^{n+1}_{t-1,u,v} + X_PQ R^{n+1}_{t,u,v}
+//                  from R^n_000 = (-2 rho)^n F_n(rho |PQ|^2)
+//   X[p][cd]     = sum_q R[p+q] (-1)^{|q|} U_CD[cd][q]
+//   out[ab][cd] += sum_p U_AB[ab][p] X[p][cd] / sqrt(zeta + eta)
+// with lanes parallel over matrix entries.  The spherical block in LDS is
+// then digested (Fock), stored (explicit quartets) or reduced (screener).
+#pragma once
+
+#include <algorithm>
+
+#include "eri_kernels.h"
+
+namespace md {
+
+constexpr int nherm(int l) { return (l + 1) * (l + 2) * (l + 3) / 6; }
+constexpr int D1 = VLX_MAX_L + 1;
+constexpr int DT = 2 * VLX_MAX_L + 1;
+constexpr int E3 = D1 * D1 * DT;  // 1D coefficients per axis; an E array holds 3 axes
+
+// linear index of Hermite (t,u,v): degree-major, then (t descending, u descending)
+__device__ __forceinline__ int hidx(int t, int u, int v)
+{
+    const int n = t + u + v, s = u + v;
+    return n * (n + 1) * (n + 2) / 6 + s * (s + 1) / 2 + v;
+}
+
+__device__ __forceinline__ void hdecode(int idx, int& t, int& u, int& v)
+{
+    int n = 0;
+    while (nherm(n) <= idx) n++;
+    const int rem = idx - (n > 0 ? nherm(n - 1) : 0);
+    int s = 0;
+    while ((s + 1) * (s + 2) / 2 <= rem) s++;
+    v = rem - s * (s + 1) / 2;
+    u = s - v;
+    t = n - s;
+}
+
+// Cartesian exponents of component c of shell l (xx, xy, xz, yy, yz, zz, ...)
+__device__ __forceinline__ void cart_exp(int l, int c, int& x, int& y, int& z)
+{
+    int i = 0;
+    for (int xx = l; xx >= 0; xx--)
+        for (int yy = l - xx; yy >= 0; yy--, i++)
+            if (i == c) {
+                x = xx;
+                y = yy;
+                z = l - xx - yy;
+                return;
+            }
+    x = y = z = 0;
+}
+
+// Spherical component m of shell l as Cartesian terms (index, factor); the
+// factors and orders are those of src/general/SphericalMomentum.hpp:18-134.
+__device__ __forceinline__ int sph_terms(int l, int m, int* ci, double* cf)
+{
+    const double f3 = 3.46410161513775458705;   // 2 sqrt(3)
+    const double g5 = 1.58113883008418966599;   // sqrt(5/2)
+    const double g15 = 7.74596669241483377036;  // 2 sqrt(15)
+    const double g3 = 1.22474487139158904910;   // sqrt(3/2)
+    switch (l * 8 + m) {
+    case 0: ci[0] = 0; cf[0] = 1.0; return 1;
+    case 8: ci[0] = 1; cf[0] = 1.0; return 1;
+    case 9: ci[0] = 2; cf[0] = 1.0; return 1;
+    case 10: ci[0] = 0; cf[0] = 1.0; return 1;
+    case 16: ci[0] = 1; cf[0] = f3; return 1;
+    case 17: ci[0] = 4; cf[0] = f3; return 1;
+    case 18: ci[0] = 0; cf[0] = -1.0; ci[1] = 3; cf[1] = -1.0; ci[2] = 5; cf[2] = 2.0; return 3;
+    case 19: ci[0] = 2; cf[0] = f3; return 1;
+    case 20: ci[0] = 0; cf[0] = 0.5 * f3; ci[1] = 3; cf[1] = -0.5 * f3; return 2;
+    case 24: ci[0] = 1; cf[0] = 3.0 * g5; ci[1] = 6; cf[1] = -g5; return 2;
+    case 25: ci[0] = 4; cf[0] = g15; return 1;
+    case 26: ci[0] = 8; cf[0] = 4.0 * g3; ci[1] = 1; cf[1] = -g3; ci[2] = 6; cf[2] = -g3; return 3;
+    case 27: ci[0] = 9; cf[0] = 2.0; ci[1] = 2; cf[1] = -3.0; ci[2] = 7; cf[2] = -3.0; return 3;
+    case 28: ci[0] = 5; cf[0] = 4.0 * g3; ci[1] = 0; cf[1] = -g3; ci[2] = 3; cf[2] = -g3; return 3;
+    case 29: ci[0] = 2; cf[0] = 0.5 * g15; ci[1] = 7; cf[1] = -0.5 * g15; return 2;
+    case 30: ci[0] = 0; cf[0] = g5; ci[1] = 3; cf[1] = -3.0 * g5; return 2;
+    default: return 0;
+    }
+}
+
+template <int la, int lb, int lc, int ld>
+struct Sizes {
+    static constexpr int L = la + lb + lc + ld;
+    static constexpr int nhab = nherm(la + lb), nhcd = nherm(lc + ld), nr = nherm(L);
+    static constexpr int nsa = 2 * la + 1, nsb = 2 * lb + 1, nsc = 2 * lc + 1, nsd = 2 * ld + 1;
+    static constexpr int nsab = nsa * nsb, nscd = nsc * nsd;
+    // LDS doubles: E (bra, ket), U_AB, U_CD, two R levels, X, out, scalars
+    static constexpr int lds = 6 * E3 + nsab * nhab + nhcd * nscd + 2 * nr + nhab * nscd + nsab * nscd + 32;
+};
+
+// 1D Hermite coefficients E^{ij}_t (i <= la, j <= lb) of one primitive pair for
+// axis x = threadIdx.x < 3; pa = P - A, pb = P - B.
+__device__ __forceinline__ void hermite_1d(int la, int lb, double p, const double* pa, const double* pb,
+                                           double* __restrict__ e)
+{
+    const int x = threadIdx.x;
+    if (x >= 3) return;
+    double* ex = e + x * E3;
+    for (int k = 0; k < E3; k++) ex[k] = 0.0;
+    const double h = 0.5 / p;
+    ex[0] = 1.0;
+    for (int i = 0; i <= la; i++)
+        for (int j = 0; j <= lb; j++) {
+            if (i == 0 && j == 0) continue;
+            for (int t = 0; t <= i + j; t++) {
+                double v;
+                if (j == 0) {
+                    const double* src = ex + ((i - 1) * D1 + j) * DT;
+                    v = pa[x] * src[t];
+                    if (t > 0) v += h * src[t - 1];
+                    if (t + 1 <= i - 1 + j) v += (t + 1) * src[t + 1];
+                } else {
+                    const double* src = ex + (i * D1 + (j - 1)) * DT;
+                    v = pb[x] * src[t];
+                    if (t > 0) v += h * src[t - 1];
+                    if (t + 1 <= i + j - 1) v += (t + 1) * src[t + 1];
+                }
+                ex[(i * D1 + j) * DT + t] = v;
+            }
+        }
+}
+
+// Bra: u[ab * nh + p]; ket (sign (-1)^{t+u+v}): u[p * ns + cd]; scaled by k.
+__device__ __forceinline__ void build_u(int la, int lb, const double* __restrict__ e, double k, bool ket,
+                                        double* __restrict__ u)
+{
+    const int nsb = 2 * lb + 1, ns = (2 * la + 1) * nsb, nh = nherm(la + lb);
+    for (int idx = threadIdx.x; idx < ns * nh; idx += blockDim.x) {
+        const int ab = idx / nh, ph = idx % nh;
+        int t, uu, v;
+        hdecode(ph, t, uu, v);
+        int cia[3], cib[3];
+        double cfa[3], cfb[3];
+        const int nta = sph_terms(la, ab / nsb, cia, cfa);
+        const int ntb = sph_terms(lb, ab % nsb, cib, cfb);
+        double acc = 0.0;
+        for (int ia = 0; ia < nta; ia++) {
+            int ax, ay, az;
+            cart_exp(la, cia[ia], ax, ay, az);
+            for (int ib = 0; ib < ntb; ib++) {
+                int bx, by, bz;
+                cart_exp(lb, cib[ib], bx, by, bz);
+                if (t > ax + bx || uu > ay + by || v > az + bz) continue;
+                acc += cfa[ia] * cfb[ib] * e[(ax * D1 + bx) * DT + t] * e[E3 + (ay * D1 + by) * DT + uu] *
+                       e[2 * E3 + (az * D1 + bz) * DT + v];
+            }
+        }
+        if (ket) {
+            if ((t + uu + v) & 1) acc = -acc;
+            u[ph * ns + ab] = k * acc;
+        } else {
+            u[ab * nh + ph] = k * acc;
+        }
+    }
+}
+
+// Cooperative spherical block (ab|cd) -> out[ab * nscd + cd] in LDS.
+// ab_vec = A - B, cd_vec = C - D (P - B = (P - A) + (A - B)).
+template <int la, int lb, int lc, int ld>
+__device__ void quartet(const PrimPair* __restrict__ bra_prims, int bra_np, const double* ab_vec,
+                        const PrimPair* __restrict__ ket_prims, int ket_np, const double* cd_vec,
+                        const double* __restrict__ tab, double* __restrict__ lds, double* __restrict__ out)
+{
+    using S = Sizes<la, lb, lc, ld>;
+    constexpr int L = S::L;
+    double* ebra = lds;
+    double* eket = ebra + 3 * E3;
+    double* uab = eket + 3 * E3;
+    double* ucd = uab + S::nsab * S::nhab;
+    double* ra = ucd + S::nhcd * S::nscd;
+    double* rb = ra + S::nr;
+    double* x = rb + S::nr;
+    double* scal = x + S::nhab * S::nscd;  // Boys values F_0..F_L
+    for (int k = threadIdx.x; k < S::nsab * S::nscd; k += blockDim.x) out[k] = 0.0;
+    for (int ib = 0; ib < bra_np; ib++) {
+        const PrimPair bp = bra_prims[ib];
+        {
+            const double pa[3] = {bp.pax, bp.pay, bp.paz};
+            const double pb[3] = {bp.pax + ab_vec[0], bp.pay + ab_vec[1], bp.paz + ab_vec[2]};
+            __syncthreads();
+            hermite_1d(la, lb, bp.zeta, pa, pb, ebra);
+            __syncthreads();
+            build_u(la, lb, ebra, bp.k, false, uab);
+        }
+        for (int ik = 0; ik < ket_np; ik++) {
+            const PrimPair kp = ket_prims[ik];
+            const double qc[3] = {kp.pax, kp.pay, kp.paz};
+            const double qd[3] = {kp.pax + cd_vec[0], kp.pay + cd_vec[1], kp.paz + cd_vec[2]};
+            __syncthreads();
+            hermite_1d(lc, ld, kp.zeta, qc, qd, eket);
+            const double s = bp.zeta + kp.zeta;
+            const double rho = bp.zeta * kp.zeta / s;
+            const double pq[3] = {bp.px - kp.px, bp.py - kp.py, bp.pz - kp.pz};
+            if (threadIdx.x == 3) {
+                double F[L + 1];
+                boys_eval<L>(rho * (pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2]), tab, F);
+                for (int m = 0; m <= L; m++) scal[m] = F[m];
+            }
+            __syncthreads();
+            build_u(lc, ld, eket, kp.k, true, ucd);
+            // R levels n = L .. 0 (ping-pong between ra and rb)
+            double* cur = (L & 1) ? rb : ra;
+            if (threadIdx.x == 0) {
+                double f = 1.0;
+                for (int m = 0; m < L; m++) f *= -2.0 * rho;
+                cur[0] = f * scal[L];
+            }
+            for (int n = L - 1; n >= 0; n--) {
+                double* prev = cur;
+                cur = (cur == ra) ? rb : ra;
+                __syncthreads();
+                double fn = 1.0;
+                for (int m = 0; m < n; m++) fn *= -2.0 * rho;
+                for (int idx = threadIdx.x; idx < nherm(L - n); idx += blockDim.x) {
+                    int t, u, v;
+                    hdecode(idx, t, u, v);
+                    double val;
+                    if (t > 0) {
+                        val = pq[0] * prev[hidx(t - 1, u, v)];
+                        if (t > 1) val += (t - 1) * prev[hidx(t - 2, u, v)];
+                    } else if (u > 0) {
+                        val = pq[1] * prev[hidx(t, u - 1, v)];
+                        if (u > 1) val += (u - 1) * prev[hidx(t, u - 2, v)];
+                    } else if (v > 0) {
+                        val = pq[2] * prev[hidx(t, u, v - 1)];
+                        if (v > 1) val += (v - 1) * prev[hidx(t, u, v - 2)];
+                    } else {
+                        val = fn * scal[n];
+                    }
+                    cur[idx] = val;
+                }
+            }
+            __syncthreads();
+            // X[p][cd] = sum_q R[p+q] U_CD[q][cd]
+            for (int idx = threadIdx.x; idx < S::nhab * S::nscd; idx += blockDim.x) {
+                const int p = idx / S::nscd, cd = idx % S::nscd;
+                int t, u, v;
+                hdecode(p, t, u, v);
+                double acc = 0.0;
+                for (int q = 0; q < S::nhcd; q++) {
+                    int tq, uq, vq;
+                    hdecode(q, tq, uq, vq);
+                    acc = fma(cur[hidx(t + tq, u + uq, v + vq)], ucd[q * S::nscd + cd], acc);
+                }
+                x[idx] = acc;
+            }
+            __syncthreads();
+            const double w = 1.0 / sqrt(s);
+            for (int idx = threadIdx.x; idx < S::nsab * S::nscd; idx += blockDim.x) {
+                const int ab = idx / S::nscd, cd = idx % S::nscd;
+                double acc = 0.0;
+                for (int p = 0; p < S::nhab; p++) acc = fma(uab[ab * S::nhab + p], x[p * S::nscd + cd], acc);
+                out[idx] += w * acc;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Fock digestion of one block in LDS (factor s included), lanes over outputs.
+template <int la, int lb, int lc, int ld>
+__device__ void digest(const PairInfo& bp, const PairInfo& kp, double s, const double* __restrict__ out,
+                       const double* __restrict__ D, double* __restrict__ F, const FockScalars& a)
+{
+    using S = Sizes<la, lb, lc, ld>;
+    constexpr int na = S::nsa, nb = S::nsb, nc = S::nsc, nd = S::nsd;
+    const int n = a.nao;
+    auto I = [&](int i, int j, int k, int l) { return out[(i * nb + j) * S::nscd + k * nd + l]; };
+    auto P = [&](int i) { return bp.ao_a + i * bp.str_a; };
+    auto Q = [&](int j) { return bp.ao_b + j * bp.str_b; };
+    auto R = [&](int k) { return kp.ao_a + k * kp.str_a; };
+    auto T = [&](int l) { return kp.ao_b + l * kp.str_b; };
+    const int lane = threadIdx.x;
+    if (a.fj != 0.0) {
+        for (int e = lane; e < na * nb; e += blockDim.x) {
+            const int i = e / nb, j = e % nb;
+            double v = 0.0;
+            for (int k = 0; k < nc; k++)
+                for (int l = 0; l < nd; l++) v = fma(I(i, j, k, l), D[(size_t)R(k) * n + T(l)], v);
+            unsafeAtomicAdd(&F[(size_t)P(i) * n + Q(j)], a.fj * s * v);
+        }
+        for (int e = lane; e < nc * nd; e += blockDim.x) {
+            const int k = e / nd, l = e % nd;
+            double v = 0.0;
+            for (int i = 0; i < na; i++)
+                for (int j = 0; j < nb; j++) v = fma(I(i, j, k, l), D[(size_t)P(i) * n + Q(j)], v);
+            unsafeAtomicAdd(&F[(size_t)R(k) * n + T(l)], a.fj * s * v);
+        }
+    }
+    if (a.fk != 0.0) {
+        for (int e = lane; e < na * nc; e += blockDim.x) {  // F'_ac += I D_bd
+            const int i = e / nc, k = e % nc;
+            double v = 0.0;
+            for (int j = 0; j < nb; j++)
+                for (int l = 0; l < nd; l++) v = fma(I(i, j, k, l), D[(size_t)Q(j) * n + T(l)], v);
+            unsafeAtomicAdd(&F[(size_t)P(i) * n + R(k)], a.fk * s * v);
+        }
+        for (int e = lane; e < na * nd; e += blockDim.x) {  // F'_ad += I D_bc
+            const int i = e / nd, l = e % nd;
+            double v = 0.0;
+            for (int j = 0; j < nb; j++)
+                for (int k = 0; k < nc; k++) v = fma(I(i, j, k, l), D[(size_t)Q(j) * n + R(k)], v);
+            unsafeAtomicAdd(&F[(size_t)P(i) * n + T(l)], a.fk * s * v);
+        }
+        for (int e = lane; e < nb * nc; e += blockDim.x) {  // F'_bc += I D_ad
+            const int j = e / nc, k = e % nc;
+            double v = 0.0;
+            for (int i = 0; i < na; i++)
+                for (int l = 0; l < nd; l++) v = fma(I(i, j, k, l), D[(size_t)P(i) * n + T(l)], v);
+            unsafeAtomicAdd(&F[(size_t)Q(j) * n + R(k)], a.fk * s * v);
+        }
+        for (int e = lane; e < nb * nd; e += blockDim.x) {  // F'_bd += I D_ac
+            const int j = e / nd, l = e % nd;
+            double v = 0.0;
+            for (int i = 0; i < na; i++)
+                for (int k = 0; k < nc; k++) v = fma(I(i, j, k, l), D[(size_t)P(i) * n + R(k)], v);
+            unsafeAtomicAdd(&F[(size_t)Q(j) * n + T(l)], a.fk * s * v);
+        }
+    }
+}
+
+}  // namespace md
+
+// Work item {j0, i0, i1}: bra i0 with kets [j0, i1) (processed one at a time).
+template <int la, int lb, int lc, int ld>
+__global__ void __launch_bounds__(VLX_WAVE)
+    fock_kernel_md(const PairInfo* __restrict__ bra, const PrimPair* __restrict__ bra_prims,
+                   const PairInfo* __restrict__ ket, const PrimPair* __restrict__ ket_prims,
+                   const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
+                   const double* __restrict__ D, double* __restrict__ F, const FockScalars a)
+{
+    using S = md::Sizes<la, lb, lc, ld>;
+    static_assert((S::lds + S::nsab * S::nscd + BOYS_TAB_SIZE) * 8 <= 160 * 1024, "LDS budget");
+    __shared__ double tab[BOYS_TAB_SIZE];
+    __shared__ double work[S::lds];
+    __shared__ double out[S::nsab * S::nscd];
+    stage_boys(tab, boys_tab);
+    for (long long it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+        const WorkItem wi = items[it];
+        const PairInfo bp = bra[wi.i0];
+        const double abv[3] = {bp.abx, bp.aby, bp.abz};
+        for (int j = wi.j0; j < wi.i1; j++) {
+            const PairInfo kp = ket[j];
+            const double cdv[3] = {kp.abx, kp.aby, kp.abz};
+            md::quartet<la, lb, lc, ld>(bra_prims + bp.prim_start, bp.nprim, abv, ket_prims + kp.prim_start,
+                                        kp.nprim, cdv, tab, work, out);
+            double s = 1.0;
+            if (bp.same) s *= 0.5;
+            if (kp.same) s *= 0.5;
+            if (a.same_class && bp.rank == j) s *= 0.5;
+            md::digest<la, lb, lc, ld>(bp, kp, s, out, D, F, a);
+            __syncthreads();
+        }
+    }
+}
+
+template <int la, int lb, int lc, int ld>
+__global__ void __launch_bounds__(VLX_WAVE) eri_out_kernel_md(const EriArgs a)
+{
+    using S = md::Sizes<la, lb, lc, ld>;
+    __shared__ double tab[BOYS_TAB_SIZE];
+    __shared__ double work[S::lds];
+    __shared__ double out[S::nsab * S::nscd];
+    stage_boys(tab, a.boys_tab);
+    for (int qi = blockIdx.x; qi < a.nquartets; qi += gridDim.x) {
+        const int2 qq = a.quartets[qi];
+        const PairInfo bp = a.bra[qq.x];
+        const PairInfo kp = a.ket[qq.y];
+        const double abv[3] = {bp.abx, bp.aby, bp.abz}, cdv[3] = {kp.abx, kp.aby, kp.abz};
+        md::quartet<la, lb, lc, ld>(a.bra_prims + bp.prim_start, bp.nprim, abv, a.ket_prims + kp.prim_start,
+                                    kp.nprim, cdv, tab, work, out);
+        for (int k = threadIdx.x; k < S::nsab * S::nscd; k += blockDim.x)
+            a.out[(size_t)qi * S::nsab * S::nscd + k] = out[k];
+        __syncthreads();
+    }
+}
+
+template <int la, int lb>
+__global__ void __launch_bounds__(VLX_WAVE) diag_kernel_md(const DiagArgs a)
+{
+    using S = md::Sizes<la, lb, la, lb>;
+    __shared__ double tab[BOYS_TAB_SIZE];
+    __shared__ double work[S::lds];
+    __shared__ double out[S::nsab * S::nscd];
+    stage_boys(tab, a.boys_tab);
+    for (int pi = blockIdx.x; pi < a.npairs; pi += gridDim.x) {
+        const PairInfo pp = a.pairs[pi];
+        const double abv[3] = {pp.abx, pp.aby, pp.abz};
+        md::quartet<la, lb, la, lb>(a.prims + pp.prim_start, pp.nprim, abv, a.prims + pp.prim_start, pp.nprim, abv,
+                                    tab, work, out);
+        double mx = 0.0;
+        for (int k = threadIdx.x; k < S::nsab * S::nscd; k += blockDim.x) mx = (mx < out[k]) ? out[k] : mx;
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(mx, off, VLX_WAVE);
+            mx = (mx < o) ? o : mx;
+        }
+        if (threadIdx.x == 0) a.maxval[pi] = mx;
+        __syncthreads();
+    }
+}
+
+#define VLX_INSTANTIATE_MD_CLASS(la, lb, lc, ld)                                                        \
+    void launch_fock_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                       \
+    {                                                                                                   \
+        (void)grid;                                                                                     \
+        const int g = static_cast<int>(std::min<long long>(a.sc.nitems, 4096));                         \
+        hipLaunchKernelGGL((fock_kernel_md<la, lb, lc, ld>), dim3(g), dim3(VLX_WAVE), 0, s, a.bra,      \
+                           a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F, a.sc);       \
+    }                                                                                                   \
+    void launch_eri_##la##lb##lc##ld(const EriArgs& a, hipStream_t s)                                   \
+    {                                                                                                   \
+        const int g = std::min(a.nquartets, 4096);                                                      \
+        hipLaunchKernelGGL((eri_out_kernel_md<la, lb, lc, ld>), dim3(g), dim3(VLX_WAVE), 0, s, a);      \
+    }
+
+#define VLX_INSTANTIATE_MD_DIAG(la, lb)                                                                 \
+    void launch_diag_##la##lb(const DiagArgs& a, hipStream_t s)                                         \
+    {                                                                                                   \
+        const int g = std::min(a.npairs, 4096);                                                         \
+        hipLaunchKernelGGL((diag_kernel_md<la, lb>), dim3(g), dim3(VLX_WAVE), 0, s, a);                 \
+    }

@@ -332,7 +332,7 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
             sch->quartets += hi - lo;
             sch->prim_quartets += (long long)ket.pairs[j].nprim * (rk.nprim_prefix[hi] - rk.nprim_prefix[lo]);
         }
-        if (ce.ket_stationary) {
+        if (ce.variant == VLX_KET_STATIONARY) {
             // work items: 64 consecutive kets x a chunk of bras; the chunk shrinks
             // for classes with little work so that the launch still has enough items
             std::vector<std::pair<int, int>> ranges;
@@ -357,7 +357,9 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
                 for (int i0 = ranges[t].first; i0 < ranges[t].second; i0 += chunk)
                     sch->items.push_back({static_cast<int>(64 * t), i0, std::min(ranges[t].second, i0 + chunk), 0});
         } else {
-            // work items: one bra x 64 consecutive kets of the prefix its bins allow
+            // work items: one bra x a run of the ket prefix its bins allow
+            // (64 kets, one per lane; or a few kets for the cooperative kernel)
+            const int kstep = (ce.variant == VLX_WAVE_MD) ? 4 : 64;
             for (int i = 0; i < bra.size(); i++) {
                 int nk;
                 if (unscreened) {
@@ -367,7 +369,8 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
                     nk = (lim < 0) ? 0 : ket.cnt_le[std::min(lim, NBINS - 1)];
                 }
                 if (same) nk = std::min(nk, bra.pairs[i].rank + 1);
-                for (int j0 = 0; j0 < nk; j0 += 64) sch->items.push_back({j0, i, nk, 0});
+                for (int j0 = 0; j0 < nk; j0 += kstep)
+                    sch->items.push_back({j0, i, (ce.variant == VLX_WAVE_MD) ? std::min(nk, j0 + kstep) : nk, 0});
             }
         }
         if (sch->items.empty()) continue;

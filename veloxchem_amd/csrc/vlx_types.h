@@ -84,12 +84,14 @@ using FockLaunchFn = void (*)(const FockArgs&, int, hipStream_t);
 using EriLaunchFn = void (*)(const EriArgs&, hipStream_t);
 using DiagLaunchFn = void (*)(const DiagArgs&, hipStream_t);
 
+enum { VLX_BRA_STATIONARY = 0, VLX_KET_STATIONARY = 1, VLX_WAVE_MD = 2 };
+
 struct ClassLaunchers {
     int x, y;             // bra / ket pair-class indices (x >= y)
     int la, lb, lc, ld;
     int vrr_flops;        // generated VRR + contraction FLOPs per primitive quartet
     int hrr_flops;        // generated HRR + cart->sph FLOPs per contracted quartet
-    int ket_stationary;   // 1: fock_kernel (work items = ket tile x bra chunk); 0: fock_kernel_bs
+    int variant;          // VLX_BRA_STATIONARY, VLX_KET_STATIONARY or VLX_WAVE_MD (work-item layout)
     FockLaunchFn fock;
     EriLaunchFn eri;
 };
