@@ -395,6 +395,199 @@ __global__ void __launch_bounds__(VLX_BLOCK)
     }
 }
 
+// LDS-tiled digestion.  A work-group takes one TileItem: bras of atom groups
+// (ga, gb) and up to 64 kets of groups (gc, gd).  Every F' contribution of
+// the item lands in six small dense tiles indexed by group-local AOs,
+//   J_AB, J_CD, K_AC, K_AD, K_BC, K_BD,
+// accumulated with LDS atomics and flushed to HBM once per item.  The four
+// wavefronts share the kets and split the bras.
+__device__ __forceinline__ void flush_tile(double* __restrict__ F, const int n, const double* __restrict__ tile,
+                                           const int nr, const int nc, const int* __restrict__ rows,
+                                           const int* __restrict__ cols, const double f)
+{
+    for (int e = threadIdx.x; e < nr * nc; e += blockDim.x) {
+        const double v = tile[e];
+        if (v != 0.0) unsafeAtomicAdd(&F[(size_t)rows[e / nc] * n + cols[e % nc]], f * v);
+    }
+}
+
+template <int la, int lb, int lc, int ld>
+__global__ void __launch_bounds__(VLX_BLOCK)
+    fock_kernel_tiled(const PairInfo* __restrict__ pairs_bra, const PrimPair* __restrict__ bra_prims,
+                      const PairInfo* __restrict__ pairs_ket, const PrimPair* __restrict__ ket_prims,
+                      const TileItem* __restrict__ items, const double* __restrict__ boys_tab,
+                      const double* __restrict__ D, double* __restrict__ F, const int* __restrict__ grp_off,
+                      const int* __restrict__ grp_n, const int* __restrict__ grp_ao, const FockScalars a)
+{
+    constexpr int na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
+    constexpr int nsph = na * nb * nc * nd;
+    __shared__ double tab[BOYS_TAB_SIZE];
+    extern __shared__ double tiles[];
+    stage_boys(tab, boys_tab);
+
+    const int n = a.nao;
+    const bool do_j = a.fj != 0.0, do_k = a.fk != 0.0;
+    const int lane = threadIdx.x & (VLX_WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / VLX_WAVE);
+
+    for (long long it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+        const TileItem ti = items[it];
+        const int nA = grp_n[ti.ga * 4 + la], nB = grp_n[ti.gb * 4 + lb];
+        const int nC = grp_n[ti.gc * 4 + lc], nD = grp_n[ti.gd * 4 + ld];
+        double* t_jab = tiles;
+        double* t_jcd = t_jab + nA * nB;
+        double* t_kac = t_jcd + nC * nD;
+        double* t_kad = t_kac + nA * nC;
+        double* t_kbc = t_kad + nA * nD;
+        double* t_kbd = t_kbc + nB * nC;
+        const int ntile = nA * nB + nC * nD + nA * nC + nA * nD + nB * nC + nB * nD;
+        for (int e = threadIdx.x; e < ntile; e += blockDim.x) tiles[e] = 0.0;
+        __syncthreads();
+
+        const int j = ti.j0 + lane;
+        const bool lane_ok = j < ti.j1;
+        const PairInfo kp = pairs_ket[lane_ok ? j : ti.j0];
+        int r[nc], t[nd], rl[nc], tl[nd];
+#pragma unroll
+        for (int k = 0; k < nc; k++) {
+            r[k] = kp.ao_a + k * kp.str_a;
+            rl[k] = kp.loc_a + k;
+        }
+#pragma unroll
+        for (int l = 0; l < nd; l++) {
+            t[l] = kp.ao_b + l * kp.str_b;
+            tl[l] = kp.loc_b + l;
+        }
+        double dcd[nc][nd];
+#pragma unroll
+        for (int k = 0; k < nc; k++)
+#pragma unroll
+            for (int l = 0; l < nd; l++) dcd[k][l] = D[(size_t)r[k] * n + t[l]];
+
+        for (int ib = ti.i0 + wave; ib < ti.i1; ib += VLX_WAVES_PER_BLOCK) {
+            const PairInfo bp = pairs_bra[ib];
+            bool active = lane_ok;
+            if (a.ithreshold >= 0) active = active && (bp.bin + kp.bin <= a.ithreshold);
+            if (a.same_class) active = active && (bp.rank >= kp.rank);
+            double s = active ? 1.0 : 0.0;
+            if (bp.same) s *= 0.5;
+            if (kp.same) s *= 0.5;
+            if (a.same_class && bp.rank == kp.rank) s *= 0.5;
+            double I[nsph];
+            eri_quartet<la, lb, lc, ld>(bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
+                                        ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz,
+                                        tab, I);
+#pragma unroll
+            for (int x = 0; x < nsph; x++) I[x] *= s;
+            int p[na], q[nb];
+#pragma unroll
+            for (int i = 0; i < na; i++) p[i] = bp.ao_a + i * bp.str_a;
+#pragma unroll
+            for (int jb = 0; jb < nb; jb++) q[jb] = bp.ao_b + jb * bp.str_b;
+            if (do_j) {
+#pragma unroll
+                for (int i = 0; i < na; i++)
+#pragma unroll
+                    for (int jb = 0; jb < nb; jb++) {
+                        double v = 0.0;
+#pragma unroll
+                        for (int k = 0; k < nc; k++)
+#pragma unroll
+                            for (int l = 0; l < nd; l++) v = fma(I[((i * nb + jb) * nc + k) * nd + l], dcd[k][l], v);
+                        v = wave_sum(v);
+                        if (lane == 0 && v != 0.0) unsafeAtomicAdd(&t_jab[(bp.loc_a + i) * nB + bp.loc_b + jb], v);
+                    }
+                if (active) {
+                    double vcd[nc][nd];
+#pragma unroll
+                    for (int k = 0; k < nc; k++)
+#pragma unroll
+                        for (int l = 0; l < nd; l++) vcd[k][l] = 0.0;
+#pragma unroll
+                    for (int i = 0; i < na; i++)
+#pragma unroll
+                        for (int jb = 0; jb < nb; jb++) {
+                            const double dab = D[(size_t)p[i] * n + q[jb]];
+#pragma unroll
+                            for (int k = 0; k < nc; k++)
+#pragma unroll
+                                for (int l = 0; l < nd; l++)
+                                    vcd[k][l] = fma(I[((i * nb + jb) * nc + k) * nd + l], dab, vcd[k][l]);
+                        }
+#pragma unroll
+                    for (int k = 0; k < nc; k++)
+#pragma unroll
+                        for (int l = 0; l < nd; l++) unsafeAtomicAdd(&t_jcd[rl[k] * nD + tl[l]], vcd[k][l]);
+                }
+            }
+            if (do_k && active) {
+#pragma unroll
+                for (int i = 0; i < na; i++) {
+#pragma unroll
+                    for (int k = 0; k < nc; k++) {  // K_ac += I D_bd
+                        double v = 0.0;
+#pragma unroll
+                        for (int jb = 0; jb < nb; jb++)
+#pragma unroll
+                            for (int l = 0; l < nd; l++)
+                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)q[jb] * n + t[l]], v);
+                        unsafeAtomicAdd(&t_kac[(bp.loc_a + i) * nC + rl[k]], v);
+                    }
+#pragma unroll
+                    for (int l = 0; l < nd; l++) {  // K_ad += I D_bc
+                        double v = 0.0;
+#pragma unroll
+                        for (int jb = 0; jb < nb; jb++)
+#pragma unroll
+                            for (int k = 0; k < nc; k++)
+                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)q[jb] * n + r[k]], v);
+                        unsafeAtomicAdd(&t_kad[(bp.loc_a + i) * nD + tl[l]], v);
+                    }
+                }
+#pragma unroll
+                for (int jb = 0; jb < nb; jb++) {
+#pragma unroll
+                    for (int k = 0; k < nc; k++) {  // K_bc += I D_ad
+                        double v = 0.0;
+#pragma unroll
+                        for (int i = 0; i < na; i++)
+#pragma unroll
+                            for (int l = 0; l < nd; l++)
+                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)p[i] * n + t[l]], v);
+                        unsafeAtomicAdd(&t_kbc[(bp.loc_b + jb) * nC + rl[k]], v);
+                    }
+#pragma unroll
+                    for (int l = 0; l < nd; l++) {  // K_bd += I D_ac
+                        double v = 0.0;
+#pragma unroll
+                        for (int i = 0; i < na; i++)
+#pragma unroll
+                            for (int k = 0; k < nc; k++)
+                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)p[i] * n + r[k]], v);
+                        unsafeAtomicAdd(&t_kbd[(bp.loc_b + jb) * nD + tl[l]], v);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const int* ra = grp_ao + grp_off[ti.ga * 4 + la];
+        const int* rb = grp_ao + grp_off[ti.gb * 4 + lb];
+        const int* rc = grp_ao + grp_off[ti.gc * 4 + lc];
+        const int* rd = grp_ao + grp_off[ti.gd * 4 + ld];
+        if (do_j) {
+            flush_tile(F, n, t_jab, nA, nB, ra, rb, a.fj);
+            flush_tile(F, n, t_jcd, nC, nD, rc, rd, a.fj);
+        }
+        if (do_k) {
+            flush_tile(F, n, t_kac, nA, nC, ra, rc, a.fk);
+            flush_tile(F, n, t_kad, nA, nD, ra, rd, a.fk);
+            flush_tile(F, n, t_kbc, nB, nC, rb, rc, a.fk);
+            flush_tile(F, n, t_kbd, nB, nD, rb, rd, a.fk);
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------- explicit quartet lists
 
 template <int la, int lb, int lc, int ld>
@@ -449,6 +642,15 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
     {                                                                                                   \
         const int grid = (a.nquartets + VLX_BLOCK - 1) / VLX_BLOCK;                                     \
         hipLaunchKernelGGL((eri_out_kernel<la, lb, lc, ld>), dim3(grid), dim3(VLX_BLOCK), 0, s, a);     \
+    }
+
+// LDS-tiled launcher (dynamic LDS = a.tile_lds_bytes); grid-stride over items
+#define VLX_INSTANTIATE_TILED(la, lb, lc, ld)                                                           \
+    void launch_tiled_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                      \
+    {                                                                                                   \
+        hipLaunchKernelGGL((fock_kernel_tiled<la, lb, lc, ld>), dim3(grid), dim3(VLX_BLOCK),            \
+                           a.tile_lds_bytes, s, a.bra, a.bra_prims, a.ket, a.ket_prims, a.titems,        \
+                           a.boys_tab, a.D, a.F, a.grp_off, a.grp_n, a.grp_ao, a.sc);                   \
     }
 
 #define VLX_INSTANTIATE_DIAG(la, lb)                                                                    \

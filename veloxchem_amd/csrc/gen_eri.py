@@ -279,6 +279,12 @@ def variant(la, lb, lc, ld, nlines):
     return 'bs'
 
 
+def tiled_eligible(la, lb, lc, ld):
+    """Classes that also get the LDS-tiled Fock kernel (cheap integrals, where
+    global atomics would dominate)."""
+    return la + lb + lc + ld <= 4
+
+
 def pair_classes(lmax):
     return [(a, b) for a in range(lmax + 1) for b in range(a + 1)]
 
@@ -311,6 +317,8 @@ def main(outdir, lmax):
                    f'VLX_INSTANTIATE_CLASS({la}, {lb}, {lc}, {ld}, {kern})']
             if (la, lb) == (lc, ld):
                 src.append(f'VLX_INSTANTIATE_DIAG({la}, {lb})')
+            if tiled_eligible(la, lb, lc, ld):
+                src.append(f'VLX_INSTANTIATE_TILED({la}, {lb}, {lc}, {ld})')
         path = outdir / f'eri_{tag}.hip'
         text = '\n'.join(src) + '\n'
         if not path.exists() or path.read_text() != text:
@@ -320,6 +328,8 @@ def main(outdir, lmax):
     d = ['// generated by gen_eri.py -- do not edit', f'#define VLX_GEN_LMAX {lmax}']
     for (la, lb, lc, ld) in written:
         d.append(f'VLX_DECLARE_CLASS({la}, {lb}, {lc}, {ld})')
+        if tiled_eligible(la, lb, lc, ld) and variants[(la, lb, lc, ld)] != 'md':
+            d.append(f'VLX_DECLARE_TILED({la}, {lb}, {lc}, {ld})')
         if (la, lb) == (lc, ld):
             d.append(f'VLX_DECLARE_DIAG({la}, {lb})')
     d.append('static const ClassLaunchers g_class_table[] = {')
@@ -328,8 +338,10 @@ def main(outdir, lmax):
         y = pcs.index((lc, ld))
         nv, nh = flops[(la, lb, lc, ld)]
         ks = {'bs': 'VLX_BRA_STATIONARY', 'ks': 'VLX_KET_STATIONARY', 'md': 'VLX_WAVE_MD'}[variants[(la, lb, lc, ld)]]
+        tl = (f'&launch_tiled_{la}{lb}{lc}{ld}'
+              if tiled_eligible(la, lb, lc, ld) and variants[(la, lb, lc, ld)] != 'md' else 'nullptr')
         d.append(f'    {{{x}, {y}, {la}, {lb}, {lc}, {ld}, {nv}, {nh}, {ks}, &launch_fock_{la}{lb}{lc}{ld}, '
-                 f'&launch_eri_{la}{lb}{lc}{ld}}},')
+                 f'&launch_eri_{la}{lb}{lc}{ld}, {tl}}},')
     d.append('};')
     d.append('static const DiagLauncher g_diag_table[] = {')
     for (la, lb) in pcs:

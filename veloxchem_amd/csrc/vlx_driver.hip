@@ -115,6 +115,11 @@ struct PairSet {
     std::vector<int> bins;
     int cnt_le[NBINS] = {0};            // pairs with bin <= b
     std::vector<long long> nprim_prefix;
+    struct Segment {
+        int start, end, ga, gb;
+        int cnt_le[NBINS];  // pairs of the segment with bin <= b
+    };
+    std::vector<Segment> segs;  // tiled order only: runs of equal (group A, group B)
     DevBuf<PairInfo> d_pairs;
     DevBuf<PrimPair> d_prims;
     int size() const { return static_cast<int>(pairs.size()); }
@@ -123,6 +128,10 @@ struct PairSet {
 struct ClassSchedule {
     int entry = -1;  // index into g_class_table
     int stream = 0;  // side stream the class runs on
+    bool tiled = false;
+    int tile_lds = 0;  // bytes of dynamic LDS (tiled)
+    std::vector<TileItem> titems;
+    DevBuf<TileItem> d_titems;
     std::vector<WorkItem> items;
     long long quartets = 0, prim_quartets = 0;
     DevBuf<WorkItem> d_items;
@@ -137,6 +146,9 @@ struct vlx_context {
     int nstreams = NSIDE;           // side streams in use (VLX_STREAMS, 1..NSIDE)
     int max_chunk = BRA_CHUNK;      // bras per work item upper bound (VLX_BRA_CHUNK)
     int min_items = 16384;          // work items a class launch aims for (VLX_MIN_ITEMS)
+    int use_tiled = 1;              // LDS-tiled kernels where built (VLX_TILED)
+    int group_atoms = 12;           // atoms per group of the tiled kernels (VLX_GROUP_ATOMS)
+    int max_tile_lds = 96 * 1024;   // larger tiles fall back to the untiled kernel
     hipStream_t stream = nullptr;
     hipStream_t side[NSIDE] = {};
     hipEvent_t ev_start = nullptr;
@@ -164,6 +176,15 @@ struct vlx_screener {
     std::vector<PairSet> all_bra;   // the same pairs sorted by shell A (bra order)
     std::vector<PairSet> kept;      // binned pairs sorted by (bin, nprim): ket order
     std::vector<PairSet> kept_bra;  // binned pairs sorted by (bin, shell A, nprim): bra order
+    std::vector<PairSet> kept_tiled;      // binned pairs sorted by (group A, group B, nprim, bin): ket order
+    std::vector<PairSet> kept_tiled_bra;  // the same sorted by (group A, group B, bin, nprim): bra order
+    std::vector<PairSet> all_tiled, all_tiled_bra;  // every pair, same two orders
+    // atom groups of the tiled kernels
+    std::vector<int> shell_group, shell_loc;  // per shell: group, local AO index of component 0
+    int ngroups = 0;
+    std::vector<int> grp_off, grp_n, grp_ao;  // (ngroups x 4) offsets/sizes, flat global AO lists
+    int grp_max[4] = {0, 0, 0, 0};           // largest (group, l) block per l
+    DevBuf<int> d_grp_off, d_grp_n, d_grp_ao;
     std::vector<double> pair_max;   // nshells x nshells
     struct PairRef {
         int cls = -1, idx = -1, swapped = 0;
@@ -211,6 +232,10 @@ void add_pair(const vlx_screener& s, PairSet& ps, int sa, int sb)
     pi.bin = 0;
     pi.rank = static_cast<int>(ps.pairs.size());
     pi.shell_a = sa;
+    pi.grp_a = s.shell_group.empty() ? 0 : s.shell_group[sa];
+    pi.grp_b = s.shell_group.empty() ? 0 : s.shell_group[sb];
+    pi.loc_a = s.shell_loc.empty() ? 0 : s.shell_loc[sa];
+    pi.loc_b = s.shell_loc.empty() ? 0 : s.shell_loc[sb];
     pi.abx = s.centers[3 * sa] - s.centers[3 * sb];
     pi.aby = s.centers[3 * sa + 1] - s.centers[3 * sb + 1];
     pi.abz = s.centers[3 * sa + 2] - s.centers[3 * sb + 2];
@@ -252,6 +277,55 @@ void reorder_set(const vlx_screener& s, const PairSet& src, const std::vector<in
         dst.bins.push_back(src.pairs[k].bin);
     }
     finalize_set(dst);
+}
+
+// Tiled orders of `src`: kets by (group A, group B, nprim, bin) so that lanes
+// of a chunk share primitive counts; bras by (group A, group B, bin, nprim) so
+// that the bras a chunk's bins allow are a prefix of each segment.  Both
+// carry the rank of the ket order.  Segments are the runs of equal (group A,
+// group B), identical in both orders.
+void build_tiled(const vlx_screener& s, const PairSet& src, PairSet& ket, PairSet& bra)
+{
+    std::vector<int> order(src.size());
+    std::iota(order.begin(), order.end(), 0);
+    auto by = [&](bool nprim_first) {
+        return [&, nprim_first](int i, int j) {
+            const PairInfo &x = src.pairs[i], &y = src.pairs[j];
+            if (x.grp_a != y.grp_a) return x.grp_a < y.grp_a;
+            if (x.grp_b != y.grp_b) return x.grp_b < y.grp_b;
+            if (nprim_first && x.nprim != y.nprim) return x.nprim < y.nprim;
+            if (x.bin != y.bin) return x.bin < y.bin;
+            return x.nprim < y.nprim;
+        };
+    };
+    std::stable_sort(order.begin(), order.end(), by(true));
+    reorder_set(s, src, order, ket);
+    std::vector<int> rank_of(src.size());
+    for (int k = 0; k < ket.size(); k++) {
+        ket.pairs[k].rank = k;
+        rank_of[order[k]] = k;
+    }
+    std::vector<int> border(src.size());
+    std::iota(border.begin(), border.end(), 0);
+    std::stable_sort(border.begin(), border.end(), by(false));
+    reorder_set(s, src, border, bra);
+    for (int k = 0; k < bra.size(); k++) bra.pairs[k].rank = rank_of[border[k]];
+    for (PairSet* ps : {&ket, &bra}) {
+        ps->segs.clear();
+        for (int k = 0; k < ps->size(); k++) {
+            const PairInfo& pk = ps->pairs[k];
+            if (ps->segs.empty() || ps->segs.back().ga != pk.grp_a || ps->segs.back().gb != pk.grp_b) {
+                PairSet::Segment sg{};
+                sg.start = k;
+                sg.ga = pk.grp_a;
+                sg.gb = pk.grp_b;
+                ps->segs.push_back(sg);
+            }
+            PairSet::Segment& sg = ps->segs.back();
+            sg.end = k + 1;
+            for (int b = std::max(pk.bin, 0); b < NBINS; b++) sg.cnt_le[b]++;
+        }
+    }
 }
 
 int upload_set(PairSet& ps)
@@ -332,7 +406,34 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
             sch->quartets += hi - lo;
             sch->prim_quartets += (long long)ket.pairs[j].nprim * (rk.nprim_prefix[hi] - rk.nprim_prefix[lo]);
         }
-        if (ce.variant == VLX_KET_STATIONARY) {
+        const int* gm = s->grp_max;
+        const int tile_lds = 8 * (gm[ce.la] * gm[ce.lb] + gm[ce.lc] * gm[ce.ld] + gm[ce.la] * gm[ce.lc] +
+                                   gm[ce.la] * gm[ce.ld] + gm[ce.lb] * gm[ce.lc] + gm[ce.lb] * gm[ce.ld]);
+        if (s->ctx->use_tiled && ce.tiled && tile_lds <= s->ctx->max_tile_lds) {
+            // tiled items: (bra segment prefix, chunk of <= 64 kets of a ket segment)
+            const PairSet& tb = unscreened ? s->all_tiled_bra[ce.x] : s->kept_tiled_bra[ce.x];
+            const PairSet& tk = unscreened ? s->all_tiled[ce.y] : s->kept_tiled[ce.y];
+            sch->tiled = true;
+            sch->tile_lds = tile_lds;
+            for (size_t sk = 0; sk < tk.segs.size(); sk++) {
+                const auto& ks = tk.segs[sk];
+                for (int j0 = ks.start; j0 < ks.end; j0 += 64) {
+                    const int j1 = std::min(ks.end, j0 + 64);
+                    int bin_lo = NBINS;
+                    for (int j = j0; j < j1; j++) bin_lo = std::min(bin_lo, std::max(tk.pairs[j].bin, 0));
+                    const int lim = unscreened ? NBINS - 1 : ithr - bin_lo;
+                    if (lim < 0) continue;
+                    // same class: bra segments at or after the ket segment (segment order is
+                    // identical in both lists); the diagonal segment is masked by rank
+                    for (size_t sb = same ? sk : 0; sb < tb.segs.size(); sb++) {
+                        const auto& bs = tb.segs[sb];
+                        const int i0 = bs.start;
+                        const int i1 = bs.start + (unscreened ? bs.end - bs.start : bs.cnt_le[std::min(lim, NBINS - 1)]);
+                        if (i1 > i0) sch->titems.push_back({i0, i1, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb});
+                    }
+                }
+            }
+        } else if (ce.variant == VLX_KET_STATIONARY) {
             // work items: 64 consecutive kets x a chunk of bras; the chunk shrinks
             // for classes with little work so that the launch still has enough items
             std::vector<std::pair<int, int>> ranges;
@@ -373,8 +474,12 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
                     sch->items.push_back({j0, i, (ce.variant == VLX_WAVE_MD) ? std::min(nk, j0 + kstep) : nk, 0});
             }
         }
-        if (sch->items.empty()) continue;
-        HIP_TRY(sch->d_items.upload(sch->items));
+        if (sch->items.empty() && sch->titems.empty()) continue;
+        if (sch->tiled) {
+            HIP_TRY(sch->d_titems.upload(sch->titems));
+        } else {
+            HIP_TRY(sch->d_items.upload(sch->items));
+        }
         list.push_back(std::move(sch));
     }
     // spread classes over the side streams, most expensive first (estimated FLOPs)
@@ -428,6 +533,8 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_STREAMS")) ctx->nstreams = std::max(1, std::min(NSIDE, std::atoi(e)));
     if (const char* e = std::getenv("VLX_BRA_CHUNK")) ctx->max_chunk = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("VLX_MIN_ITEMS")) ctx->min_items = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("VLX_TILED")) ctx->use_tiled = std::atoi(e) != 0;
+    if (const char* e = std::getenv("VLX_GROUP_ATOMS")) ctx->group_atoms = std::max(1, std::atoi(e));
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (int k = 0; k < NSIDE; k++) {
@@ -543,8 +650,58 @@ int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* 
     s->norms.assign(norms, norms + nprims_total);
     s->centers.assign(centers, centers + 3 * nshells);
 
+    // atoms: runs of shells with identical centres; groups: consecutive atoms
+    s->shell_group.assign(nshells, 0);
+    s->shell_loc.assign(nshells, 0);
+    {
+        int atom = 0, group = 0, atoms_in_group = 0;
+        std::vector<int> count(4, 0);
+        std::vector<std::vector<int>> ao_lists;  // per (group, l)
+        ao_lists.assign(4, {});
+        std::vector<std::vector<int>> all_lists;
+        for (int sh = 0; sh < nshells; sh++) {
+            const bool new_atom = sh > 0 && (centers[3 * sh] != centers[3 * sh - 3] ||
+                                             centers[3 * sh + 1] != centers[3 * sh - 2] ||
+                                             centers[3 * sh + 2] != centers[3 * sh - 1]);
+            if (new_atom) {
+                atom++;
+                atoms_in_group++;
+                if (atoms_in_group >= ctx->group_atoms) {
+                    for (int l = 0; l < 4; l++) all_lists.push_back(ao_lists[l]);
+                    ao_lists.assign(4, {});
+                    group++;
+                    atoms_in_group = 0;
+                }
+            }
+            s->shell_group[sh] = group;
+            s->shell_loc[sh] = static_cast<int>(ao_lists[l[sh]].size());
+            for (int m = 0; m < 2 * l[sh] + 1; m++) ao_lists[l[sh]].push_back(ao_base[sh] + m * ao_stride[sh]);
+        }
+        for (int ll = 0; ll < 4; ll++) all_lists.push_back(ao_lists[ll]);
+        s->ngroups = group + 1;
+        s->grp_off.assign(s->ngroups * 4, 0);
+        s->grp_n.assign(s->ngroups * 4, 0);
+        s->grp_ao.clear();
+        for (int g = 0; g < s->ngroups; g++)
+            for (int ll = 0; ll < 4; ll++) {
+                const auto& v = all_lists[g * 4 + ll];
+                s->grp_off[g * 4 + ll] = static_cast<int>(s->grp_ao.size());
+                s->grp_n[g * 4 + ll] = static_cast<int>(v.size());
+                s->grp_max[ll] = std::max(s->grp_max[ll], static_cast<int>(v.size()));
+                s->grp_ao.insert(s->grp_ao.end(), v.begin(), v.end());
+            }
+        (void)atom;
+        (void)count;
+        HIP_TRY(s->d_grp_off.upload(s->grp_off));
+        HIP_TRY(s->d_grp_n.upload(s->grp_n));
+        HIP_TRY(s->d_grp_ao.upload(s->grp_ao));
+    }
     const int npc = npair_classes();
     s->all.resize(npc);
+    s->kept_tiled.resize(npc);
+    s->all_tiled.resize(npc);
+    s->kept_tiled_bra.resize(npc);
+    s->all_tiled_bra.resize(npc);
     s->all_bra.resize(npc);
     s->kept.resize(npc);
     s->kept_bra.resize(npc);
@@ -552,7 +709,8 @@ int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* 
         for (int lb = 0; lb <= la; lb++) {
             s->all[pair_class_index(la, lb)].la = la;
             s->all[pair_class_index(la, lb)].lb = lb;
-            for (auto* v : {&s->all_bra, &s->kept, &s->kept_bra}) {
+            for (auto* v : {&s->all_bra, &s->kept, &s->kept_bra, &s->kept_tiled, &s->all_tiled, &s->kept_tiled_bra,
+                            &s->all_tiled_bra}) {
                 (*v)[pair_class_index(la, lb)].la = la;
                 (*v)[pair_class_index(la, lb)].lb = lb;
             }
@@ -623,7 +781,10 @@ int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* 
         std::iota(aorder.begin(), aorder.end(), 0);
         std::stable_sort(aorder.begin(), aorder.end(), [&](int i, int j) { return ps.shell_a[i] < ps.shell_a[j]; });
         reorder_set(*s, ps, aorder, s->all_bra[x]);
-        for (PairSet* up : {&ps, &ks, &s->kept_bra[x], &s->all_bra[x]}) {
+        build_tiled(*s, ks, s->kept_tiled[x], s->kept_tiled_bra[x]);
+        build_tiled(*s, ps, s->all_tiled[x], s->all_tiled_bra[x]);
+        for (PairSet* up : {&ps, &ks, &s->kept_bra[x], &s->all_bra[x], &s->kept_tiled[x], &s->all_tiled[x],
+                            &s->kept_tiled_bra[x], &s->all_tiled_bra[x]}) {
             rc = upload_set(*up);
             if (rc) return rc;
         }
@@ -768,8 +929,10 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
     for (const auto& schp : *list) {
         const ClassSchedule& sch = *schp;
         const auto& ce = g_class_table[sch.entry];
-        const PairSet& bra = unscreened ? scr->all_bra[ce.x] : scr->kept_bra[ce.x];
-        const PairSet& ket = unscreened ? scr->all[ce.y] : scr->kept[ce.y];
+        const PairSet& bra = sch.tiled ? (unscreened ? scr->all_tiled_bra[ce.x] : scr->kept_tiled_bra[ce.x])
+                                       : (unscreened ? scr->all_bra[ce.x] : scr->kept_bra[ce.x]);
+        const PairSet& ket = sch.tiled ? (unscreened ? scr->all_tiled[ce.y] : scr->kept_tiled[ce.y])
+                                       : (unscreened ? scr->all[ce.y] : scr->kept[ce.y]);
         FockArgs a{};
         a.bra = bra.d_pairs.ptr;
         a.bra_prims = bra.d_prims.ptr;
@@ -779,7 +942,12 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
         a.boys_tab = ctx->d_boys.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;
         a.D = d_density;
         a.F = ctx->d_faccum.ptr;
-        a.sc.nitems = static_cast<long long>(sch.items.size());
+        a.titems = sch.d_titems.ptr;
+        a.grp_off = scr->d_grp_off.ptr;
+        a.grp_n = scr->d_grp_n.ptr;
+        a.grp_ao = scr->d_grp_ao.ptr;
+        a.tile_lds_bytes = sch.tile_lds;
+        a.sc.nitems = static_cast<long long>(sch.tiled ? sch.titems.size() : sch.items.size());
         a.sc.nket = ket.size();
         a.sc.ithreshold = ithreshold;
         a.sc.same_class = (ce.x == ce.y) ? 1 : 0;
@@ -787,7 +955,8 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
         a.sc.fj = fj;
         a.sc.fk = fk;
         const long long waves = std::min<long long>(a.sc.nitems, 8192);
-        const int grid = static_cast<int>((waves + 3) / 4);
+        const int grid = sch.tiled ? static_cast<int>(std::min<long long>(a.sc.nitems, 4096))
+                                   : static_cast<int>((waves + 3) / 4);
         vlx_context::Pending pend{sch.entry, nullptr, nullptr};
         if (ctx->timing) {
             for (hipEvent_t* ev : {&pend.start, &pend.stop}) {
@@ -800,7 +969,8 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
             }
             HIP_TRY(hipEventRecord(pend.start, ctx->side[sch.stream]));
         }
-        ce.fock(a, grid, ctx->side[sch.stream]);
+        if (sch.tiled) ce.tiled(a, grid, ctx->side[sch.stream]);
+        else ce.fock(a, grid, ctx->side[sch.stream]);
         HIP_TRY(hipGetLastError());
         if (ctx->timing) {
             HIP_TRY(hipEventRecord(pend.stop, ctx->side[sch.stream]));

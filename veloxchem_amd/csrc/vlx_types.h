@@ -19,7 +19,7 @@ struct alignas(64) PrimPair {
     double k;             // sqrt(2) pi^(5/4) N_a N_b exp(-a b |AB|^2 / zeta) / zeta
 };
 
-// One contracted shell pair (A, B) with l_A >= l_B (64 B).
+// One contracted shell pair (A, B) with l_A >= l_B (80 B).
 struct alignas(16) PairInfo {
     int ao_a, str_a;  // AO index of component 0 of A and stride between components
     int ao_b, str_b;
@@ -29,7 +29,16 @@ struct alignas(16) PairInfo {
     int bin;          // Schwarz bin (0..15)
     int rank;         // position in the ket ordering of its pair class
     int shell_a;      // shell index of A
+    int grp_a, grp_b; // atom groups of A and B
+    int loc_a, loc_b; // local AO index of component 0 within (group, l) blocks
     double abx, aby, abz;  // A - B
+};
+
+// Tiled Fock work: bras [i0, i1) of atom groups (ga, gb) against kets
+// [j0, j1) (at most 64, one per lane) of atom groups (gc, gd).
+struct TileItem {
+    int i0, i1, j0, j1;
+    int ga, gb, gc, gd;
 };
 
 // One unit of Fock work: kets [j0, j0 + 64) of the ket list (one per lane)
@@ -55,6 +64,11 @@ struct FockArgs {
     const PairInfo* ket;           // ket list, sorted by (bin, primitive count)
     const PrimPair* ket_prims;
     const WorkItem* items;
+    const TileItem* titems;        // tiled variant
+    const int* grp_off;            // (ngroups x 4): offset of the (group, l) AO list in grp_ao
+    const int* grp_n;              // (ngroups x 4): number of AOs in the (group, l) block
+    const int* grp_ao;             // global AO index of each local AO
+    int tile_lds_bytes;            // dynamic LDS of the tiled variant
     const double* boys_tab;        // BOYS_TAB_SIZE doubles for L = la+lb+lc+ld
     const double* D;               // nao x nao, row-major
     double* F;                     // nao x nao accumulator F' (G = F' + F'^T)
@@ -84,7 +98,7 @@ using FockLaunchFn = void (*)(const FockArgs&, int, hipStream_t);
 using EriLaunchFn = void (*)(const EriArgs&, hipStream_t);
 using DiagLaunchFn = void (*)(const DiagArgs&, hipStream_t);
 
-enum { VLX_BRA_STATIONARY = 0, VLX_KET_STATIONARY = 1, VLX_WAVE_MD = 2 };
+enum { VLX_BRA_STATIONARY = 0, VLX_KET_STATIONARY = 1, VLX_WAVE_MD = 2, VLX_TILED = 3 };
 
 struct ClassLaunchers {
     int x, y;             // bra / ket pair-class indices (x >= y)
@@ -94,6 +108,7 @@ struct ClassLaunchers {
     int variant;          // VLX_BRA_STATIONARY, VLX_KET_STATIONARY or VLX_WAVE_MD (work-item layout)
     FockLaunchFn fock;
     EriLaunchFn eri;
+    FockLaunchFn tiled;   // LDS-tiled Fock kernel, or nullptr
 };
 
 struct DiagLauncher {
@@ -105,4 +120,5 @@ struct DiagLauncher {
 #define VLX_DECLARE_CLASS(la, lb, lc, ld)                                        \
     void launch_fock_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s); \
     void launch_eri_##la##lb##lc##ld(const EriArgs& a, hipStream_t s);
+#define VLX_DECLARE_TILED(la, lb, lc, ld) void launch_tiled_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);
 #define VLX_DECLARE_DIAG(la, lb) void launch_diag_##la##lb(const DiagArgs& a, hipStream_t s);
