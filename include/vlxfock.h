@@ -111,6 +111,13 @@ void* vlx_context_stream(vlx_context* ctx);
 int vlx_fock_work(vlx_screener* scr, int ithreshold, int max_records, int* nrecords, int* classes /* 4 per */,
                   long long* quartets, long long* prim_quartets, double* flops);
 
+/* Schedule efficiency of one build at ithreshold, per class record in the
+ * same order as vlx_fock_work: lane slots (work items x bras x 64 lanes, or
+ * per-lane quartets of the cooperative kernel) and primitive slots (the
+ * primitive-pair iterations the wavefronts execute, masked lanes included). */
+int vlx_fock_schedule_stats(vlx_screener* scr, int ithreshold, int max_records, int* nrecords,
+                            long long* lane_slots, long long* prim_slots);
+
 /* Per-class kernel timing with HIP events on the context's stream
  * (off by default).  vlx_context_class_times synchronises the stream and
  * returns, per class (la, lb, lc, ld), the summed kernel time in ms and the

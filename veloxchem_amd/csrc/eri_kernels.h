@@ -415,9 +415,10 @@ template <int la, int lb, int lc, int ld>
 __global__ void __launch_bounds__(VLX_BLOCK)
     fock_kernel_tiled(const PairInfo* __restrict__ pairs_bra, const PrimPair* __restrict__ bra_prims,
                       const PairInfo* __restrict__ pairs_ket, const PrimPair* __restrict__ ket_prims,
-                      const TileItem* __restrict__ items, const double* __restrict__ boys_tab,
-                      const double* __restrict__ D, double* __restrict__ F, const int* __restrict__ grp_off,
-                      const int* __restrict__ grp_n, const int* __restrict__ grp_ao, const FockScalars a)
+                      const TileItem* __restrict__ items, const int2* __restrict__ kentries,
+                      const double* __restrict__ boys_tab, const double* __restrict__ D, double* __restrict__ F,
+                      const int* __restrict__ grp_off, const int* __restrict__ grp_n,
+                      const int* __restrict__ grp_ao, const FockScalars a)
 {
     constexpr int na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
     constexpr int nsph = na * nb * nc * nd;
@@ -444,9 +445,12 @@ __global__ void __launch_bounds__(VLX_BLOCK)
         for (int e = threadIdx.x; e < ntile; e += blockDim.x) tiles[e] = 0.0;
         __syncthreads();
 
-        const int j = ti.j0 + lane;
-        const bool lane_ok = j < ti.j1;
-        const PairInfo kp = pairs_ket[lane_ok ? j : ti.j0];
+        // lane = one primitive pair of a ket pair: the contraction over ket
+        // primitives happens in the LDS tiles (every step after the VRR is linear)
+        const int e = ti.j0 + lane;
+        const bool lane_ok = e < ti.j1;
+        const int2 ke = kentries[lane_ok ? e : ti.j0];
+        const PairInfo kp = pairs_ket[ke.x];
         int r[nc], t[nd], rl[nc], tl[nd];
 #pragma unroll
         for (int k = 0; k < nc; k++) {
@@ -475,7 +479,7 @@ __global__ void __launch_bounds__(VLX_BLOCK)
             if (a.same_class && bp.rank == kp.rank) s *= 0.5;
             double I[nsph];
             eri_quartet<la, lb, lc, ld>(bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
-                                        ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz,
+                                        ket_prims + kp.prim_start + ke.y, active ? 1 : 0, kp.abx, kp.aby, kp.abz,
                                         tab, I);
 #pragma unroll
             for (int x = 0; x < nsph; x++) I[x] *= s;
@@ -650,7 +654,7 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
     {                                                                                                   \
         hipLaunchKernelGGL((fock_kernel_tiled<la, lb, lc, ld>), dim3(grid), dim3(VLX_BLOCK),            \
                            a.tile_lds_bytes, s, a.bra, a.bra_prims, a.ket, a.ket_prims, a.titems,        \
-                           a.boys_tab, a.D, a.F, a.grp_off, a.grp_n, a.grp_ao, a.sc);                   \
+                           a.kentries, a.boys_tab, a.D, a.F, a.grp_off, a.grp_n, a.grp_ao, a.sc);       \
     }
 
 #define VLX_INSTANTIATE_DIAG(la, lb)                                                                    \

@@ -120,6 +120,9 @@ struct PairSet {
         int cnt_le[NBINS];  // pairs of the segment with bin <= b
     };
     std::vector<Segment> segs;  // tiled order only: runs of equal (group A, group B)
+    std::vector<int2> kentries;           // tiled ket order: (pair, primitive) entries
+    std::vector<int> seg_e0, seg_e1;      // entry range of each segment
+    DevBuf<int2> d_kentries;
     DevBuf<PairInfo> d_pairs;
     DevBuf<PrimPair> d_prims;
     int size() const { return static_cast<int>(pairs.size()); }
@@ -127,6 +130,7 @@ struct PairSet {
 
 struct ClassSchedule {
     int entry = -1;  // index into g_class_table
+    long long lane_slots = 0, prim_slots = 0;  // schedule efficiency diagnostics
     int stream = 0;  // side stream the class runs on
     bool tiled = false;
     int tile_lds = 0;  // bytes of dynamic LDS (tiled)
@@ -279,11 +283,10 @@ void reorder_set(const vlx_screener& s, const PairSet& src, const std::vector<in
     finalize_set(dst);
 }
 
-// Tiled orders of `src`: kets by (group A, group B, nprim, bin) so that lanes
-// of a chunk share primitive counts; bras by (group A, group B, bin, nprim) so
-// that the bras a chunk's bins allow are a prefix of each segment.  Both
-// carry the rank of the ket order.  Segments are the runs of equal (group A,
-// group B), identical in both orders.
+// Tiled orders of `src`, both by (group A, group B, bin, nprim): the bras a
+// ket chunk's bins allow are a prefix of each segment.  Kets are expanded to
+// (pair, primitive) entries, one per lane.  Segments are the runs of equal
+// (group A, group B).
 void build_tiled(const vlx_screener& s, const PairSet& src, PairSet& ket, PairSet& bra)
 {
     std::vector<int> order(src.size());
@@ -298,7 +301,7 @@ void build_tiled(const vlx_screener& s, const PairSet& src, PairSet& ket, PairSe
             return x.nprim < y.nprim;
         };
     };
-    std::stable_sort(order.begin(), order.end(), by(true));
+    std::stable_sort(order.begin(), order.end(), by(false));
     reorder_set(s, src, order, ket);
     std::vector<int> rank_of(src.size());
     for (int k = 0; k < ket.size(); k++) {
@@ -310,6 +313,9 @@ void build_tiled(const vlx_screener& s, const PairSet& src, PairSet& ket, PairSe
     std::stable_sort(border.begin(), border.end(), by(false));
     reorder_set(s, src, border, bra);
     for (int k = 0; k < bra.size(); k++) bra.pairs[k].rank = rank_of[border[k]];
+    ket.kentries.clear();
+    for (int k = 0; k < ket.size(); k++)
+        for (int q = 0; q < ket.pairs[k].nprim; q++) ket.kentries.push_back(make_int2(k, q));
     for (PairSet* ps : {&ket, &bra}) {
         ps->segs.clear();
         for (int k = 0; k < ps->size(); k++) {
@@ -326,12 +332,22 @@ void build_tiled(const vlx_screener& s, const PairSet& src, PairSet& ket, PairSe
             for (int b = std::max(pk.bin, 0); b < NBINS; b++) sg.cnt_le[b]++;
         }
     }
+    // entry ranges of the ket segments
+    std::vector<long long> eoff(ket.size() + 1, 0);
+    for (int k = 0; k < ket.size(); k++) eoff[k + 1] = eoff[k] + ket.pairs[k].nprim;
+    ket.seg_e0.clear();
+    ket.seg_e1.clear();
+    for (const auto& sg : ket.segs) {
+        ket.seg_e0.push_back(static_cast<int>(eoff[sg.start]));
+        ket.seg_e1.push_back(static_cast<int>(eoff[sg.end]));
+    }
 }
 
 int upload_set(PairSet& ps)
 {
     HIP_TRY(ps.d_pairs.upload(ps.pairs));
     HIP_TRY(ps.d_prims.upload(ps.prims));
+    if (!ps.kentries.empty()) HIP_TRY(ps.d_kentries.upload(ps.kentries));
     return VLX_OK;
 }
 
@@ -417,10 +433,10 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
             sch->tile_lds = tile_lds;
             for (size_t sk = 0; sk < tk.segs.size(); sk++) {
                 const auto& ks = tk.segs[sk];
-                for (int j0 = ks.start; j0 < ks.end; j0 += 64) {
-                    const int j1 = std::min(ks.end, j0 + 64);
+                for (int j0 = tk.seg_e0[sk]; j0 < tk.seg_e1[sk]; j0 += 64) {
+                    const int j1 = std::min(tk.seg_e1[sk], j0 + 64);
                     int bin_lo = NBINS;
-                    for (int j = j0; j < j1; j++) bin_lo = std::min(bin_lo, std::max(tk.pairs[j].bin, 0));
+                    for (int j = j0; j < j1; j++) bin_lo = std::min(bin_lo, std::max(tk.pairs[tk.kentries[j].x].bin, 0));
                     const int lim = unscreened ? NBINS - 1 : ithr - bin_lo;
                     if (lim < 0) continue;
                     // same class: bra segments at or after the ket segment (segment order is
@@ -472,6 +488,32 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
                 if (same) nk = std::min(nk, bra.pairs[i].rank + 1);
                 for (int j0 = 0; j0 < nk; j0 += kstep)
                     sch->items.push_back({j0, i, (ce.variant == VLX_WAVE_MD) ? std::min(nk, j0 + kstep) : nk, 0});
+            }
+        }
+        // schedule efficiency: slots the wavefronts execute, masked lanes included
+        if (sch->tiled) {
+            const PairSet& tb = unscreened ? s->all_tiled_bra[ce.x] : s->kept_tiled_bra[ce.x];
+            const PairSet& tk = unscreened ? s->all_tiled[ce.y] : s->kept_tiled[ce.y];
+            (void)tk;
+            for (const auto& ti : sch->titems) {
+                sch->lane_slots += 64LL * (ti.i1 - ti.i0);
+                sch->prim_slots += (tb.nprim_prefix[ti.i1] - tb.nprim_prefix[ti.i0]) * 64;
+            }
+        } else if (ce.variant == VLX_KET_STATIONARY) {
+            for (const auto& wi : sch->items) {
+                int kmax = 0;
+                for (int j = wi.j0; j < std::min(wi.j0 + 64, ket.size()); j++) kmax = std::max(kmax, ket.pairs[j].nprim);
+                sch->lane_slots += 64LL * (wi.i1 - wi.i0);
+                sch->prim_slots += (long long)kmax * (bra.nprim_prefix[wi.i1] - bra.nprim_prefix[wi.i0]) * 64;
+            }
+        } else {
+            for (const auto& wi : sch->items) {
+                const int j1 = (ce.variant == VLX_WAVE_MD) ? wi.i1 : std::min(wi.j0 + 64, wi.i1);
+                int kmax = 0;
+                for (int j = wi.j0; j < j1; j++) kmax = std::max(kmax, ket.pairs[j].nprim);
+                const int lanes = (ce.variant == VLX_WAVE_MD) ? (j1 - wi.j0) : 64;
+                sch->lane_slots += lanes;
+                sch->prim_slots += (long long)kmax * bra.pairs[wi.i0].nprim * lanes;
             }
         }
         if (sch->items.empty() && sch->titems.empty()) continue;
@@ -817,6 +859,21 @@ int vlx_screener_number_of_pairs(const vlx_screener* scr, long long* out)
     return VLX_OK;
 }
 
+int vlx_fock_schedule_stats(vlx_screener* scr, int ithreshold, int max_records, int* nrecords,
+                            long long* lane_slots, long long* prim_slots)
+{
+    if (!scr || !nrecords) return fail(VLX_ERR_ARG, "NULL argument");
+    std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
+    int rc = get_schedules(scr, ithreshold, &list);
+    if (rc) return rc;
+    *nrecords = static_cast<int>(list->size());
+    for (int r = 0; r < std::min(max_records, *nrecords); r++) {
+        if (lane_slots) lane_slots[r] = (*list)[r]->lane_slots;
+        if (prim_slots) prim_slots[r] = (*list)[r]->prim_slots;
+    }
+    return VLX_OK;
+}
+
 int vlx_context_set_timing(vlx_context* ctx, int enable)
 {
     if (!ctx) return fail(VLX_ERR_ARG, "context is NULL");
@@ -943,6 +1000,7 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
         a.D = d_density;
         a.F = ctx->d_faccum.ptr;
         a.titems = sch.d_titems.ptr;
+        a.kentries = ket.d_kentries.ptr;
         a.grp_off = scr->d_grp_off.ptr;
         a.grp_n = scr->d_grp_n.ptr;
         a.grp_ao = scr->d_grp_ao.ptr;

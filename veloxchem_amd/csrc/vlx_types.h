@@ -34,8 +34,8 @@ struct alignas(16) PairInfo {
     double abx, aby, abz;  // A - B
 };
 
-// Tiled Fock work: bras [i0, i1) of atom groups (ga, gb) against kets
-// [j0, j1) (at most 64, one per lane) of atom groups (gc, gd).
+// Tiled Fock work: bras [i0, i1) of atom groups (ga, gb) against ket
+// primitive entries [j0, j1) (at most 64, one per lane) of atom groups (gc, gd).
 struct TileItem {
     int i0, i1, j0, j1;
     int ga, gb, gc, gd;
@@ -65,6 +65,7 @@ struct FockArgs {
     const PrimPair* ket_prims;
     const WorkItem* items;
     const TileItem* titems;        // tiled variant
+    const int2* kentries;          // tiled variant: (ket pair, primitive) per lane entry
     const int* grp_off;            // (ngroups x 4): offset of the (group, l) AO list in grp_ao
     const int* grp_n;              // (ngroups x 4): number of AOs in the (group, l) block
     const int* grp_ao;             // global AO index of each local AO

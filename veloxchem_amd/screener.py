@@ -92,5 +92,19 @@ class T4CScreener:
         return [(tuple(int(x) for x in cls[4 * i:4 * i + 4]), int(nq[i]), int(npq[i]), float(fl[i]))
                 for i in range(n.value)]
 
+    def schedule_stats(self, ithreshold):
+        """Per class (same order as work()): (lane slots, primitive slots) the kernels execute."""
+        L = _native.lib()
+        L.vlx_fock_schedule_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int), _native._c_ll_p, _native._c_ll_p]
+        n = ctypes.c_int()
+        _native.check(L.vlx_fock_schedule_stats(self.handle, int(ithreshold), 0, ctypes.byref(n), None, None))
+        ls = np.zeros(max(n.value, 1), dtype=np.int64)
+        ps = np.zeros(max(n.value, 1), dtype=np.int64)
+        _native.check(L.vlx_fock_schedule_stats(self.handle, int(ithreshold), n.value, ctypes.byref(n),
+                                                ls.ctypes.data_as(_native._c_ll_p),
+                                                ps.ctypes.data_as(_native._c_ll_p)))
+        return [(int(ls[i]), int(ps[i])) for i in range(n.value)]
+
     def __eq__(self, other):
         return isinstance(other, T4CScreener) and self._key == other._key
