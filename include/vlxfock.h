@@ -135,6 +135,20 @@ int vlx_context_class_times(vlx_context* ctx, int max_records, int* nrecords, in
  */
 int vlx_eri_quartets(vlx_context* ctx, vlx_screener* scr, int nquartets, const int* quartets, double* out);
 
+/*
+ * One-electron matrices of the SCF loop around the Fock build, host-side
+ * (O(N^2) work, threaded): overlap S, kinetic energy T and nuclear
+ * attraction V = -sum_C Z_C <a|1/|r - C||b> (electron charge included), each
+ * dense nao x nao row-major in the AO order of the shell table (same table as
+ * vlx_screener_create).  Replaces COverlapDriver::compute
+ * (src/t2c_overlap/OverlapDriver.hpp:50), CKineticEnergyDriver::compute
+ * (src/t2c_kinetic_energy/KineticEnergyDriver.hpp) and
+ * compute_nuclear_potential_integrals (src/pymodule/oneeints.py:71-95).
+ */
+int vlx_one_electron(int nshells, const int* l, const int* nprim, const int* prim_offset, const double* exps,
+                     const double* norms, const double* centers, const int* ao_base, const int* ao_stride, int nao,
+                     int natoms, const double* charges, const double* atom_xyz, double* S, double* T, double* V);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,9 @@ def lib():
         L.vlx_copy_to_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong]
         L.vlx_copy_to_host.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong]
         L.vlx_eri_quartets.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_double_p]
+        L.vlx_one_electron.argtypes = [ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_double_p, _c_double_p,
+                                       _c_double_p, _c_int_p, _c_int_p, ctypes.c_int, ctypes.c_int, _c_double_p,
+                                       _c_double_p, _c_double_p, _c_double_p, _c_double_p]
         _lib = L
     return _lib
 

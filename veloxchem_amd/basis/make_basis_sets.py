@@ -1,6 +1,6 @@
 """Re-encode the basis-set data this package ships as one JSON file.
 
-The published Gaussian basis sets (STO-3G, def2-SVP, def2-SVPD, def2-TZVP and
+The published Gaussian basis sets (STO-3G, def2-SVP, def2-SVPD, def2-TZVP, def2-QZVP and
 the minimal AO-START-GUESS set used by the SAD guess) are read from the plain
 text files of a VeloxChem checkout (format: ``@BASIS_SET name``, then per element
 ``@ATOMBASIS X`` blocks of shells ``L nprim 1`` followed by ``exponent coeff``
@@ -17,7 +17,7 @@ import sys
 from pathlib import Path
 
 ELEMENTS = ['H', 'HE', 'LI', 'BE', 'B', 'C', 'N', 'O', 'F', 'NE']
-BASIS_FILES = ['STO-3G', 'DEF2-SVP', 'DEF2-SVPD', 'DEF2-TZVP', 'AO-START-GUESS']
+BASIS_FILES = ['STO-3G', 'DEF2-SVP', 'DEF2-SVPD', 'DEF2-TZVP', 'DEF2-QZVP', 'AO-START-GUESS']
 ANGULAR = {'S': 0, 'P': 1, 'D': 2, 'F': 3, 'G': 4, 'H': 5, 'I': 6}
 
 

@@ -30,10 +30,11 @@ _ALIASES = {
     'DEF2-SVP': 'DEF2-SVP',
     'DEF2-SVPD': 'DEF2-SVPD',
     'DEF2-TZVP': 'DEF2-TZVP',
+    'DEF2-QZVP': 'DEF2-QZVP',
     'AO-START-GUESS': 'AO-START-GUESS',
 }
 
-MAX_ANGULAR_MOMENTUM = 3
+MAX_ANGULAR_MOMENTUM = 4  # basis data and one-electron integrals; the Fock build stops at f
 
 
 def _basis_library():
@@ -111,13 +112,50 @@ class MolecularBasis:
             shells = []
             for sh in lib[key]:
                 if sh['l'] > MAX_ANGULAR_MOMENTUM:
-                    raise ValueError('MolecularBasis: shells above f are not supported')
+                    raise ValueError('MolecularBasis: shells above g are not supported')
                 shells.append((sh['l'], list(sh['exps']),
                                normalize_contraction(sh['l'], sh['exps'], sh['coefs'])))
             atom_shells.append(shells)
         basis = MolecularBasis(atom_shells, name)
         basis._coords = molecule.get_coordinates_in_bohr()
+        basis._elem_ids = list(molecule.get_element_ids())
         return basis
+
+    def reduce_to_valence_basis(self):
+        """Keeps the shells with L up to the element's valence L (0 for H..Be, 1 for B..Ne).
+
+        Restates CAtomBasis::reduce_to_valence_basis (src/orbdata/AtomBasis.cpp:135-152)
+        with chem_elem::max_angular_momentum (src/moldata/ChemicalElement.cpp:70-81).
+        """
+        shells = []
+        for z, atom in zip(self._elem_ids, self._atom_shells):
+            mang = 0 if z < 5 else 1
+            shells.append([sh for sh in atom if sh[0] <= mang])
+        basis = MolecularBasis(shells, self._label + '(Valence)')
+        basis._coords = self._coords
+        basis._elem_ids = list(self._elem_ids)
+        return basis
+
+    def number_of_atom_basis_functions(self, atom, lang):
+        """Contracted functions of angular momentum lang on one atom."""
+        return sum(1 for sh in self._atom_shells[atom] if sh[0] == lang)
+
+    def atom_ao_indices(self):
+        """AO indices of each atom, in AO order (L, then component, then atom, then function).
+
+        Same order as SadGuessDriver.get_ao_indices_of_atoms
+        (src/pymodule/sadguessdriver.py:817-839).
+        """
+        natoms = len(self._atom_shells)
+        out = [[] for _ in range(natoms)]
+        ao = 0
+        for lang in range(self.max_angular_momentum() + 1):
+            for _ in range(2 * lang + 1):
+                for a in range(natoms):
+                    for _ in range(self.number_of_atom_basis_functions(a, lang)):
+                        out[a].append(ao)
+                        ao += 1
+        return out
 
     def get_label(self):
         return self._label

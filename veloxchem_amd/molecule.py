@@ -81,6 +81,18 @@ class Molecule:
     def number_of_electrons(self):
         return sum(self.get_element_ids()) - self._charge
 
+    def set_multiplicity(self, multiplicity):
+        self._multiplicity = int(multiplicity)
+
+    def get_multiplicity(self):
+        return getattr(self, '_multiplicity', 1)
+
+    def number_of_alpha_electrons(self):
+        return (self.number_of_electrons() + self.get_multiplicity() - 1) // 2
+
+    def number_of_beta_electrons(self):
+        return (self.number_of_electrons() - self.get_multiplicity() + 1) // 2
+
     def nuclear_repulsion_energy(self):
         z = np.array(self.get_element_ids(), dtype=np.float64)
         r = self._coords
