@@ -2,10 +2,10 @@
 
 Workload (one "step" = one build): seeded 32-water cluster in a 10 Angstrom
 box (seed 32), def2-SVP (768 AOs), FP64 G = 2J - K ('2jk', exchange factor
-1, omega 0, ithreshold 12).  The density is a seeded symmetric positive
-semi-definite pseudo-density (n_occ = 160); with density-independent Schwarz
-screening the work of a build does not depend on D.  D and G stay resident
-in HBM; the timed region is K builds between stream synchronisations.
+1, omega 0, ithreshold 12) on the SAD guess density.  D and G stay resident
+in HBM; the timed region is K builds between stream synchronisations.  After
+the timed builds, the full RHF SCF of the same cluster (L2-DIIS, gradient
+norm 1e-8) is run once and its wall time reported beside the build rate.
 
 Printed: one JSON line with value = builds/s, the roofline of the dominant
 class kernel (FP64 vector-ALU bound; per-launch algorithmic FLOPs over the
@@ -31,16 +31,28 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 
 def build_workload(nmol, box, seed, basis):
     from veloxchem_amd import MolecularBasis, T4CScreener
-    from veloxchem_amd.workloads import pseudo_density, water_cluster
+    from veloxchem_amd.sadguess import sad_density
+    from veloxchem_amd.workloads import water_cluster
     mol = water_cluster(nmol, box, seed)
     bas = MolecularBasis.read(mol, basis)
     t0 = time.perf_counter()
     scr = T4CScreener()
     scr.partition(bas, mol, 'eri')
     t_screen = time.perf_counter() - t0
-    n = bas.get_dimensions_of_basis()
-    dm = pseudo_density(n, 5 * nmol, seed=3)
+    dm = sad_density(mol, MolecularBasis.read(mol, 'ao-start-guess'), bas)
     return mol, bas, scr, dm, t_screen
+
+
+def run_scf(mol, bas, conv_thresh):
+    from veloxchem_amd.scfdriver import ScfRestrictedDriver
+    drv = ScfRestrictedDriver()
+    drv.conv_thresh = conv_thresh
+    res = drv.compute(mol, bas)
+    tm = res['timings']
+    return {'scf_time_s': round(tm['total'], 3), 'iterations': res['num_iter'], 'converged': res['is_converged'],
+            'energy': res['scf_energy'], 'conv_thresh': conv_thresh, 'fock_time_s': round(tm['fock'], 3),
+            'one_electron_s': round(tm['one_electron'], 3), 'linear_algebra_s': round(tm['linear_algebra'], 3),
+            'screener_s': round(tm['screener'], 3), 'precision': 'fp64'}
 
 
 def cpu_baseline(mol, bas, dm, ithr, target_s):
@@ -77,6 +89,8 @@ def main():
     ap.add_argument('--ithreshold', type=int, default=12)
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-scf', action='store_true')
+    ap.add_argument('--scf-conv', type=float, default=1.0e-8)
     ap.add_argument('--label', default='2jk', choices=['2jk', 'j', 'k'])
     args = ap.parse_args()
     if args.gpus != 1:
@@ -124,7 +138,8 @@ def main():
     build_tflops = flops_build / (ms_per_step * 1e-3) / 1e12
 
     result = {
-        'metric': 'J+K Fock builds/s (FP64, 2jk)',
+        'metric': 'J+K Fock builds/s + SCF time, mixed vs FP64 on GPU, at |\u0394E|<=1e-6 Eh',
+        'mode': 'fp64 2jk build (value); fp64 RHF SCF (scf)',
         'value': 1e3 / ms_per_step,
         'unit': 'builds/s',
         'n_gpus': 1,
@@ -135,10 +150,10 @@ def main():
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': 'f64',
-        'data': 'synthetic (seeded water cluster, seeded pseudo-density)',
+        'data': 'synthetic (seeded water cluster, SAD guess density)',
         'config': {
             'workload': f'{args.nmol}-water cluster (seed {args.seed}, {args.box} A box), {args.basis}, '
-                        f'{n} AOs, 2jk FP64 build, ithreshold {args.ithreshold}',
+                        f'{n} AOs, 2jk FP64 build on the SAD density, ithreshold {args.ithreshold}',
             'nao': n,
             'shell_quartets': int(sum(w[1] for w in work)),
             'primitive_quartets': int(sum(w[2] for w in work)),
@@ -160,10 +175,12 @@ def main():
         'class_ms': {'%d%d%d%d' % c: round(v[0] / v[1], 4) for c, v in sorted(times.items())},
         'checksum': float(np.sum(g * dm)),
     }
-    if not args.no_cpu_baseline:
-        result['cpu_baseline'] = cpu_baseline(mol, bas, dm, args.ithreshold, args.cpu_seconds)
     d_dm.free()
     d_g.free()
+    if not args.no_scf:
+        result['scf'] = run_scf(mol, bas, args.scf_conv)
+    if not args.no_cpu_baseline:
+        result['cpu_baseline'] = cpu_baseline(mol, bas, dm, args.ithreshold, args.cpu_seconds)
     print(json.dumps(result))
 
 

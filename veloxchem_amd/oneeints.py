@@ -33,9 +33,11 @@ def _shell_table(molecule, basis, ket_basis=None):
     return st, nao + ket_basis.get_dimensions_of_basis()
 
 
-def _one_electron(molecule, basis, charges=None, coordinates=None, ket_basis=None):
+def _one_electron(molecule, basis, charges=None, coordinates=None, ket_basis=None, need_v=True):
     st, nao = _shell_table(molecule, basis, ket_basis)
-    if charges is None and coordinates is None:
+    if not need_v:
+        charges, coordinates = np.zeros(0), np.zeros((0, 3))
+    elif charges is None and coordinates is None:
         charges = np.array(molecule.get_element_ids(), dtype=np.float64)
         coordinates = molecule.get_coordinates_in_bohr()
     elif charges is None or coordinates is None:
@@ -63,7 +65,7 @@ def compute_one_electron_integrals(molecule, basis):
 
 def compute_overlap_integrals(molecule, basis, ket_basis=None):
     """Overlap matrix; with ket_basis the nao(basis) x nao(ket_basis) bra/ket block."""
-    S = _one_electron(molecule, basis, ket_basis=ket_basis)[0]
+    S = _one_electron(molecule, basis, ket_basis=ket_basis, need_v=False)[0]
     if ket_basis is None:
         return S
     n = basis.get_dimensions_of_basis()
@@ -71,7 +73,7 @@ def compute_overlap_integrals(molecule, basis, ket_basis=None):
 
 
 def compute_kinetic_energy_integrals(molecule, basis):
-    return _one_electron(molecule, basis)[1]
+    return _one_electron(molecule, basis, need_v=False)[1]
 
 
 def compute_nuclear_potential_integrals(molecule, basis, charges=None, coordinates=None):

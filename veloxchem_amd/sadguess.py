@@ -18,7 +18,11 @@ the SCF starting point; the converged energy does not depend on it.
 
 import numpy as np
 
-from .oneeints import compute_overlap_integrals
+from .oneeints import _one_electron
+
+
+def _overlap_joint(molecule, basis, ket_basis):
+    return _one_electron(molecule, basis, ket_basis=ket_basis, need_v=False)[0]
 
 
 def occupation_numbers(elem_id, nelec):
@@ -35,8 +39,10 @@ def occupation_numbers(elem_id, nelec):
 
 def sad_density(molecule, min_basis, ao_basis):
     """Closed-shell alpha density (nao x nao) of the SAD guess."""
-    S12 = compute_overlap_integrals(molecule, min_basis, ao_basis)
-    S22 = compute_overlap_integrals(molecule, ao_basis)
+    # one pass over the joint (minimal + AO) shell table gives S12 and S22
+    S = _overlap_joint(molecule, min_basis, ao_basis)
+    n1 = min_basis.get_dimensions_of_basis()
+    S12, S22 = S[:n1, n1:], S[n1:, n1:]
     nao_1, nao_2 = S12.shape
     ao1 = min_basis.atom_ao_indices()
     ao2 = ao_basis.atom_ao_indices()
