@@ -40,25 +40,17 @@ def dimer():
     return Molecule.read_xyz_string(DIMER_XYZ)
 
 
-class _OracleFock:
-    """Fock provider with the FockDriver.compute signature, backed by the oracle (test double)."""
-
-    def compute(self, screener, dm, label, xf, omega, ithr):
-        from oracle import oracle as orc
-        g, _ = orc.fock(screener, dm.to_numpy(), label, xf, ithr, 0)
-
-        class _M:
-
-            def to_numpy(self):
-                return g
-
-        return _M()
-
-
 def _oracle_driver():
+    """SCF driven by the oracle's Fock build on the CPU (test double for the GPU build)."""
+    import torch
     from oracle import oracle as orc
+
+    def fock_2e(screener, D, thresh_int):
+        g, _ = orc.fock(screener, D.numpy(), '2jk', 1.0, thresh_int, 0)
+        return torch.from_numpy(g)
+
     drv = ScfRestrictedDriver()
-    drv.fock_driver = _OracleFock()
+    drv.fock_2e = fock_2e
     drv.screener_factory = lambda basis, mol: orc.basis_from(basis, mol)
     return drv
 

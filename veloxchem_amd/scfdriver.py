@@ -20,18 +20,22 @@ scfrestdriver.py, c2diis.py):
   over at most max_err_vecs Fock matrices (c2diis.py:45-243,
   scfrestdriver.py:133-200); MOs from eigh(X^T F X), density C_occ C_occ^T.
 
-The dense algebra (N x N products, eigh) runs in numpy on the host; the
-two-electron part is the only GPU work.
+The density, Fock and orthogonaliser stay on the GPU as float64 torch
+tensors (torch is used as device memory and for the dense N x N products and
+eigh); the two-electron part goes through vlx_fock_compute_device on the
+same buffers.  DIIS error vectors are formed once per stored Fock matrix
+(the reference recomputes them each iteration from the same F, D, S, X).
 """
 
+import ctypes
 import math
 import time
 from collections import deque
 
 import numpy as np
+import torch
 
-from .fockdriver import FockDriver
-from .matrix import make_matrix, mat_t
+from . import _native
 from .molecularbasis import MolecularBasis
 from .oneeints import compute_one_electron_integrals
 from .sadguess import sad_density
@@ -43,23 +47,46 @@ def c2diis_weights(errors):
 
     Eigenvectors of the B matrix B_ij = <e_i, e_j>, each rescaled to unit
     coefficient sum (skipped if the sum is below 1e-6), and the one whose
-    combined error vector has the smallest norm is taken (c2diis.py:156-243).
+    combined error vector has the smallest norm (w^T B w) is taken
+    (c2diis.py:156-243).  errors: sequence of arrays or tensors, or an
+    (n x n) B matrix given as ``bmat=``.
     """
-    n = len(errors)
-    B = np.empty((n, n))
-    for i in range(n):
-        for j in range(i, n):
-            B[i, j] = B[j, i] = np.vdot(errors[i], errors[j])
+    if isinstance(errors, dict):
+        B = np.asarray(errors['bmat'], dtype=np.float64)
+    else:
+        n = len(errors)
+        B = np.empty((n, n))
+        for i in range(n):
+            for j in range(i, n):
+                B[i, j] = B[j, i] = float(np.vdot(np.asarray(errors[i]), np.asarray(errors[j])))
+    n = B.shape[0]
     _, vecs = np.linalg.eigh(B)
     sums = vecs.sum(axis=0)
     cands = [vecs[:, k] / sums[k] for k in range(n) if abs(sums[k]) > 1.0e-6]
     best, fmin = cands[0], 1.0e8
     for w in cands:
-        e = sum(f * v for f, v in zip(w, errors))
-        r = np.vdot(e, e)
+        r = float(w @ B @ w)
         if fmin > r:
             fmin, best = r, w
     return best
+
+
+class _DeviceFock:
+    """G = 2J - K of a device density through vlx_fock_compute_device (same buffers, no host copies)."""
+
+    def __init__(self, device):
+        self.device = device
+        self.lib = _native.lib()
+        self.ctx = _native.context(device)
+
+    def __call__(self, screener, D, thresh_int):
+        G = torch.empty_like(D)
+        torch.cuda.current_stream(D.device).synchronize()
+        _native.check(self.lib.vlx_fock_compute_device(self.ctx, screener.handle, ctypes.c_void_p(D.data_ptr()),
+                                                        b'2jk', 1.0, 0.0, int(thresh_int),
+                                                        ctypes.c_void_p(G.data_ptr())))
+        _native.check(self.lib.vlx_context_synchronize(self.ctx))
+        return G
 
 
 class ScfRestrictedDriver:
@@ -75,8 +102,10 @@ class ScfRestrictedDriver:
         self.ovl_thresh = 1.0e-6
         self.diis_thresh = 1000.0
         self.eri_thresh = 1.0e-12
-        self.fock_driver = None
+        # fock_2e(screener, D, thresh_int) -> G on D's device; default: the GPU build
+        self.fock_2e = None
         self.screener_factory = None
+        self.device = None
         self.history = []
         self.timings = {}
         self._scf_energy = 0.0
@@ -112,23 +141,31 @@ class ScfRestrictedDriver:
         return nteri
 
     def compute(self, molecule, ao_basis, min_basis=None):
-        """Runs the SCF; returns a dict with 'scf_energy', 'C_alpha', 'E_alpha', 'D_alpha', 'F_alpha', 'S'."""
+        """Runs the SCF; returns a dict with 'scf_energy', 'C_alpha', 'E_alpha', 'D_alpha', 'F_alpha', 'S' (numpy)."""
         if molecule.get_multiplicity() != 1 or molecule.number_of_electrons() % 2:
             raise ValueError('ScfRestrictedDriver: closed-shell molecules only')
         if min_basis is None:
             min_basis = MolecularBasis.read(molecule, 'AO-START-GUESS')
-        if self.fock_driver is None:
-            self.fock_driver = FockDriver(self.comm)
+        if self.fock_2e is None:
+            dev = _native.current_device()
+            self.fock_2e = _DeviceFock(dev)
+            self.device = torch.device('cuda', dev)
+        elif self.device is None:
+            self.device = torch.device('cpu')
         t0 = time.perf_counter()
         self.history = []
-        self.timings = {'fock': 0.0, 'one_electron': 0.0, 'linear_algebra': 0.0, 'screener': 0.0}
+        self.timings = {'fock': 0.0, 'one_electron': 0.0, 'linear_algebra': 0.0, 'screener': 0.0, 'guess': 0.0}
         acc = self.acc_type.upper()
         if acc == 'DIIS':
+            t = time.perf_counter()
             den = sad_density(molecule, min_basis, ao_basis)
+            self.timings['guess'] += time.perf_counter() - t
             res = self._comp_diis(molecule, ao_basis, den, self.conv_thresh, self.max_iter)
         elif acc == 'L2_DIIS':
             val_basis = ao_basis.reduce_to_valence_basis()
+            t = time.perf_counter()
             den = sad_density(molecule, min_basis, val_basis)
+            self.timings['guess'] += time.perf_counter() - t
             first = self._comp_diis(molecule, val_basis, den, 1.0e-3, 5)
             den = self._project_density(molecule, ao_basis, val_basis, first['C_alpha'])
             res = self._comp_diis(molecule, ao_basis, den, self.conv_thresh, self.max_iter)
@@ -158,24 +195,19 @@ class ScfRestrictedDriver:
         nocc = molecule.number_of_alpha_electrons()
         return c[:, :nocc] @ c[:, :nocc].T
 
-    def _fock_2e(self, screener, den, thresh_int, basis):
-        dm = make_matrix(basis, mat_t.symmetric)
-        dm.set_values(den)
+    def _comp_diis(self, molecule, basis, den_np, conv_thresh, max_iter):
+        dev = self.device
         t = time.perf_counter()
-        g = self.fock_driver.compute(screener, dm, '2jk', 1.0, 0.0, thresh_int).to_numpy()
-        self.timings['fock'] += time.perf_counter() - t
-        return g
-
-    def _comp_diis(self, molecule, basis, den, conv_thresh, max_iter):
-        t = time.perf_counter()
-        S, T, V = compute_one_electron_integrals(molecule, basis)
+        S_np, T_np, V_np = compute_one_electron_integrals(molecule, basis)
         self.timings['one_electron'] += time.perf_counter() - t
-        hcore = T + V
-        e_nuc = molecule.nuclear_repulsion_energy()
         t = time.perf_counter()
-        w, U = np.linalg.eigh(S)
+        S = torch.from_numpy(S_np).to(dev)
+        hcore = torch.from_numpy(T_np + V_np).to(dev)
+        den = torch.from_numpy(np.ascontiguousarray(den_np)).to(dev)
+        e_nuc = molecule.nuclear_repulsion_energy()
+        w, U = torch.linalg.eigh(S)
         keep = w > self.ovl_thresh
-        X = U[:, keep] * (1.0 / np.sqrt(w[keep]))
+        X = (U[:, keep] * (1.0 / torch.sqrt(w[keep]))).contiguous()
         self.timings['linear_algebra'] += time.perf_counter() - t
         t = time.perf_counter()
         if self.screener_factory is None:
@@ -185,24 +217,27 @@ class ScfRestrictedDriver:
             screener = self.screener_factory(basis, molecule)
         self.timings['screener'] += time.perf_counter() - t
         nocc = molecule.number_of_alpha_electrons()
-        focks, dens = deque(), deque()
+        focks, errs = deque(), deque()
         e_grad = None
         e_old = 0.0
         self._is_converged = False
         C = eps = F = None
+        fmax = 1.0 / math.sqrt(self.ovl_thresh)
         for it in range(max_iter + 1):
             self._num_iter = it
             thresh = self.eri_thresh if e_grad is None else self._get_dyn_threshold(e_grad)
             thresh_int = int(-math.log10(thresh))
-            G = self._fock_2e(screener, den, thresh_int, basis)
             t = time.perf_counter()
-            e_el = np.sum(den * G) + 2.0 * np.sum(den * hcore)
+            G = self.fock_2e(screener, den, thresh_int)
+            self.timings['fock'] += time.perf_counter() - t
+            t = time.perf_counter()
+            e_el = float(torch.sum(den * G) + 2.0 * torch.sum(den * hcore))
             F = G + hcore
             fds = F @ den @ S
             err = X.T @ (fds - fds.T) @ X
-            e_grad = 2.0 * np.linalg.norm(err)
+            e_grad = 2.0 * float(torch.linalg.norm(err))
             e_scf = e_el + e_nuc
-            self.history.append({'energy': e_scf, 'gradient_norm': e_grad, 'max_gradient': np.max(np.abs(err)),
+            self.history.append({'energy': e_scf, 'gradient_norm': e_grad, 'max_gradient': float(err.abs().max()),
                                  'diff_energy': e_scf - e_old, 'eri_threshold': thresh_int})
             e_old = e_scf
             self._scf_energy = e_scf
@@ -213,30 +248,29 @@ class ScfRestrictedDriver:
             if e_grad < self.diis_thresh:
                 if len(focks) == self.max_err_vecs:
                     focks.popleft()
-                    dens.popleft()
-                focks.append(F.copy())
-                dens.append(den.copy())
+                    errs.popleft()
+                # the error vector of a stored (F, D) pair is this iteration's err
+                focks.append(F)
+                errs.append(err)
             if len(focks) > 1:
-                errs = []
-                for f, d in zip(focks, dens):
-                    x = f @ d @ S
-                    errs.append(X.T @ (x - x.T) @ X)
-                wts = c2diis_weights(errs)
-                f_eff = sum(wi * fi for wi, fi in zip(wts, focks))
+                E = torch.stack([e.reshape(-1) for e in errs])
+                wts = c2diis_weights({'bmat': (E @ E.T).cpu().numpy()})
+                f_eff = torch.tensordot(torch.from_numpy(np.ascontiguousarray(wts)).to(dev),
+                                        torch.stack(list(focks)), dims=1)
             elif len(focks) == 1:
-                f_eff = focks[0].copy()
+                f_eff = focks[0]
             else:
                 f_eff = F
-            eps, vecs = np.linalg.eigh(X.T @ f_eff @ X)
+            eps, vecs = torch.linalg.eigh(X.T @ f_eff @ X)
             C = X @ vecs
             # drop MOs with coefficients above 1/sqrt(ovl_thresh) (scfdriver.py:2724-2747)
-            big = np.max(np.abs(C), axis=0) > 1.0 / math.sqrt(self.ovl_thresh)
-            if np.any(big):
+            big = C.abs().amax(dim=0) > fmax
+            if bool(big.any()):
                 C, eps = C[:, ~big], eps[~big]
-            den = C[:, :nocc] @ C[:, :nocc].T
+            den = (C[:, :nocc] @ C[:, :nocc].T).contiguous()
             self.timings['linear_algebra'] += time.perf_counter() - t
         if C is None:
-            eps, vecs = np.linalg.eigh(X.T @ F @ X)
+            eps, vecs = torch.linalg.eigh(X.T @ F @ X)
             C = X @ vecs
-        return {'scf_energy': self._scf_energy, 'C_alpha': C, 'E_alpha': eps, 'D_alpha': den, 'F_alpha': F,
-                'S': S}
+        return {'scf_energy': self._scf_energy, 'C_alpha': C.cpu().numpy(), 'E_alpha': eps.cpu().numpy(),
+                'D_alpha': den.cpu().numpy(), 'F_alpha': F.cpu().numpy(), 'S': S_np}
