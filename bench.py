@@ -43,16 +43,21 @@ def build_workload(nmol, box, seed, basis):
     return mol, bas, scr, dm, t_screen
 
 
-def run_scf(mol, bas, conv_thresh):
+def run_scf(mol, bas, conv_thresh, precision='fp64', tau=0.0):
     from veloxchem_amd.scfdriver import ScfRestrictedDriver
     drv = ScfRestrictedDriver()
     drv.conv_thresh = conv_thresh
+    drv.precision = precision
+    drv.mixed_threshold = tau
     res = drv.compute(mol, bas)
     tm = res['timings']
-    return {'scf_time_s': round(tm['total'], 3), 'iterations': res['num_iter'], 'converged': res['is_converged'],
-            'energy': res['scf_energy'], 'conv_thresh': conv_thresh, 'fock_time_s': round(tm['fock'], 3),
-            'one_electron_s': round(tm['one_electron'], 3), 'linear_algebra_s': round(tm['linear_algebra'], 3),
-            'screener_s': round(tm['screener'], 3), 'precision': 'fp64'}
+    out = {'scf_time_s': round(tm['total'], 3), 'iterations': res['num_iter'], 'converged': res['is_converged'],
+           'energy': res['scf_energy'], 'conv_thresh': conv_thresh, 'fock_time_s': round(tm['fock'], 3),
+           'one_electron_s': round(tm['one_electron'], 3), 'linear_algebra_s': round(tm['linear_algebra'], 3),
+           'screener_s': round(tm['screener'], 3), 'guess_s': round(tm['guess'], 3), 'precision': precision}
+    if precision == 'mixed':
+        out['mixed_threshold'] = tau
+    return out
 
 
 def cpu_baseline(mol, bas, dm, ithr, target_s):
@@ -90,6 +95,8 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-scf', action='store_true')
+    ap.add_argument('--mixed', default='1e-6,1e-8,1e-10',
+                    help='Schwarz x max|D| thresholds of the mixed-precision sweep (comma list, empty: none)')
     ap.add_argument('--scf-conv', type=float, default=1.0e-8)
     ap.add_argument('--label', default='2jk', choices=['2jk', 'j', 'k'])
     args = ap.parse_args()
@@ -126,6 +133,28 @@ def main():
     _native.check(lib.vlx_context_set_timing(ctx, 0))
     times = _native.class_times(0)
     g = d_g.download()
+
+    # mixed-precision sweep on the same density: build rate and error vs the FP64 build
+    sweep = []
+    taus = [float(x) for x in args.mixed.split(',') if x.strip()]
+    for tau in taus:
+        _native.set_precision('mixed', tau)
+        for _ in range(max(1, args.warmup)):
+            one_build()
+        _native.check(lib.vlx_context_synchronize(ctx))
+        t0m = time.perf_counter()
+        for _ in range(args.steps):
+            one_build()
+        _native.check(lib.vlx_context_synchronize(ctx))
+        ms_m = (time.perf_counter() - t0m) * 1e3 / args.steps
+        gm = d_g.download()
+        k32 = _native.last_fp32_bin_sum()
+        p64, p32 = scr.precision_split(args.ithreshold, k32)
+        sweep.append({'threshold': tau, 'builds_per_s': 1e3 / ms_m, 'ms_per_step': ms_m, 'fp32_bin_sum': k32,
+                      'fp32_prim_fraction': p32 / max(1, p64 + p32),
+                      'max_abs_dG': float(np.max(np.abs(gm - g))),
+                      'dE2_vs_fp64': float(np.sum(dm * (gm - g)))})
+    _native.set_precision('fp64')
 
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     work = scr.work(args.ithreshold)
@@ -177,8 +206,15 @@ def main():
     }
     d_dm.free()
     d_g.free()
+    if sweep:
+        result['mixed_sweep'] = sweep
     if not args.no_scf:
         result['scf'] = run_scf(mol, bas, args.scf_conv)
+        if taus:
+            tau = max(taus)
+            mixed = run_scf(mol, bas, args.scf_conv, 'mixed', tau)
+            mixed['dE_vs_fp64'] = mixed['energy'] - result['scf']['energy']
+            result['scf_mixed'] = mixed
     if not args.no_cpu_baseline:
         result['cpu_baseline'] = cpu_baseline(mol, bas, dm, args.ithreshold, args.cpu_seconds)
     print(json.dumps(result))

@@ -46,6 +46,23 @@ void vlx_context_destroy(vlx_context* ctx);
 int vlx_set_block_size_factor(vlx_context* ctx, int factor);
 
 /*
+ * Precision of the recursions (no reference counterpart: the reference
+ * computes every quartet in FP64).  VLX_PRECISION_FP64 (default) keeps the
+ * FP64 path.  VLX_PRECISION_MIXED runs the Boys function and recursions of a
+ * quartet in FP32 when its Schwarz bound times max|D| of the build is below
+ * `threshold` (decided per tile of kets at the screener's decimal bins:
+ * bin_ab + bin_cd >= ceil(2 + log10(max|D| / threshold))), for the classes
+ * with the LDS-tiled kernel (la+lb+lc+ld <= 4); J/K accumulation stays FP64.
+ */
+enum { VLX_PRECISION_FP64 = 0, VLX_PRECISION_MIXED = 1 };
+int vlx_set_precision(vlx_context* ctx, int mode, double threshold);
+/* Bin-sum threshold the last build used (99 = no FP32 work). */
+int vlx_context_last_fp32_bin_sum(vlx_context* ctx, int* k32);
+/* Primitive slots of one build run in FP64 and in FP32 at bin-sum threshold k32. */
+int vlx_fock_precision_split(vlx_screener* scr, int ithreshold, int k32, long long* prim_slots_fp64,
+                             long long* prim_slots_fp32);
+
+/*
  * T4CScreener::partition(basis, molecule, "eri")
  * (src/t4c_func/T4CScreener.cpp:63-118, src/orbdata/BlockedGtoPairBlock.cpp:22-152).
  * Builds all shell pairs, computes max over components of (ab|ab) on the GPU,

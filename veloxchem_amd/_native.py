@@ -31,6 +31,14 @@ def lib():
     """The loaded library; raises if it has not been built."""
     global _lib
     if _lib is None:
+        # One HIP runtime per process: torch ships its own libamdhip64 (SONAME
+        # libamdhip64.so.7, loaded through the unversioned name).  Loaded after
+        # this library's /opt/rocm copy, it would be a second runtime that
+        # finds no GPU; loaded first, this library binds to the same one.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not LIB_PATH.exists():
             raise NativeError(f'{LIB_PATH} is missing: build it with __graft_entry__.build() '
                               '(the Fock build has no CPU fallback)')
@@ -63,6 +71,9 @@ def lib():
         L.vlx_copy_to_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong]
         L.vlx_copy_to_host.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong]
         L.vlx_eri_quartets.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_double_p]
+        L.vlx_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.vlx_context_last_fp32_bin_sum.argtypes = [ctypes.c_void_p, _c_int_p]
+        L.vlx_fock_precision_split.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_ll_p, _c_ll_p]
         L.vlx_one_electron.argtypes = [ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_double_p, _c_double_p,
                                        _c_double_p, _c_int_p, _c_int_p, ctypes.c_int, ctypes.c_int, _c_double_p,
                                        _c_double_p, _c_double_p, _c_double_p, _c_double_p]
@@ -94,6 +105,22 @@ def context(device=0):
         check(lib().vlx_context_create(int(device), ctypes.byref(h)))
         _contexts[device] = h
     return _contexts[device]
+
+
+PRECISION_MODES = {'fp64': 0, 'mixed': 1}
+
+
+def set_precision(mode='fp64', threshold=0.0, device=0):
+    """'fp64', or 'mixed': FP32 recursions for quartets whose Schwarz bound x max|D| < threshold."""
+    if mode not in PRECISION_MODES:
+        raise ValueError(f'unknown precision mode {mode!r}')
+    check(lib().vlx_set_precision(context(device), PRECISION_MODES[mode], float(threshold)))
+
+
+def last_fp32_bin_sum(device=0):
+    k = ctypes.c_int()
+    check(lib().vlx_context_last_fp32_bin_sum(context(device), ctypes.byref(k)))
+    return k.value
 
 
 def class_times(device=0):

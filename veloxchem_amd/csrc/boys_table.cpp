@@ -32,10 +32,12 @@ void vlx_build_boys_table(double* tab)
     for (int L = 0; L <= VLX_MAX_LTOT; L++)
         for (int k = 0; k < BOYS_NGRID; k++) {
             long double fact = 1.0L;
+            double* row = tab + (size_t)(L * BOYS_NGRID + k) * BOYS_ROW;
             for (int j = 0; j <= BOYS_ORDER; j++) {
                 if (j > 0) fact *= j;
                 const long double sgn = (j & 1) ? -1.0L : 1.0L;
-                tab[(L * BOYS_NGRID + k) * (BOYS_ORDER + 1) + j] = (double)(sgn * f[k][L + j] / fact);
+                row[j] = (double)(sgn * f[k][L + j] / fact);
             }
+            row[BOYS_ORDER + 1] = (double)expl(-(long double)k * BOYS_DT);
         }
 }

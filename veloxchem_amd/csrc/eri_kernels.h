@@ -22,11 +22,50 @@ constexpr int VLX_WAVE = 64;
 constexpr int VLX_BLOCK = 256;
 constexpr int VLX_WAVES_PER_BLOCK = VLX_BLOCK / VLX_WAVE;
 
+// Arithmetic helpers of the generated recursions, per real type.  The FP64
+// forms are plain IEEE operations; the FP32 forms use the hardware
+// reciprocal and exponential (1 ulp), which the mixed-precision path accepts
+// for quartets whose Schwarz bound is small.
+template <typename R> __device__ __forceinline__ R vdiv(R a, R b);
+template <> __device__ __forceinline__ double vdiv<double>(double a, double b) { return a / b; }
+template <> __device__ __forceinline__ float vdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <typename R> __device__ __forceinline__ R vsqrt(R x);
+template <> __device__ __forceinline__ double vsqrt<double>(double x) { return sqrt(x); }
+template <> __device__ __forceinline__ float vsqrt<float>(float x) { return __builtin_sqrtf(x); }
+template <typename R> __device__ __forceinline__ R vexp(R x);
+template <> __device__ __forceinline__ double vexp<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float vexp<float>(float x) { return __expf(x); }
+// 1/x and 1/sqrt(x) for the positive, normal arguments of the set-up
+// (zeta, eta, zeta + eta, T >= BOYS_TMAX): hardware estimate refined by two
+// Newton steps in FP64 (each step squares the relative error), the estimate
+// alone in FP32.
+template <typename R> __device__ __forceinline__ R vrcp(R x);
+template <> __device__ __forceinline__ double vrcp<double>(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+template <> __device__ __forceinline__ float vrcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <typename R> __device__ __forceinline__ R vrsqrt(R x);
+template <> __device__ __forceinline__ double vrsqrt<double>(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double e = fma(-h * y, y, 0.5);
+    y = fma(y, e, y);
+    e = fma(-h * y, y, 0.5);
+    return fma(y, e, y);
+}
+template <> __device__ __forceinline__ float vrsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Per-class primitive-quartet recursion, generated (one specialisation per
+// class): EriClass<la,lb,lc,ld>::quartet<R> writes the spherical block
+// (ab|cd) in real type R.
 template <int la, int lb, int lc, int ld>
-__device__ __forceinline__ void eri_quartet(const PrimPair* __restrict__ bra_prims, const int bra_np, const double ABx,
-                                            const double ABy, const double ABz, const PrimPair* __restrict__ ket_prims,
-                                            const int ket_np, const double CDx, const double CDy, const double CDz,
-                                            const double* __restrict__ boys_tab, double* __restrict__ out);
+struct EriClass;
 
 // Boys function F_0..F_L(T).  T < BOYS_TMAX: F_L from the order-BOYS_ORDER
 // Taylor expansion around the nearest grid point (table row k holds
@@ -34,36 +73,97 @@ __device__ __forceinline__ void eri_quartet(const PrimPair* __restrict__ bra_pri
 // F_m = (2T F_{m+1} + e^-T) / (2m+1).  T >= BOYS_TMAX: F_0 = sqrt(pi/T)/2
 // (erf(sqrt(T)) = 1 to double precision there) and upward recursion
 // F_{m+1} = ((2m+1) F_m - e^-T) / (2T), which is stable for T > L.
-template <int L>
-__device__ __forceinline__ void boys_eval(const double T, const double* __restrict__ tab, double* __restrict__ F)
+// (-1)^n / n!
+constexpr double exp_series_coef(int n)
 {
-    const double et = exp(-T);
-    if (T < BOYS_TMAX) {
-        const int k = static_cast<int>(T * BOYS_INV_DT + 0.5);
-        const double d = T - BOYS_DT * k;
-        const double* c = tab + k * (BOYS_ORDER + 1);
-        double f = c[BOYS_ORDER];
+    double c = 1.0;
+    for (int m = 1; m <= n; m++) c /= m;
+    return (n & 1) ? -c : c;
+}
+
+template <int L, typename R>
+__device__ __forceinline__ void boys_eval(const R T, const R* __restrict__ tab, R* __restrict__ F)
+{
+    // Taylor orders: full table order in FP64; 5 terms reach FP32 rounding
+    // (0.125^6 / 6! ~ 5e-9)
+    constexpr int ORD = (sizeof(R) == 8) ? BOYS_ORDER : 5;
+    if (T < R(BOYS_TMAX)) {
+        const int k = static_cast<int>(T * R(BOYS_INV_DT) + R(0.5));
+        const R d = T - R(BOYS_DT) * R(k);
+        const R* c = tab + k * BOYS_ROW;
+        R f = c[ORD];
 #pragma unroll
-        for (int j = BOYS_ORDER - 1; j >= 0; j--) f = fma(f, d, c[j]);
+        for (int j = ORD - 1; j >= 0; j--) f = fma(f, d, c[j]);
         F[L] = f;
+        if constexpr (L > 0) {
+            // exp(-T) = exp(-T_k) * sum_{n <= ORD} (-d)^n / n!, Horner in d
+            R s = R(exp_series_coef(ORD));
 #pragma unroll
-        for (int m = L - 1; m >= 0; m--) F[m] = (2.0 * T * F[m + 1] + et) * (1.0 / (2 * m + 1));
+            for (int n = ORD - 1; n >= 0; n--) s = fma(s, d, R(exp_series_coef(n)));
+            const R et = c[BOYS_ORDER + 1] * s;
+#pragma unroll
+            for (int m = L - 1; m >= 0; m--) F[m] = (R(2.0) * T * F[m + 1] + et) * R(1.0 / (2 * m + 1));
+        }
     } else {
-        const double it = 1.0 / T;
-        F[0] = 0.5 * sqrt(3.14159265358979323846 * it);
+        const R et = vexp<R>(-T);
+        const R rs = vrsqrt<R>(T);
+        const R it = rs * rs;
+        F[0] = R(0.88622692545275801365) * rs;  // sqrt(pi) / 2 / sqrt(T)
 #pragma unroll
-        for (int m = 0; m < L; m++) F[m + 1] = ((2 * m + 1) * F[m] - et) * (0.5 * it);
+        for (int m = 0; m < L; m++) F[m + 1] = (R(2 * m + 1) * F[m] - et) * (R(0.5) * it);
     }
 }
 
-__device__ __forceinline__ double wave_sum(double v)
+// FP64 quartet (all FP64 kernels)
+template <int la, int lb, int lc, int ld>
+__device__ __forceinline__ void eri_quartet(const PrimPair* __restrict__ bra_prims, const int bra_np, const double ABx,
+                                            const double ABy, const double ABz, const PrimPair* __restrict__ ket_prims,
+                                            const int ket_np, const double CDx, const double CDy, const double CDz,
+                                            const double* __restrict__ boys_tab, double* __restrict__ out)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, VLX_WAVE);
-    return v;
+    EriClass<la, lb, lc, ld>::template quartet<double>(bra_prims, bra_np, ABx, ABy, ABz, ket_prims, ket_np, CDx, CDy,
+                                                       CDz, boys_tab, out);
 }
 
-__device__ __forceinline__ void stage_boys(double* lds, const double* __restrict__ tab)
+// Cross-lane moves within rows of 16 lanes (DPP): quad_perm [1,0,3,2] (0xB1),
+// quad_perm [2,3,0,1] (0x4E), row_half_mirror (0x141), row_mirror (0x140).
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ float read_lane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ __forceinline__ double read_lane(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// Sum over the 64 lanes of a wavefront (all lanes must be active); the
+// result is wave-uniform.  Butterfly within each row of 16 lanes by DPP
+// moves, then the four row sums are read out with readlane.
+template <typename R>
+__device__ __forceinline__ R wave_sum(R v)
+{
+    v += dpp_move<0xB1>(v);
+    v += dpp_move<0x4E>(v);
+    v += dpp_move<0x141>(v);
+    v += dpp_move<0x140>(v);
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
+template <typename R>
+__device__ __forceinline__ void stage_boys(R* lds, const R* __restrict__ tab)
 {
     for (int t = threadIdx.x; t < BOYS_TAB_SIZE; t += blockDim.x) lds[t] = tab[t];
     __syncthreads();
@@ -411,20 +511,20 @@ __device__ __forceinline__ void flush_tile(double* __restrict__ F, const int n, 
     }
 }
 
-template <int la, int lb, int lc, int ld>
-__global__ void __launch_bounds__(VLX_BLOCK)
+template <int la, int lb, int lc, int ld, typename R, int NW>
+__global__ void __launch_bounds__(NW * VLX_WAVE)
     fock_kernel_tiled(const PairInfo* __restrict__ pairs_bra, const PrimPair* __restrict__ bra_prims,
                       const PairInfo* __restrict__ pairs_ket, const PrimPair* __restrict__ ket_prims,
                       const TileItem* __restrict__ items, const int2* __restrict__ kentries,
-                      const double* __restrict__ boys_tab, const double* __restrict__ D, double* __restrict__ F,
+                      const R* __restrict__ boys_tab, const double* __restrict__ D, double* __restrict__ F,
                       const int* __restrict__ grp_off, const int* __restrict__ grp_n,
                       const int* __restrict__ grp_ao, const FockScalars a)
 {
     constexpr int na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
     constexpr int nsph = na * nb * nc * nd;
-    __shared__ double tab[BOYS_TAB_SIZE];
+    __shared__ R tab[BOYS_TAB_SIZE];
     extern __shared__ double tiles[];
-    stage_boys(tab, boys_tab);
+    stage_boys<R>(tab, boys_tab);
 
     const int n = a.nao;
     const bool do_j = a.fj != 0.0, do_k = a.fk != 0.0;
@@ -462,25 +562,25 @@ __global__ void __launch_bounds__(VLX_BLOCK)
             t[l] = kp.ao_b + l * kp.str_b;
             tl[l] = kp.loc_b + l;
         }
-        double dcd[nc][nd];
+        R dcd[nc][nd];
 #pragma unroll
         for (int k = 0; k < nc; k++)
 #pragma unroll
-            for (int l = 0; l < nd; l++) dcd[k][l] = D[(size_t)r[k] * n + t[l]];
+            for (int l = 0; l < nd; l++) dcd[k][l] = R(D[(size_t)r[k] * n + t[l]]);
 
-        for (int ib = ti.i0 + wave; ib < ti.i1; ib += VLX_WAVES_PER_BLOCK) {
+        for (int ib = ti.i0 + wave; ib < ti.i1; ib += NW) {
             const PairInfo bp = pairs_bra[ib];
             bool active = lane_ok;
             if (a.ithreshold >= 0) active = active && (bp.bin + kp.bin <= a.ithreshold);
             if (a.same_class) active = active && (bp.rank >= kp.rank);
-            double s = active ? 1.0 : 0.0;
-            if (bp.same) s *= 0.5;
-            if (kp.same) s *= 0.5;
-            if (a.same_class && bp.rank == kp.rank) s *= 0.5;
-            double I[nsph];
-            eri_quartet<la, lb, lc, ld>(bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
-                                        ket_prims + kp.prim_start + ke.y, active ? 1 : 0, kp.abx, kp.aby, kp.abz,
-                                        tab, I);
+            R s = active ? R(1) : R(0);
+            if (bp.same) s *= R(0.5);
+            if (kp.same) s *= R(0.5);
+            if (a.same_class && bp.rank == kp.rank) s *= R(0.5);
+            R I[nsph];
+            EriClass<la, lb, lc, ld>::template quartet<R>(bra_prims + bp.prim_start, bp.nprim, R(bp.abx), R(bp.aby),
+                                                          R(bp.abz), ket_prims + kp.prim_start + ke.y, active ? 1 : 0,
+                                                          R(kp.abx), R(kp.aby), R(kp.abz), tab, I);
 #pragma unroll
             for (int x = 0; x < nsph; x++) I[x] *= s;
             int p[na], q[nb];
@@ -493,25 +593,26 @@ __global__ void __launch_bounds__(VLX_BLOCK)
                 for (int i = 0; i < na; i++)
 #pragma unroll
                     for (int jb = 0; jb < nb; jb++) {
-                        double v = 0.0;
+                        R v = R(0);
 #pragma unroll
                         for (int k = 0; k < nc; k++)
 #pragma unroll
                             for (int l = 0; l < nd; l++) v = fma(I[((i * nb + jb) * nc + k) * nd + l], dcd[k][l], v);
                         v = wave_sum(v);
-                        if (lane == 0 && v != 0.0) unsafeAtomicAdd(&t_jab[(bp.loc_a + i) * nB + bp.loc_b + jb], v);
+                        // one wave owns bra ib within the item: a plain store, no atomic
+                        if (lane == 0) t_jab[(bp.loc_a + i) * nB + bp.loc_b + jb] = static_cast<double>(v);
                     }
                 if (active) {
-                    double vcd[nc][nd];
+                    R vcd[nc][nd];
 #pragma unroll
                     for (int k = 0; k < nc; k++)
 #pragma unroll
-                        for (int l = 0; l < nd; l++) vcd[k][l] = 0.0;
+                        for (int l = 0; l < nd; l++) vcd[k][l] = R(0);
 #pragma unroll
                     for (int i = 0; i < na; i++)
 #pragma unroll
                         for (int jb = 0; jb < nb; jb++) {
-                            const double dab = D[(size_t)p[i] * n + q[jb]];
+                            const R dab = R(D[(size_t)p[i] * n + q[jb]]);
 #pragma unroll
                             for (int k = 0; k < nc; k++)
 #pragma unroll
@@ -521,7 +622,8 @@ __global__ void __launch_bounds__(VLX_BLOCK)
 #pragma unroll
                     for (int k = 0; k < nc; k++)
 #pragma unroll
-                        for (int l = 0; l < nd; l++) unsafeAtomicAdd(&t_jcd[rl[k] * nD + tl[l]], vcd[k][l]);
+                        for (int l = 0; l < nd; l++)
+                            unsafeAtomicAdd(&t_jcd[rl[k] * nD + tl[l]], static_cast<double>(vcd[k][l]));
                 }
             }
             if (do_k && active) {
@@ -529,46 +631,46 @@ __global__ void __launch_bounds__(VLX_BLOCK)
                 for (int i = 0; i < na; i++) {
 #pragma unroll
                     for (int k = 0; k < nc; k++) {  // K_ac += I D_bd
-                        double v = 0.0;
+                        R v = R(0);
 #pragma unroll
                         for (int jb = 0; jb < nb; jb++)
 #pragma unroll
                             for (int l = 0; l < nd; l++)
-                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)q[jb] * n + t[l]], v);
-                        unsafeAtomicAdd(&t_kac[(bp.loc_a + i) * nC + rl[k]], v);
+                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(D[(size_t)q[jb] * n + t[l]]), v);
+                        unsafeAtomicAdd(&t_kac[(bp.loc_a + i) * nC + rl[k]], static_cast<double>(v));
                     }
 #pragma unroll
                     for (int l = 0; l < nd; l++) {  // K_ad += I D_bc
-                        double v = 0.0;
+                        R v = R(0);
 #pragma unroll
                         for (int jb = 0; jb < nb; jb++)
 #pragma unroll
                             for (int k = 0; k < nc; k++)
-                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)q[jb] * n + r[k]], v);
-                        unsafeAtomicAdd(&t_kad[(bp.loc_a + i) * nD + tl[l]], v);
+                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(D[(size_t)q[jb] * n + r[k]]), v);
+                        unsafeAtomicAdd(&t_kad[(bp.loc_a + i) * nD + tl[l]], static_cast<double>(v));
                     }
                 }
 #pragma unroll
                 for (int jb = 0; jb < nb; jb++) {
 #pragma unroll
                     for (int k = 0; k < nc; k++) {  // K_bc += I D_ad
-                        double v = 0.0;
+                        R v = R(0);
 #pragma unroll
                         for (int i = 0; i < na; i++)
 #pragma unroll
                             for (int l = 0; l < nd; l++)
-                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)p[i] * n + t[l]], v);
-                        unsafeAtomicAdd(&t_kbc[(bp.loc_b + jb) * nC + rl[k]], v);
+                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(D[(size_t)p[i] * n + t[l]]), v);
+                        unsafeAtomicAdd(&t_kbc[(bp.loc_b + jb) * nC + rl[k]], static_cast<double>(v));
                     }
 #pragma unroll
                     for (int l = 0; l < nd; l++) {  // K_bd += I D_ac
-                        double v = 0.0;
+                        R v = R(0);
 #pragma unroll
                         for (int i = 0; i < na; i++)
 #pragma unroll
                             for (int k = 0; k < nc; k++)
-                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)p[i] * n + r[k]], v);
-                        unsafeAtomicAdd(&t_kbd[(bp.loc_b + jb) * nD + tl[l]], v);
+                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(D[(size_t)p[i] * n + r[k]]), v);
+                        unsafeAtomicAdd(&t_kbd[(bp.loc_b + jb) * nD + tl[l]], static_cast<double>(v));
                     }
                 }
             }
@@ -649,12 +751,22 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
     }
 
 // LDS-tiled launcher (dynamic LDS = a.tile_lds_bytes); grid-stride over items
-#define VLX_INSTANTIATE_TILED(la, lb, lc, ld)                                                           \
+// Tiled kernels: NWB waves per block (the item's bras split over them; the
+// LDS tiles are shared), or 4 waves when FockArgs::tiled_waves == 4.
+#define VLX_TILED_LAUNCH(la, lb, lc, ld, R, NW, tab)                                                    \
+    hipLaunchKernelGGL((fock_kernel_tiled<la, lb, lc, ld, R, NW>), dim3(grid), dim3((NW) * VLX_WAVE),  \
+                       a.tile_lds_bytes, s, a.bra, a.bra_prims, a.ket, a.ket_prims, a.titems, a.kentries,  \
+                       tab, a.D, a.F, a.grp_off, a.grp_n, a.grp_ao, a.sc)
+#define VLX_INSTANTIATE_TILED(la, lb, lc, ld, NWB)                                                      \
     void launch_tiled_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                      \
     {                                                                                                   \
-        hipLaunchKernelGGL((fock_kernel_tiled<la, lb, lc, ld>), dim3(grid), dim3(VLX_BLOCK),            \
-                           a.tile_lds_bytes, s, a.bra, a.bra_prims, a.ket, a.ket_prims, a.titems,        \
-                           a.kentries, a.boys_tab, a.D, a.F, a.grp_off, a.grp_n, a.grp_ao, a.sc);       \
+        if (a.tiled_waves == 4 || (NWB) == 4) VLX_TILED_LAUNCH(la, lb, lc, ld, double, 4, a.boys_tab);  \
+        else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, a.boys_tab);                                 \
+    }                                                                                                   \
+    void launch_tiled32_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                    \
+    {                                                                                                   \
+        if (a.tiled_waves == 4 || (NWB) == 4) VLX_TILED_LAUNCH(la, lb, lc, ld, float, 4, a.boys_tab_f); \
+        else VLX_TILED_LAUNCH(la, lb, lc, ld, float, NWB, a.boys_tab_f);                                \
     }
 
 #define VLX_INSTANTIATE_DIAG(la, lb)                                                                    \

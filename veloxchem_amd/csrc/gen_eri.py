@@ -89,7 +89,7 @@ class Code:
     def new(self, expr):
         name = f'{self.prefix}{self.n}'
         self.n += 1
-        self.lines.append(f'const double {name} = {expr};')
+        self.lines.append(f'const R {name} = {expr};')
         return name
 
 
@@ -112,7 +112,7 @@ def gen_vrr(code, la, lb, lc, ld, L):
             terms = [f'PA{AX[i]} * {node(am, c, m)}', f'WP{AX[i]} * {node(am, c, m + 1)}']
             if am[i] > 0:
                 amm = add(am, i, -1)
-                terms.append(f'{lit(am[i])} * oo2z * ({node(amm, c, m)} - rz * {node(amm, c, m + 1)})')
+                terms.append(f'R({lit(am[i])}) * oo2z * ({node(amm, c, m)} - rz * {node(amm, c, m + 1)})')
             memo[key] = code.new(' + '.join(terms))
             return memo[key]
         cands = [i for i in range(3) if c[i] > 0]
@@ -121,9 +121,9 @@ def gen_vrr(code, la, lb, lc, ld, L):
         terms = [f'QC{AX[i]} * {node(a, cm, m)}', f'WQ{AX[i]} * {node(a, cm, m + 1)}']
         if cm[i] > 0:
             cmm = add(cm, i, -1)
-            terms.append(f'{lit(cm[i])} * oo2e * ({node(a, cmm, m)} - re * {node(a, cmm, m + 1)})')
+            terms.append(f'R({lit(cm[i])}) * oo2e * ({node(a, cmm, m)} - re * {node(a, cmm, m + 1)})')
         if a[i] > 0:
-            terms.append(f'{lit(a[i])} * oo2ze * {node(add(a, i, -1), cm, m + 1)}')
+            terms.append(f'R({lit(a[i])}) * oo2ze * {node(add(a, i, -1), cm, m + 1)}')
         memo[key] = code.new(' + '.join(terms))
         return memo[key]
 
@@ -155,41 +155,44 @@ def gen_class(la, lb, lc, ld):
     w(f'// class ({la}{lb}|{lc}{ld}), L = {L}: {nacc} contracted [e0|f0] values, '
       f'{len(vrr.lines)} VRR intermediates per primitive quartet')
     w('template <>')
-    w(f'__device__ __forceinline__ void eri_quartet<{la}, {lb}, {lc}, {ld}>(')
-    w('    const PrimPair* __restrict__ bra_prims, const int bra_np, const double ABx, const double ABy, const double ABz,')
-    w('    const PrimPair* __restrict__ ket_prims, const int ket_np, const double CDx, const double CDy, const double CDz,')
-    w('    const double* __restrict__ boys_tab, double* __restrict__ out)')
+    w(f'struct EriClass<{la}, {lb}, {lc}, {ld}> {{')
+    w('template <typename R>')
+    w('static __device__ __forceinline__ void quartet(')
+    w('    const PrimPair* __restrict__ bra_prims, const int bra_np, const R ABx, const R ABy, const R ABz,')
+    w('    const PrimPair* __restrict__ ket_prims, const int ket_np, const R CDx, const R CDy, const R CDz,')
+    w('    const R* __restrict__ boys_tab, R* __restrict__ out)')
     w('{')
-    w(f'    double acc[{nacc}];')
+    w(f'    R acc[{nacc}];')
     w('#pragma unroll')
-    w(f'    for (int i = 0; i < {nacc}; i++) acc[i] = 0.0;')
+    w(f'    for (int i = 0; i < {nacc}; i++) acc[i] = R(0);')
     w('    for (int ik = 0; ik < ket_np; ik++) {')
     w('        const PrimPair kp = ket_prims[ik];')
-    w('        const double eta = kp.zeta;')
+    w('        const R eta = R(kp.zeta);')
     w('        for (int ib = 0; ib < bra_np; ib++) {')
     w('            const PrimPair bp = bra_prims[ib];')
-    w('            const double zeta = bp.zeta;')
-    w('            const double oo2z = 0.5 / zeta;')
-    w('            const double PAx = bp.pax, PAy = bp.pay, PAz = bp.paz;')
-    w('            const double inv_s = 1.0 / (zeta + eta);')
-    w('            const double rho = zeta * eta * inv_s;')
-    w('            const double PQx = bp.px - kp.px, PQy = bp.py - kp.py, PQz = bp.pz - kp.pz;')
-    w('            const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);')
-    w('            const double pref = bp.k * kp.k * sqrt(inv_s);')
-    w(f'            double F[{L + 1}];')
-    w(f'            boys_eval<{L}>(T, boys_tab, F);')
+    w('            const R zeta = R(bp.zeta);')
+    w('            const R oo2z = R(0.5) * vrcp<R>(zeta);')
+    w('            const R PAx = R(bp.pax), PAy = R(bp.pay), PAz = R(bp.paz);')
+    w('            const R rs = vrsqrt<R>(zeta + eta);  // (zeta + eta)^(-1/2)')
+    w('            const R inv_s = rs * rs;')
+    w('            const R rho = zeta * eta * inv_s;')
+    w('            const R PQx = R(bp.px - kp.px), PQy = R(bp.py - kp.py), PQz = R(bp.pz - kp.pz);')
+    w('            const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);')
+    w('            const R pref = R(bp.k * kp.k) * rs;')
+    w(f'            R F[{L + 1}];')
+    w(f'            boys_eval<{L}, R>(T, boys_tab, F);')
     for m in range(L + 1):
-        w(f'            const double b{m} = pref * F[{m}];')
+        w(f'            const R b{m} = pref * F[{m}];')
     if L > 0:
-        w('            const double re = zeta * inv_s, rz = eta * inv_s;')
+        w('            const R re = zeta * inv_s, rz = eta * inv_s;')
         # W - P = -(eta / (zeta + eta)) (P - Q),  W - Q = (zeta / (zeta + eta)) (P - Q)
-        w('            const double WPx = -rz * PQx, WPy = -rz * PQy, WPz = -rz * PQz;')
+        w('            const R WPx = -rz * PQx, WPy = -rz * PQy, WPz = -rz * PQz;')
         w('            (void)WPx; (void)WPy; (void)WPz; (void)rz; (void)re; (void)PAx; (void)PAy; (void)PAz; (void)oo2z;')
-        w('            const double WQx = re * PQx, WQy = re * PQy, WQz = re * PQz;')
+        w('            const R WQx = re * PQx, WQy = re * PQy, WQz = re * PQz;')
         w('            (void)WQx; (void)WQy; (void)WQz;')
-        w('            const double QCx = kp.pax, QCy = kp.pay, QCz = kp.paz;')
+        w('            const R QCx = R(kp.pax), QCy = R(kp.pay), QCz = R(kp.paz);')
         w('            (void)QCx; (void)QCy; (void)QCz;')
-        w('            const double oo2e = 0.5 / eta, oo2ze = 0.5 * inv_s;')
+        w('            const R oo2e = R(0.5) * vrcp<R>(eta), oo2ze = R(0.5) * inv_s;')
         w('            (void)oo2e; (void)oo2ze;')
     for ln in vrr.lines:
         w('            ' + ln)
@@ -223,7 +226,7 @@ def gen_class(la, lb, lc, ld):
                     for idd, fd in SPH[ld][kd]:
                         coef = fc * fd
                         v = knode(cc[ic], dd[idd])
-                        terms.append(v if coef == 1.0 else f'{lit(coef)} * {v}')
+                        terms.append(v if coef == 1.0 else f'R({lit(coef)}) * {v}')
                 ket_sph[(e, kc, kd)] = hrr.new(' + '.join(terms)) if len(terms) > 1 or '*' in terms[0] else terms[0]
 
     # ---- bra HRR + bra cart->sph, for every ket spherical pair
@@ -250,13 +253,14 @@ def gen_class(la, lb, lc, ld):
                         for ib, fb in SPH[lb][kb]:
                             coef = fa * fb
                             v = bnode(ca[ia], cb[ib])
-                            terms.append(v if coef == 1.0 else f'{lit(coef)} * {v}')
+                            terms.append(v if coef == 1.0 else f'R({lit(coef)}) * {v}')
                     idx = ((ka * nsph(lb) + kb) * nsph(lc) + kc) * nsph(ld) + kd
                     hrr.lines.append(f'out[{idx}] = {" + ".join(terms)};')
     w('    (void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz;')
     for ln in hrr.lines:
         w('    ' + ln)
     w('}')
+    w('};')
     vrr_flops = sum(count_flops(ln.split(' = ', 1)[1]) for ln in vrr.lines) + nacc
     hrr_flops = sum(count_flops(ln.split(' = ', 1)[1]) for ln in hrr.lines)
     return tag, '\n'.join(out), vrr_flops, hrr_flops
@@ -279,10 +283,18 @@ def variant(la, lb, lc, ld, nlines):
     return 'bs'
 
 
+def tiled_waves(la, lb, lc, ld):
+    """Waves per block of the tiled kernel: more waves share one set of LDS
+    tiles where the integrals need few registers (the VGPR budget per lane is
+    512 / waves-per-SIMD)."""
+    L = la + lb + lc + ld
+    return 16 if L <= 2 else (8 if L == 3 else 4)
+
+
 def tiled_eligible(la, lb, lc, ld):
-    """Classes that also get the LDS-tiled Fock kernel (cheap integrals, where
-    global atomics would dominate)."""
-    return la + lb + lc + ld <= 4
+    """Classes that also get the LDS-tiled Fock kernel.  L <= 4 use it by
+    default; for L = 5 it is built and chosen at run time (VLX_TILED_LMAX)."""
+    return la + lb + lc + ld <= 5
 
 
 def pair_classes(lmax):
@@ -318,7 +330,7 @@ def main(outdir, lmax):
             if (la, lb) == (lc, ld):
                 src.append(f'VLX_INSTANTIATE_DIAG({la}, {lb})')
             if tiled_eligible(la, lb, lc, ld):
-                src.append(f'VLX_INSTANTIATE_TILED({la}, {lb}, {lc}, {ld})')
+                src.append(f'VLX_INSTANTIATE_TILED({la}, {lb}, {lc}, {ld}, {tiled_waves(la, lb, lc, ld)})')
         path = outdir / f'eri_{tag}.hip'
         text = '\n'.join(src) + '\n'
         if not path.exists() or path.read_text() != text:
@@ -338,10 +350,11 @@ def main(outdir, lmax):
         y = pcs.index((lc, ld))
         nv, nh = flops[(la, lb, lc, ld)]
         ks = {'bs': 'VLX_BRA_STATIONARY', 'ks': 'VLX_KET_STATIONARY', 'md': 'VLX_WAVE_MD'}[variants[(la, lb, lc, ld)]]
-        tl = (f'&launch_tiled_{la}{lb}{lc}{ld}'
-              if tiled_eligible(la, lb, lc, ld) and variants[(la, lb, lc, ld)] != 'md' else 'nullptr')
+        has_tiled = tiled_eligible(la, lb, lc, ld) and variants[(la, lb, lc, ld)] != 'md'
+        tl = f'&launch_tiled_{la}{lb}{lc}{ld}' if has_tiled else 'nullptr'
+        tl32 = f'&launch_tiled32_{la}{lb}{lc}{ld}' if has_tiled else 'nullptr'
         d.append(f'    {{{x}, {y}, {la}, {lb}, {lc}, {ld}, {nv}, {nh}, {ks}, &launch_fock_{la}{lb}{lc}{ld}, '
-                 f'&launch_eri_{la}{lb}{lc}{ld}, {tl}}},')
+                 f'&launch_eri_{la}{lb}{lc}{ld}, {tl}, {tl32}}},')
     d.append('};')
     d.append('static const DiagLauncher g_diag_table[] = {')
     for (la, lb) in pcs:

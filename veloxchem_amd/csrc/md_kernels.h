@@ -98,6 +98,8 @@ struct Sizes {
     static constexpr int nsab = nsa * nsb, nscd = nsc * nsd;
     // LDS doubles: E (bra, ket), U_AB, U_CD, two R levels, X, out, scalars
     static constexpr int lds = 6 * E3 + nsab * nhab + nhcd * nscd + 2 * nr + nhab * nscd + nsab * nscd + 32;
+    // the Boys table is staged in LDS when it fits beside the work arrays, else read from HBM/L2
+    static constexpr bool tab_in_lds = (lds + nsab * nscd + BOYS_TAB_SIZE) * 8 <= 160 * 1024;
 };
 
 // 1D Hermite coefficients E^{ij}_t (i <= la, j <= lb) of one primitive pair for
@@ -205,7 +207,7 @@ __device__ void quartet(const PrimPair* __restrict__ bra_prims, int bra_np, cons
             const double pq[3] = {bp.px - kp.px, bp.py - kp.py, bp.pz - kp.pz};
             if (threadIdx.x == 3) {
                 double F[L + 1];
-                boys_eval<L>(rho * (pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2]), tab, F);
+                boys_eval<L, double>(rho * (pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2]), tab, F);
                 for (int m = 0; m <= L; m++) scal[m] = F[m];
             }
             __syncthreads();
@@ -342,11 +344,13 @@ __global__ void __launch_bounds__(VLX_WAVE)
                    const double* __restrict__ D, double* __restrict__ F, const FockScalars a)
 {
     using S = md::Sizes<la, lb, lc, ld>;
-    static_assert((S::lds + S::nsab * S::nscd + BOYS_TAB_SIZE) * 8 <= 160 * 1024, "LDS budget");
-    __shared__ double tab[BOYS_TAB_SIZE];
+    static_assert((S::lds + S::nsab * S::nscd + (S::tab_in_lds ? BOYS_TAB_SIZE : 0)) * 8 <= 160 * 1024,
+                  "LDS budget");
+    __shared__ double tab_lds[S::tab_in_lds ? BOYS_TAB_SIZE : 1];
     __shared__ double work[S::lds];
     __shared__ double out[S::nsab * S::nscd];
-    stage_boys(tab, boys_tab);
+    const double* tab = S::tab_in_lds ? tab_lds : boys_tab;
+    if constexpr (S::tab_in_lds) stage_boys(tab_lds, boys_tab);
     for (long long it = blockIdx.x; it < a.nitems; it += gridDim.x) {
         const WorkItem wi = items[it];
         const PairInfo bp = bra[wi.i0];
@@ -370,10 +374,11 @@ template <int la, int lb, int lc, int ld>
 __global__ void __launch_bounds__(VLX_WAVE) eri_out_kernel_md(const EriArgs a)
 {
     using S = md::Sizes<la, lb, lc, ld>;
-    __shared__ double tab[BOYS_TAB_SIZE];
+    __shared__ double tab_lds[S::tab_in_lds ? BOYS_TAB_SIZE : 1];
     __shared__ double work[S::lds];
     __shared__ double out[S::nsab * S::nscd];
-    stage_boys(tab, a.boys_tab);
+    const double* tab = S::tab_in_lds ? tab_lds : a.boys_tab;
+    if constexpr (S::tab_in_lds) stage_boys(tab_lds, a.boys_tab);
     for (int qi = blockIdx.x; qi < a.nquartets; qi += gridDim.x) {
         const int2 qq = a.quartets[qi];
         const PairInfo bp = a.bra[qq.x];
@@ -391,10 +396,11 @@ template <int la, int lb>
 __global__ void __launch_bounds__(VLX_WAVE) diag_kernel_md(const DiagArgs a)
 {
     using S = md::Sizes<la, lb, la, lb>;
-    __shared__ double tab[BOYS_TAB_SIZE];
+    __shared__ double tab_lds[S::tab_in_lds ? BOYS_TAB_SIZE : 1];
     __shared__ double work[S::lds];
     __shared__ double out[S::nsab * S::nscd];
-    stage_boys(tab, a.boys_tab);
+    const double* tab = S::tab_in_lds ? tab_lds : a.boys_tab;
+    if constexpr (S::tab_in_lds) stage_boys(tab_lds, a.boys_tab);
     for (int pi = blockIdx.x; pi < a.npairs; pi += gridDim.x) {
         const PairInfo pp = a.pairs[pi];
         const double abv[3] = {pp.abx, pp.aby, pp.abz};

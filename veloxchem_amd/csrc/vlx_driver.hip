@@ -136,6 +136,9 @@ struct ClassSchedule {
     int tile_lds = 0;  // bytes of dynamic LDS (tiled)
     std::vector<TileItem> titems;
     DevBuf<TileItem> d_titems;
+    std::vector<TileItem> titems32;  // tiled items run with FP32 recursions (mixed precision)
+    DevBuf<TileItem> d_titems32;
+    long long prim_slots32 = 0;      // primitive slots of the FP32 items
     std::vector<WorkItem> items;
     long long quartets = 0, prim_quartets = 0;
     DevBuf<WorkItem> d_items;
@@ -145,20 +148,28 @@ constexpr int BRA_CHUNK = 128;  // default upper bound of bras per work item
 
 constexpr int NSIDE = 4;  // side streams that run class kernels concurrently
 
+constexpr int NO_FP32 = 99;  // bin-sum threshold that sends no quartet to FP32
+
 struct vlx_context {
     int device = 0;
     int nstreams = NSIDE;           // side streams in use (VLX_STREAMS, 1..NSIDE)
     int max_chunk = BRA_CHUNK;      // bras per work item upper bound (VLX_BRA_CHUNK)
     int min_items = 16384;          // work items a class launch aims for (VLX_MIN_ITEMS)
     int use_tiled = 1;              // LDS-tiled kernels where built (VLX_TILED)
+    int tiled_lmax = 4;             // largest la+lb+lc+ld that uses the tiled kernel (VLX_TILED_LMAX)
     int group_atoms = 12;           // atoms per group of the tiled kernels (VLX_GROUP_ATOMS)
+    int tiled_waves = 0;            // 4: 4-wave blocks for every tiled class; 0: per-class default (VLX_TILED_WAVES)
     int max_tile_lds = 96 * 1024;   // larger tiles fall back to the untiled kernel
     hipStream_t stream = nullptr;
     hipStream_t side[NSIDE] = {};
     hipEvent_t ev_start = nullptr;
     hipEvent_t ev_done[NSIDE] = {};
     DevBuf<double> d_boys;
+    DevBuf<float> d_boys_f;         // the Boys table in FP32 (mixed precision)
     DevBuf<double> d_density, d_fock, d_faccum;
+    DevBuf<unsigned long long> d_dmax;  // max |D| of the current build (bit pattern)
+    double mixed_tau = 0.0;         // mixed precision: FP32 below this Schwarz x max|D| bound; 0 = FP64 only
+    int last_k32 = NO_FP32;         // bin-sum threshold of the last build
     int block_size_factor = 8;
     // optional per-class timing
     bool timing = false;
@@ -194,7 +205,7 @@ struct vlx_screener {
         int cls = -1, idx = -1, swapped = 0;
     };
     std::vector<PairRef> pair_ref;  // nshells x nshells -> position in all[]
-    std::map<int, std::vector<std::unique_ptr<ClassSchedule>>> schedules;  // by ithreshold
+    std::map<int, std::vector<std::unique_ptr<ClassSchedule>>> schedules;  // by (ithreshold, FP32 bin-sum threshold)
 };
 
 namespace {
@@ -381,6 +392,18 @@ __global__ void zero_kernel(double* __restrict__ p, size_t n)
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
 }
 
+// max |x| as the bit pattern of a non-negative double (ordered like the value)
+__global__ void absmax_kernel(const double* __restrict__ x, size_t n, unsigned long long* __restrict__ out)
+{
+    double m = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(x[i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(m)));
+}
+
+__global__ void clear_u64_kernel(unsigned long long* __restrict__ p) { *p = 0ull; }
+
 // G = F' + F'^T
 __global__ void symmetrize_kernel(const double* __restrict__ f, double* __restrict__ g, int n)
 {
@@ -389,9 +412,14 @@ __global__ void symmetrize_kernel(const double* __restrict__ f, double* __restri
     if (q < n) g[(size_t)p * n + q] = f[(size_t)p * n + q] + f[(size_t)q * n + p];
 }
 
-int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSchedule>>** out)
+// Work items of every class at ithreshold.  k32 < NO_FP32 also splits the
+// tiled items: the bras of an item whose bin plus the smallest ket bin of
+// its chunk reaches k32 form an FP32 item (every quartet in it has
+// Schwarz bound < 10^(2 - k32)); the rest stay FP64.
+int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_ptr<ClassSchedule>>** out)
 {
-    auto it = s->schedules.find(ithr);
+    const int key = (ithr + 1) * 128 + k32;
+    auto it = s->schedules.find(key);
     if (it != s->schedules.end()) {
         *out = &it->second;
         return VLX_OK;
@@ -425,7 +453,8 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
         const int* gm = s->grp_max;
         const int tile_lds = 8 * (gm[ce.la] * gm[ce.lb] + gm[ce.lc] * gm[ce.ld] + gm[ce.la] * gm[ce.lc] +
                                    gm[ce.la] * gm[ce.ld] + gm[ce.lb] * gm[ce.lc] + gm[ce.lb] * gm[ce.ld]);
-        if (s->ctx->use_tiled && ce.tiled && tile_lds <= s->ctx->max_tile_lds) {
+        if (s->ctx->use_tiled && ce.tiled && ce.la + ce.lb + ce.lc + ce.ld <= s->ctx->tiled_lmax &&
+            tile_lds <= s->ctx->max_tile_lds) {
             // tiled items: (bra segment prefix, chunk of <= 64 kets of a ket segment)
             const PairSet& tb = unscreened ? s->all_tiled_bra[ce.x] : s->kept_tiled_bra[ce.x];
             const PairSet& tk = unscreened ? s->all_tiled[ce.y] : s->kept_tiled[ce.y];
@@ -445,7 +474,15 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
                         const auto& bs = tb.segs[sb];
                         const int i0 = bs.start;
                         const int i1 = bs.start + (unscreened ? bs.end - bs.start : bs.cnt_le[std::min(lim, NBINS - 1)]);
-                        if (i1 > i0) sch->titems.push_back({i0, i1, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb});
+                        if (i1 <= i0) continue;
+                        // bras [im, i1) have bin >= k32 - bin_lo: FP32 for every lane of the chunk
+                        int im = i1;
+                        if (k32 < NO_FP32 && !unscreened) {
+                            const int b = k32 - bin_lo - 1;
+                            im = (b < 0) ? i0 : std::min(i1, bs.start + bs.cnt_le[std::min(b, NBINS - 1)]);
+                        }
+                        if (im > i0) sch->titems.push_back({i0, im, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb});
+                        if (i1 > im) sch->titems32.push_back({im, i1, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb});
                     }
                 }
             }
@@ -499,6 +536,11 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
                 sch->lane_slots += 64LL * (ti.i1 - ti.i0);
                 sch->prim_slots += (tb.nprim_prefix[ti.i1] - tb.nprim_prefix[ti.i0]) * 64;
             }
+            for (const auto& ti : sch->titems32) {
+                sch->lane_slots += 64LL * (ti.i1 - ti.i0);
+                sch->prim_slots32 += (tb.nprim_prefix[ti.i1] - tb.nprim_prefix[ti.i0]) * 64;
+            }
+            sch->prim_slots += sch->prim_slots32;
         } else if (ce.variant == VLX_KET_STATIONARY) {
             for (const auto& wi : sch->items) {
                 int kmax = 0;
@@ -516,9 +558,10 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
                 sch->prim_slots += (long long)kmax * bra.pairs[wi.i0].nprim * lanes;
             }
         }
-        if (sch->items.empty() && sch->titems.empty()) continue;
+        if (sch->items.empty() && sch->titems.empty() && sch->titems32.empty()) continue;
         if (sch->tiled) {
-            HIP_TRY(sch->d_titems.upload(sch->titems));
+            if (!sch->titems.empty()) HIP_TRY(sch->d_titems.upload(sch->titems));
+            if (!sch->titems32.empty()) HIP_TRY(sch->d_titems32.upload(sch->titems32));
         } else {
             HIP_TRY(sch->d_items.upload(sch->items));
         }
@@ -538,7 +581,7 @@ int get_schedules(vlx_screener* s, int ithr, std::vector<std::unique_ptr<ClassSc
         c.second->stream = k;
         load[k] += c.first;
     }
-    auto res = s->schedules.emplace(ithr, std::move(list));
+    auto res = s->schedules.emplace(key, std::move(list));
     *out = &res.first->second;
     return VLX_OK;
 }
@@ -576,7 +619,9 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_BRA_CHUNK")) ctx->max_chunk = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("VLX_MIN_ITEMS")) ctx->min_items = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("VLX_TILED")) ctx->use_tiled = std::atoi(e) != 0;
+    if (const char* e = std::getenv("VLX_TILED_LMAX")) ctx->tiled_lmax = std::atoi(e);
     if (const char* e = std::getenv("VLX_GROUP_ATOMS")) ctx->group_atoms = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("VLX_TILED_WAVES")) ctx->tiled_waves = std::atoi(e);
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (int k = 0; k < NSIDE; k++) {
@@ -587,6 +632,8 @@ int vlx_context_create(int device, vlx_context** out)
     std::vector<double> tab((size_t)(VLX_MAX_LTOT + 1) * BOYS_TAB_SIZE);
     vlx_build_boys_table(tab.data());
     HIP_TRY(ctx->d_boys.upload(tab));
+    std::vector<float> tabf(tab.begin(), tab.end());
+    HIP_TRY(ctx->d_boys_f.upload(tabf));
     *out = ctx.release();
     return VLX_OK;
 }
@@ -616,6 +663,44 @@ int vlx_set_block_size_factor(vlx_context* ctx, int factor)
     if (!ctx) return fail(VLX_ERR_ARG, "context is NULL");
     if (factor < 1 || factor > 128) return fail(VLX_ERR_ARG, "block size factor %d outside 1..128", factor);
     ctx->block_size_factor = factor;
+    return VLX_OK;
+}
+
+int vlx_set_precision(vlx_context* ctx, int mode, double threshold)
+{
+    if (!ctx) return fail(VLX_ERR_ARG, "context is NULL");
+    if (mode == VLX_PRECISION_FP64) {
+        ctx->mixed_tau = 0.0;
+    } else if (mode == VLX_PRECISION_MIXED) {
+        if (!(threshold > 0.0) || !std::isfinite(threshold))
+            return fail(VLX_ERR_ARG, "mixed-precision threshold must be positive and finite");
+        ctx->mixed_tau = threshold;
+    } else {
+        return fail(VLX_ERR_ARG, "unknown precision mode %d", mode);
+    }
+    return VLX_OK;
+}
+
+int vlx_context_last_fp32_bin_sum(vlx_context* ctx, int* k32)
+{
+    if (!ctx || !k32) return fail(VLX_ERR_ARG, "NULL argument");
+    *k32 = ctx->last_k32;
+    return VLX_OK;
+}
+
+int vlx_fock_precision_split(vlx_screener* scr, int ithreshold, int k32, long long* prim_slots_fp64,
+                             long long* prim_slots_fp32)
+{
+    if (!scr || !prim_slots_fp64 || !prim_slots_fp32) return fail(VLX_ERR_ARG, "NULL argument");
+    if (k32 < 0) return fail(VLX_ERR_ARG, "bin-sum threshold must be >= 0");
+    std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
+    int rc = get_schedules(scr, ithreshold, std::min(k32, NO_FP32), &list);
+    if (rc) return rc;
+    *prim_slots_fp64 = *prim_slots_fp32 = 0;
+    for (const auto& sp : *list) {
+        *prim_slots_fp64 += sp->prim_slots - sp->prim_slots32;
+        *prim_slots_fp32 += sp->prim_slots32;
+    }
     return VLX_OK;
 }
 
@@ -864,7 +949,7 @@ int vlx_fock_schedule_stats(vlx_screener* scr, int ithreshold, int max_records, 
 {
     if (!scr || !nrecords) return fail(VLX_ERR_ARG, "NULL argument");
     std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
-    int rc = get_schedules(scr, ithreshold, &list);
+    int rc = get_schedules(scr, ithreshold, NO_FP32, &list);
     if (rc) return rc;
     *nrecords = static_cast<int>(list->size());
     for (int r = 0; r < std::min(max_records, *nrecords); r++) {
@@ -937,7 +1022,7 @@ int vlx_fock_work(vlx_screener* scr, int ithreshold, int max_records, int* nreco
     if (!scr || !nrecords) return fail(VLX_ERR_ARG, "NULL argument");
     if (ithreshold > 30) return fail(VLX_ERR_ARG, "ithreshold %d outside the accepted range (< 0 or 0..30)", ithreshold);
     std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
-    int rc = get_schedules(scr, ithreshold, &list);
+    int rc = get_schedules(scr, ithreshold, NO_FP32, &list);
     if (rc) return rc;
     *nrecords = static_cast<int>(list->size());
     for (int r = 0; r < std::min(max_records, *nrecords); r++) {
@@ -977,8 +1062,27 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
     HIP_TRY(ctx->d_faccum.alloc(nn));
     zero_kernel<<<std::min<size_t>((nn + 255) / 256, 4096), 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, nn);
     HIP_TRY(hipGetLastError());
+    // mixed precision: FP32 where Schwarz bound x max|D| < tau, i.e. for bin sums
+    // >= k32 = ceil(2 + log10(max|D| / tau)) (bins: sqrt(max) < 10^(1 - bin))
+    int k32 = NO_FP32;
+    if (ctx->mixed_tau > 0.0 && ithreshold >= 0) {
+        HIP_TRY(ctx->d_dmax.alloc(1));
+        clear_u64_kernel<<<1, 1, 0, ctx->stream>>>(ctx->d_dmax.ptr);
+        absmax_kernel<<<std::min<size_t>((nn + 255) / 256, 1024), 256, 0, ctx->stream>>>(d_density, nn, ctx->d_dmax.ptr);
+        HIP_TRY(hipGetLastError());
+        unsigned long long bits = 0;
+        HIP_TRY(hipMemcpyAsync(&bits, ctx->d_dmax.ptr, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        double dmax;
+        std::memcpy(&dmax, &bits, sizeof(dmax));
+        if (dmax > 0.0) {
+            const double kk = std::ceil(2.0 + std::log10(dmax / ctx->mixed_tau));
+            k32 = static_cast<int>(std::max(0.0, std::min<double>(NO_FP32, kk)));
+        }
+    }
+    ctx->last_k32 = k32;
     std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
-    rc = get_schedules(scr, ithreshold, &list);
+    rc = get_schedules(scr, ithreshold, k32, &list);
     if (rc) return rc;
     const bool unscreened = ithreshold < 0;
     HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
@@ -997,6 +1101,7 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
         a.ket_prims = ket.d_prims.ptr;
         a.items = sch.d_items.ptr;
         a.boys_tab = ctx->d_boys.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;
+        a.boys_tab_f = ctx->d_boys_f.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;
         a.D = d_density;
         a.F = ctx->d_faccum.ptr;
         a.titems = sch.d_titems.ptr;
@@ -1005,6 +1110,7 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
         a.grp_n = scr->d_grp_n.ptr;
         a.grp_ao = scr->d_grp_ao.ptr;
         a.tile_lds_bytes = sch.tile_lds;
+        a.tiled_waves = ctx->tiled_waves;
         a.sc.nitems = static_cast<long long>(sch.tiled ? sch.titems.size() : sch.items.size());
         a.sc.nket = ket.size();
         a.sc.ithreshold = ithreshold;
@@ -1027,8 +1133,18 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
             }
             HIP_TRY(hipEventRecord(pend.start, ctx->side[sch.stream]));
         }
-        if (sch.tiled) ce.tiled(a, grid, ctx->side[sch.stream]);
-        else ce.fock(a, grid, ctx->side[sch.stream]);
+        if (sch.tiled) {
+            if (!sch.titems.empty()) ce.tiled(a, grid, ctx->side[sch.stream]);
+            if (!sch.titems32.empty()) {
+                FockArgs a32 = a;
+                a32.titems = sch.d_titems32.ptr;
+                a32.sc.nitems = static_cast<long long>(sch.titems32.size());
+                const int grid32 = static_cast<int>(std::min<long long>(a32.sc.nitems, 4096));
+                ce.tiled32(a32, grid32, ctx->side[sch.stream]);
+            }
+        } else {
+            ce.fock(a, grid, ctx->side[sch.stream]);
+        }
         HIP_TRY(hipGetLastError());
         if (ctx->timing) {
             HIP_TRY(hipEventRecord(pend.stop, ctx->side[sch.stream]));

@@ -70,7 +70,9 @@ struct FockArgs {
     const int* grp_n;              // (ngroups x 4): number of AOs in the (group, l) block
     const int* grp_ao;             // global AO index of each local AO
     int tile_lds_bytes;            // dynamic LDS of the tiled variant
+    int tiled_waves;               // 4: force 4-wave blocks for the tiled kernel; else the class default
     const double* boys_tab;        // BOYS_TAB_SIZE doubles for L = la+lb+lc+ld
+    const float* boys_tab_f;       // the same table in FP32 (mixed-precision tiled kernel)
     const double* D;               // nao x nao, row-major
     double* F;                     // nao x nao accumulator F' (G = F' + F'^T)
     FockScalars sc;
@@ -110,6 +112,7 @@ struct ClassLaunchers {
     FockLaunchFn fock;
     EriLaunchFn eri;
     FockLaunchFn tiled;   // LDS-tiled Fock kernel, or nullptr
+    FockLaunchFn tiled32; // the tiled kernel with FP32 recursions (mixed precision), or nullptr
 };
 
 struct DiagLauncher {
@@ -121,5 +124,7 @@ struct DiagLauncher {
 #define VLX_DECLARE_CLASS(la, lb, lc, ld)                                        \
     void launch_fock_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s); \
     void launch_eri_##la##lb##lc##ld(const EriArgs& a, hipStream_t s);
-#define VLX_DECLARE_TILED(la, lb, lc, ld) void launch_tiled_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);
+#define VLX_DECLARE_TILED(la, lb, lc, ld)                                          \
+    void launch_tiled_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s); \
+    void launch_tiled32_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);
 #define VLX_DECLARE_DIAG(la, lb) void launch_diag_##la##lb(const DiagArgs& a, hipStream_t s);

@@ -104,6 +104,9 @@ class ScfRestrictedDriver:
         self.eri_thresh = 1.0e-12
         # fock_2e(screener, D, thresh_int) -> G on D's device; default: the GPU build
         self.fock_2e = None
+        # 'fp64', or 'mixed': FP32 recursions where Schwarz bound x max|D| < mixed_threshold
+        self.precision = 'fp64'
+        self.mixed_threshold = 1.0e-8
         self.screener_factory = None
         self.device = None
         self.history = []
@@ -146,36 +149,43 @@ class ScfRestrictedDriver:
             raise ValueError('ScfRestrictedDriver: closed-shell molecules only')
         if min_basis is None:
             min_basis = MolecularBasis.read(molecule, 'AO-START-GUESS')
-        if self.fock_2e is None:
+        if self.fock_2e is None or isinstance(self.fock_2e, _DeviceFock):
             dev = _native.current_device()
             self.fock_2e = _DeviceFock(dev)
             self.device = torch.device('cuda', dev)
+            _native.set_precision(self.precision, self.mixed_threshold if self.precision == 'mixed' else 0.0, dev)
         elif self.device is None:
             self.device = torch.device('cpu')
         t0 = time.perf_counter()
         self.history = []
         self.timings = {'fock': 0.0, 'one_electron': 0.0, 'linear_algebra': 0.0, 'screener': 0.0, 'guess': 0.0}
+        try:
+            res = self._run(molecule, ao_basis, min_basis)
+        finally:
+            if isinstance(self.fock_2e, _DeviceFock):
+                _native.set_precision('fp64', 0.0, self.fock_2e.device)
+        self.timings['total'] = time.perf_counter() - t0
+        res['timings'] = dict(self.timings)
+        res['num_iter'] = self._num_iter
+        res['is_converged'] = self._is_converged
+        return res
+
+    def _run(self, molecule, ao_basis, min_basis):
         acc = self.acc_type.upper()
         if acc == 'DIIS':
             t = time.perf_counter()
             den = sad_density(molecule, min_basis, ao_basis)
             self.timings['guess'] += time.perf_counter() - t
-            res = self._comp_diis(molecule, ao_basis, den, self.conv_thresh, self.max_iter)
-        elif acc == 'L2_DIIS':
+            return self._comp_diis(molecule, ao_basis, den, self.conv_thresh, self.max_iter)
+        if acc == 'L2_DIIS':
             val_basis = ao_basis.reduce_to_valence_basis()
             t = time.perf_counter()
             den = sad_density(molecule, min_basis, val_basis)
             self.timings['guess'] += time.perf_counter() - t
             first = self._comp_diis(molecule, val_basis, den, 1.0e-3, 5)
             den = self._project_density(molecule, ao_basis, val_basis, first['C_alpha'])
-            res = self._comp_diis(molecule, ao_basis, den, self.conv_thresh, self.max_iter)
-        else:
-            raise ValueError(f'ScfRestrictedDriver: unknown acc_type {self.acc_type!r}')
-        self.timings['total'] = time.perf_counter() - t0
-        res['timings'] = dict(self.timings)
-        res['num_iter'] = self._num_iter
-        res['is_converged'] = self._is_converged
-        return res
+            return self._comp_diis(molecule, ao_basis, den, self.conv_thresh, self.max_iter)
+        raise ValueError(f'ScfRestrictedDriver: unknown acc_type {self.acc_type!r}')
 
     def _project_density(self, molecule, ao_basis, val_basis, c_val):
         """Valence MOs inserted into the full AO basis (scfdriver.py:923-990), aufbau density."""

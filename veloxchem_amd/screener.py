@@ -92,6 +92,13 @@ class T4CScreener:
         return [(tuple(int(x) for x in cls[4 * i:4 * i + 4]), int(nq[i]), int(npq[i]), float(fl[i]))
                 for i in range(n.value)]
 
+    def precision_split(self, ithreshold, k32):
+        """(FP64, FP32) primitive slots of one build when bin sums >= k32 run in FP32."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        _native.check(_native.lib().vlx_fock_precision_split(self.handle, int(ithreshold), int(k32), ctypes.byref(a),
+                                                             ctypes.byref(b)))
+        return a.value, b.value
+
     def schedule_stats(self, ithreshold):
         """Per class (same order as work()): (lane slots, primitive slots) the kernels execute."""
         L = _native.lib()
