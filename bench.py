@@ -95,6 +95,10 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-scf', action='store_true')
+    ap.add_argument('--no-autotune', action='store_true', help='skip the per-class kernel-variant autotuner')
+    ap.add_argument('--precision', default='fp64', choices=['fp64', 'mixed'],
+                    help='precision of the main timed builds (value)')
+    ap.add_argument('--tau', type=float, default=1e-6, help='mixed-precision threshold of --precision mixed')
     ap.add_argument('--mixed', default='1e-6,1e-8,1e-10',
                     help='Schwarz x max|D| thresholds of the mixed-precision sweep (comma list, empty: none)')
     ap.add_argument('--scf-conv', type=float, default=1.0e-8)
@@ -119,6 +123,15 @@ def main():
         _native.check(lib.vlx_fock_compute_device(ctx, scr.handle, d_dm.ptr, label, 1.0, 0.0, args.ithreshold,
                                                   d_g.ptr))
 
+    tuned, t_tune = None, 0.0
+    if not args.no_autotune:
+        from veloxchem_amd import FockDriver
+        t0 = time.perf_counter()
+        tuned = FockDriver()._autotune(scr, dm, args.ithreshold, repeats=2)
+        t_tune = time.perf_counter() - t0
+
+    if args.precision == 'mixed':
+        _native.set_precision('mixed', args.tau)
     for _ in range(args.warmup):
         one_build()
     _native.check(lib.vlx_context_synchronize(ctx))
@@ -133,6 +146,7 @@ def main():
     _native.check(lib.vlx_context_set_timing(ctx, 0))
     times = _native.class_times(0)
     g = d_g.download()
+    _native.set_precision('fp64')
 
     # mixed-precision sweep on the same density: build rate and error vs the FP64 build
     sweep = []
@@ -187,6 +201,7 @@ def main():
             'shell_quartets': int(sum(w[1] for w in work)),
             'primitive_quartets': int(sum(w[2] for w in work)),
             'screener_setup_s': round(t_screen, 3),
+            'autotune_s': round(t_tune, 3),
         },
         'roofline': {
             'bound': 'valu',
@@ -206,6 +221,8 @@ def main():
     }
     d_dm.free()
     d_g.free()
+    if tuned:
+        result['autotune'] = {'%d%d%d%d' % c: v for c, v in sorted(tuned.items())}
     if sweep:
         result['mixed_sweep'] = sweep
     if not args.no_scf:

@@ -135,6 +135,19 @@ int vlx_fock_work(vlx_screener* scr, int ithreshold, int max_records, int* nreco
 int vlx_fock_schedule_stats(vlx_screener* scr, int ithreshold, int max_records, int* nrecords,
                             long long* lane_slots, long long* prim_slots);
 
+/*
+ * Autotuner (no reference counterpart): times every angular class of one
+ * 2jk build at ithreshold on `d_density` under each kernel variant
+ * (0 = untiled, 1 = LDS-tiled with 4-wave blocks, 2 = LDS-tiled with the
+ * class's wide block), `repeats` builds each on a single stream, and keeps
+ * the fastest variant per class for later builds with this screener (and as
+ * the default of screeners created later on this context).
+ * Optional outputs: *nchosen records of classes (4 ints each) and variants
+ * (arrays of at least 64 classes).  Results do not depend on the variant.
+ */
+int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_density, int ithreshold, int repeats,
+                      int* nchosen, int* classes, int* variants);
+
 /* Per-class kernel timing with HIP events on the context's stream
  * (off by default).  vlx_context_class_times synchronises the stream and
  * returns, per class (la, lb, lc, ld), the summed kernel time in ms and the

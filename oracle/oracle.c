@@ -19,6 +19,11 @@
  *     over all spherical components of (ab|ab) starting from zero
  *     (src/t4c_func/T4CUtils.cpp:513-519) and quartets kept iff
  *     bin_bra + bin_ket <= ithreshold (src/general/OpenMPFunc.cpp:171,234);
+ *   - the erf-attenuated operator erf(omega r)/r of the range-separated labels
+ *     (j_rs, k_rs, kx_rs: src/t4c_func/T4CMatrixDistributor.cpp:51-54) by the
+ *     substitution F_m(T) -> s^(m+1/2) F_m(s T), s = omega^2 / (omega^2 + rho)
+ *     (src/t4c_func/T4CUtils.cpp:422-465, src/math/BoysFunc.hpp:24890-24940),
+ *     which in the Hermite form is rho -> s rho plus a factor sqrt(s);
  *   - J/K digestion per AO quartet with the component-symmetry skips and the
  *     1/2 factors of src/t4c_func/T4CLocalDistributor.cpp:130-280 (2jk),
  *     :293-560 (2jkx), :568-766 (j), :769-1005 (k), :1013-1245 (kx), then
@@ -201,7 +206,7 @@ static void hermite_r(int L, double alpha, const double *pq, double *r0 /* RDIM^
 }
 
 /* Cartesian (ab|cd) block, index ((a*nb + b)*nc + c)*nd + d */
-static void eri_cart(const oracle_basis_t *bs, int sa, int sb, int sc, int sd, double *out)
+static void eri_cart(const oracle_basis_t *bs, int sa, int sb, int sc, int sd, double omega, double *out)
 {
     const int la = bs->l[sa], lb = bs->l[sb], lc = bs->l[sc], ld = bs->l[sd];
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
@@ -236,8 +241,9 @@ static void eri_cart(const oracle_basis_t *bs, int sa, int sb, int sc, int sd, d
                     const double kcd = exp(-c * d / q * cd2) * bs->norms[bs->prim_offset[sc] + ic] * bs->norms[bs->prim_offset[sd] + id];
                     for (int x = 0; x < 3; x++) pq[x] = P[x] - Q[x];
                     const double alpha = p * q / (p + q);
-                    hermite_r(L, alpha, pq, r0);
-                    const double pref = 2.0 * pow(M_PI, 2.5) / (p * q * sqrt(p + q)) * kab * kcd;
+                    const double s = (omega != 0.0) ? omega * omega / (omega * omega + alpha) : 1.0;
+                    hermite_r(L, alpha * s, pq, r0);
+                    const double pref = 2.0 * pow(M_PI, 2.5) / (p * q * sqrt(p + q)) * kab * kcd * sqrt(s);
                     for (int i = 0; i < na; i++)
                         for (int j = 0; j < nb; j++) {
                             const int *xa = ea[i], *xb = eb[j];
@@ -269,7 +275,20 @@ static void eri_cart(const oracle_basis_t *bs, int sa, int sb, int sc, int sd, d
 }
 
 /* spherical (ab|cd) block, index ((a*nb + b)*nc + c)*nd + d over 2l+1 comps */
+static int eri_sph_omega(const oracle_basis_t *bs, int sa, int sb, int sc, int sd, double omega, double *out);
+
 int oracle_eri_sph(const oracle_basis_t *bs, int sa, int sb, int sc, int sd, double *out)
+{
+    return eri_sph_omega(bs, sa, sb, sc, sd, 0.0, out);
+}
+
+/* erf-attenuated (ab|erf(omega r)/r|cd) block, same layout */
+int oracle_eri_sph_erf(const oracle_basis_t *bs, int sa, int sb, int sc, int sd, double omega, double *out)
+{
+    return eri_sph_omega(bs, sa, sb, sc, sd, omega, out);
+}
+
+static int eri_sph_omega(const oracle_basis_t *bs, int sa, int sb, int sc, int sd, double omega, double *out)
 {
     const int ls[4] = {bs->l[sa], bs->l[sb], bs->l[sc], bs->l[sd]};
     for (int i = 0; i < 4; i++)
@@ -278,7 +297,7 @@ int oracle_eri_sph(const oracle_basis_t *bs, int sa, int sb, int sc, int sd, dou
     for (int i = 0; i < 4; i++) { nc_[i] = ncart(ls[i]); ns_[i] = nsph(ls[i]); }
     const int ncart4 = nc_[0] * nc_[1] * nc_[2] * nc_[3];
     double *cur = malloc(sizeof(double) * ncart4);
-    eri_cart(bs, sa, sb, sc, sd, cur);
+    eri_cart(bs, sa, sb, sc, sd, omega, cur);
     /* transform one index at a time; dims[] tracks the current shape */
     int dims[4] = {nc_[0], nc_[1], nc_[2], nc_[3]};
     for (int ax = 0; ax < 4; ax++) {
@@ -423,8 +442,8 @@ static void digest(const oracle_basis_t *bs, int sa, int sb, int sc, int sd, con
 }
 
 /* internal: build with an optional bra-pair stride (1 = everything) */
-static long fock_impl(const oracle_basis_t *bs, const double *dm, int label, double xf, int ithreshold, int nthreads,
-                      int stride, const int *bins_in, double *gout, long *total);
+static long fock_impl(const oracle_basis_t *bs, const double *dm, int label, double xf, double omega, int ithreshold,
+                      int nthreads, int stride, const int *bins_in, double *gout, long *total);
 
 /*
  * Bounded CPU-baseline sample: only bra pairs ip with ip % stride == 0 are
@@ -434,7 +453,7 @@ static long fock_impl(const oracle_basis_t *bs, const double *dm, int label, dou
 long oracle_fock_sampled(const oracle_basis_t *bs, const double *dm, int label, double xf, int ithreshold,
                          int nthreads, int stride, const int *bins, double *gout, long *total)
 {
-    return fock_impl(bs, dm, label, xf, ithreshold, nthreads, stride, bins, gout, total);
+    return fock_impl(bs, dm, label, xf, 0.0, ithreshold, nthreads, stride, bins, gout, total);
 }
 
 /*
@@ -448,12 +467,25 @@ long oracle_fock(const oracle_basis_t *bs, const double *dm, int label, double x
                  double *gout)
 {
     long total = 0;
-    return fock_impl(bs, dm, label, xf, ithreshold, nthreads, 1, NULL, gout, &total);
+    return fock_impl(bs, dm, label, xf, 0.0, ithreshold, nthreads, 1, NULL, gout, &total);
+}
+
+/*
+ * Range-separated labels: label j/k/kx digestion of the erf(omega r)/r
+ * integrals (the reference's j_rs, k_rs, kx_rs).  Screening uses the
+ * Coulomb pair maxima, as the reference's screener does.
+ */
+long oracle_fock_erf(const oracle_basis_t *bs, const double *dm, int label, double xf, double omega, int ithreshold,
+                     int nthreads, double *gout)
+{
+    long total = 0;
+    if (label != LBL_J && label != LBL_K && label != LBL_KX) return -1;
+    return fock_impl(bs, dm, label, xf, omega, ithreshold, nthreads, 1, NULL, gout, &total);
 }
 
 /* bins_in: optional nshells x nshells pair bins from oracle_pair_bins */
-static long fock_impl(const oracle_basis_t *bs, const double *dm, int label, double xf, int ithreshold, int nthreads,
-                      int stride, const int *bins_in, double *gout, long *total)
+static long fock_impl(const oracle_basis_t *bs, const double *dm, int label, double xf, double omega, int ithreshold,
+                      int nthreads, int stride, const int *bins_in, double *gout, long *total)
 {
     const int ns = bs->nshells, n = bs->nao;
     if (stride < 1) return -1;
@@ -496,7 +528,7 @@ static long fock_impl(const oracle_basis_t *bs, const double *dm, int label, dou
             for (int jp = ip; jp < npairs; jp++) {
                 const int bj = bins[pa[jp] * ns + pb[jp]];
                 if (ithreshold >= 0 && bi + bj > ithreshold) continue;
-                oracle_eri_sph(bs, pa[ip], pb[ip], pa[jp], pb[jp], buf);
+                eri_sph_omega(bs, pa[ip], pb[ip], pa[jp], pb[jp], omega, buf);
                 digest(bs, pa[ip], pb[ip], pa[jp], pb[jp], buf, dm, label, xf, fp);
                 nquart++;
             }

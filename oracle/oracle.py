@@ -122,6 +122,34 @@ def fock(ob, density, label='2jk', exchange_factor=1.0, ithreshold=15, nthreads=
     return g, int(nq)
 
 
+def fock_erf(ob, density, label='k', exchange_factor=1.0, omega=0.0, ithreshold=15, nthreads=0):
+    """Range-separated build (labels j/k/kx on erf(omega r)/r integrals): (G, quartets)."""
+    n = ob.struct.nao
+    dm = np.ascontiguousarray(density, dtype=np.float64)
+    assert dm.shape == (n, n)
+    g = np.zeros((n, n))
+    L = lib()
+    L.oracle_fock_erf.restype = ctypes.c_long
+    nq = L.oracle_fock_erf(ctypes.byref(ob.struct), dm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                           ctypes.c_int(LABELS[label]), ctypes.c_double(exchange_factor), ctypes.c_double(omega),
+                           ctypes.c_int(ithreshold), ctypes.c_int(nthreads),
+                           g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if nq < 0:
+        raise ValueError('oracle_fock_erf: bad input')
+    return g, int(nq)
+
+
+def eri_sph_erf(ob, sa, sb, sc, sd, omega):
+    """erf-attenuated spherical block (ab|erf(omega r)/r|cd)."""
+    ls = [int(ob.arrays['l'][x]) for x in (sa, sb, sc, sd)]
+    out = np.zeros([2 * l + 1 for l in ls])
+    rc = lib().oracle_eri_sph_erf(ctypes.byref(ob.struct), sa, sb, sc, sd, ctypes.c_double(omega),
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if rc:
+        raise ValueError('oracle_eri_sph_erf: bad input')
+    return out
+
+
 def fock_sampled(ob, density, label, exchange_factor, ithreshold, nthreads, stride, bins):
     """Partial build over bra pairs ip % stride == 0 with precomputed pair bins
     (from pair_maxima): (quartets computed, quartets in a full build)."""

@@ -183,8 +183,12 @@ def test_error_paths():
     drv = FockDriver()
     with pytest.raises(ValueError):
         drv.compute(scr, dm, 'xyz', 0.0, 0.0, 12)
-    with pytest.raises(NotImplementedError):
-        drv.compute(scr, dm, '2jk', 1.0, 0.4, 12)
+    # omega is ignored by the Coulomb labels, as in the reference
+    g0 = drv.compute(scr, dm, '2jk', 1.0, 0.0, 12)
+    g1 = drv.compute(scr, dm, '2jk', 1.0, 0.4, 12)
+    assert np.max(np.abs(np.asarray(g0) - np.asarray(g1))) < 1e-13
+    # erf(0 r)/r = 0
+    assert np.max(np.abs(np.asarray(drv.compute(scr, dm, 'k_rs', 0.0, 0.0, 12)))) == 0.0
     with pytest.raises(ValueError):
         drv.compute(scr, dm[:5, :5], '2jk', 1.0, 0.0, 12)
     with pytest.raises(ValueError):
@@ -193,3 +197,40 @@ def test_error_paths():
     bad[0, 1] += 1.0
     with pytest.raises(ValueError):
         drv.compute(scr, bad, '2jk', 1.0, 0.0, 12)
+
+
+@pytest.mark.parametrize('label,xf', [('k_rs', 0.0), ('kx_rs', 0.21)])
+def test_dimer_tzvp_erf_fixture(label, xf):
+    """Reference tests/test_fock_erf_driver.py: k_rs / kx_rs (0.21), omega 0.63, ithreshold 15."""
+    mol, bas = dimer('def2-tzvp')
+    scr = _screened(mol, bas)
+    dm = make_matrix(bas, mat_t.symmetric)
+    dm.set_values(load('h2o.dimer.tzvp.density.npy'))
+    g = FockDriver()._compute_fock_omp(scr, dm, label, xf, 0.63, 15).to_numpy()
+    ref = load('h2o.dimer.tzvp.erf.k.npy') * (xf if label == 'kx_rs' else 1.0)
+    assert reference_equal(g, ref)
+
+
+def test_erf_j_vs_oracle():
+    """j_rs on a 4-water cluster (every class through d) against the oracle at 1e-10."""
+    mol = water_cluster(4, 6.0, 7)
+    bas = MolecularBasis.read(mol, 'def2-svp')
+    scr = _screened(mol, bas)
+    D = pseudo_density(bas.get_dimensions_of_basis(), 20, seed=5)
+    g = FockDriver().compute(scr, D, 'j_rs', 0.0, 0.4, 12)
+    ref, _ = orc.fock_erf(orc.basis_from(bas, mol), D, 'j', 0.0, 0.4, 12)
+    assert np.max(np.abs(np.asarray(g) - ref)) < TOL
+
+
+def test_autotune_choices_keep_parity():
+    """The autotuner times every kernel variant per class; the build it leaves behind still matches the oracle."""
+    mol = water_cluster(4, 6.0, 7)
+    bas = MolecularBasis.read(mol, 'def2-svp')
+    scr = _screened(mol, bas)
+    D = pseudo_density(bas.get_dimensions_of_basis(), 20, seed=9)
+    drv = FockDriver()
+    choice = drv._autotune(scr, D, 12, repeats=1)
+    assert choice and set(choice.values()) <= set(FockDriver.VARIANTS)
+    g = drv.compute(scr, D, '2jk', 1.0, 0.0, 12)
+    ref, _ = orc.fock(orc.basis_from(bas, mol), D, '2jk', 1.0, 12, 0)
+    assert np.max(np.abs(np.asarray(g) - ref)) < TOL

@@ -119,10 +119,11 @@ template <int la, int lb, int lc, int ld>
 __device__ __forceinline__ void eri_quartet(const PrimPair* __restrict__ bra_prims, const int bra_np, const double ABx,
                                             const double ABy, const double ABz, const PrimPair* __restrict__ ket_prims,
                                             const int ket_np, const double CDx, const double CDy, const double CDz,
-                                            const double* __restrict__ boys_tab, double* __restrict__ out)
+                                            const double* __restrict__ boys_tab, const double omega2,
+                                            double* __restrict__ out)
 {
     EriClass<la, lb, lc, ld>::template quartet<double>(bra_prims, bra_np, ABx, ABy, ABz, ket_prims, ket_np, CDx, CDy,
-                                                       CDz, boys_tab, out);
+                                                       CDz, boys_tab, omega2, out);
 }
 
 // Cross-lane moves within rows of 16 lanes (DPP): quad_perm [1,0,3,2] (0xB1),
@@ -271,7 +272,7 @@ __global__ void __launch_bounds__(VLX_BLOCK)
             double I[nsph];
             eri_quartet<la, lb, lc, ld>(bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
                                         ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz,
-                                        tab, I);
+                                        tab, a.omega2, I);
 #pragma unroll
             for (int x = 0; x < nsph; x++) I[x] *= s;
             int q[nb];
@@ -401,7 +402,8 @@ __global__ void __launch_bounds__(VLX_BLOCK)
         if (a.same_class && bp.rank == j) s *= 0.5;
         double I[nsph];
         eri_quartet<la, lb, lc, ld>(bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
-                                    ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz, tab, I);
+                                    ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz, tab,
+                                    a.omega2, I);
 #pragma unroll
         for (int x = 0; x < nsph; x++) I[x] *= s;
         int p[na], q[nb], r[nc], t[nd];
@@ -580,7 +582,7 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
             R I[nsph];
             EriClass<la, lb, lc, ld>::template quartet<R>(bra_prims + bp.prim_start, bp.nprim, R(bp.abx), R(bp.aby),
                                                           R(bp.abz), ket_prims + kp.prim_start + ke.y, active ? 1 : 0,
-                                                          R(kp.abx), R(kp.aby), R(kp.abz), tab, I);
+                                                          R(kp.abx), R(kp.aby), R(kp.abz), tab, R(a.omega2), I);
 #pragma unroll
             for (int x = 0; x < nsph; x++) I[x] *= s;
             int p[na], q[nb];
@@ -709,7 +711,7 @@ __global__ void __launch_bounds__(VLX_BLOCK) eri_out_kernel(const EriArgs a)
     const PairInfo kp = a.ket[qq.y];
     double I[nsph];
     eri_quartet<la, lb, lc, ld>(a.bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
-                                a.ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz, tab, I);
+                                a.ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz, tab, 0.0, I);
     if (active) {
         double* dst = a.out + (size_t)qi * nsph;
 #pragma unroll
@@ -730,7 +732,7 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
     const PairInfo pp = a.pairs[active ? pi : 0];
     double I[nsph];
     eri_quartet<la, lb, la, lb>(a.prims + pp.prim_start, pp.nprim, pp.abx, pp.aby, pp.abz, a.prims + pp.prim_start,
-                                active ? pp.nprim : 0, pp.abx, pp.aby, pp.abz, tab, I);
+                                active ? pp.nprim : 0, pp.abx, pp.aby, pp.abz, tab, 0.0, I);
     double mx = 0.0;
 #pragma unroll
     for (int k = 0; k < nsph; k++) mx = (mx < I[k]) ? I[k] : mx;

@@ -160,7 +160,7 @@ def gen_class(la, lb, lc, ld):
     w('static __device__ __forceinline__ void quartet(')
     w('    const PrimPair* __restrict__ bra_prims, const int bra_np, const R ABx, const R ABy, const R ABz,')
     w('    const PrimPair* __restrict__ ket_prims, const int ket_np, const R CDx, const R CDy, const R CDz,')
-    w('    const R* __restrict__ boys_tab, R* __restrict__ out)')
+    w('    const R* __restrict__ boys_tab, const R omega2, R* __restrict__ out)')
     w('{')
     w(f'    R acc[{nacc}];')
     w('#pragma unroll')
@@ -180,7 +180,18 @@ def gen_class(la, lb, lc, ld):
     w('            const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);')
     w('            const R pref = R(bp.k * kp.k) * rs;')
     w(f'            R F[{L + 1}];')
-    w(f'            boys_eval<{L}, R>(T, boys_tab, F);')
+    # erf(omega r)/r: F_m(T) -> s^(m+1/2) F_m(s T), s = omega^2 / (omega^2 + rho)
+    w('            const bool attenuated = omega2 > R(0);')
+    w('            const R sfac = attenuated ? omega2 * vrcp<R>(omega2 + rho) : R(1);')
+    w(f'            boys_eval<{L}, R>(attenuated ? T * sfac : T, boys_tab, F);')
+    w('            if (attenuated) {')
+    w('                R fm = vsqrt<R>(sfac);')
+    w('#pragma unroll')
+    w(f'                for (int m = 0; m <= {L}; m++) {{')
+    w('                    F[m] *= fm;')
+    w('                    fm *= sfac;')
+    w('                }')
+    w('            }')
     for m in range(L + 1):
         w(f'            const R b{m} = pref * F[{m}];')
     if L > 0:

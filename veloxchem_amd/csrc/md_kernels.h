@@ -173,7 +173,8 @@ __device__ __forceinline__ void build_u(int la, int lb, const double* __restrict
 template <int la, int lb, int lc, int ld>
 __device__ void quartet(const PrimPair* __restrict__ bra_prims, int bra_np, const double* ab_vec,
                         const PrimPair* __restrict__ ket_prims, int ket_np, const double* cd_vec,
-                        const double* __restrict__ tab, double* __restrict__ lds, double* __restrict__ out)
+                        const double* __restrict__ tab, const double omega2, double* __restrict__ lds,
+                        double* __restrict__ out)
 {
     using S = Sizes<la, lb, lc, ld>;
     constexpr int L = S::L;
@@ -203,7 +204,10 @@ __device__ void quartet(const PrimPair* __restrict__ bra_prims, int bra_np, cons
             __syncthreads();
             hermite_1d(lc, ld, kp.zeta, qc, qd, eket);
             const double s = bp.zeta + kp.zeta;
-            const double rho = bp.zeta * kp.zeta / s;
+            // erf(omega r)/r: rho -> sfac rho and a factor sqrt(sfac), sfac = omega^2 / (omega^2 + rho)
+            const double rho_c = bp.zeta * kp.zeta / s;
+            const double sfac = (omega2 > 0.0) ? omega2 / (omega2 + rho_c) : 1.0;
+            const double rho = (omega2 > 0.0) ? rho_c * sfac : rho_c;
             const double pq[3] = {bp.px - kp.px, bp.py - kp.py, bp.pz - kp.pz};
             if (threadIdx.x == 3) {
                 double F[L + 1];
@@ -259,7 +263,7 @@ __device__ void quartet(const PrimPair* __restrict__ bra_prims, int bra_np, cons
                 x[idx] = acc;
             }
             __syncthreads();
-            const double w = 1.0 / sqrt(s);
+            const double w = (omega2 > 0.0) ? sqrt(sfac) / sqrt(s) : 1.0 / sqrt(s);
             for (int idx = threadIdx.x; idx < S::nsab * S::nscd; idx += blockDim.x) {
                 const int ab = idx / S::nscd, cd = idx % S::nscd;
                 double acc = 0.0;
@@ -359,7 +363,7 @@ __global__ void __launch_bounds__(VLX_WAVE)
             const PairInfo kp = ket[j];
             const double cdv[3] = {kp.abx, kp.aby, kp.abz};
             md::quartet<la, lb, lc, ld>(bra_prims + bp.prim_start, bp.nprim, abv, ket_prims + kp.prim_start,
-                                        kp.nprim, cdv, tab, work, out);
+                                        kp.nprim, cdv, tab, a.omega2, work, out);
             double s = 1.0;
             if (bp.same) s *= 0.5;
             if (kp.same) s *= 0.5;
@@ -385,7 +389,7 @@ __global__ void __launch_bounds__(VLX_WAVE) eri_out_kernel_md(const EriArgs a)
         const PairInfo kp = a.ket[qq.y];
         const double abv[3] = {bp.abx, bp.aby, bp.abz}, cdv[3] = {kp.abx, kp.aby, kp.abz};
         md::quartet<la, lb, lc, ld>(a.bra_prims + bp.prim_start, bp.nprim, abv, a.ket_prims + kp.prim_start,
-                                    kp.nprim, cdv, tab, work, out);
+                                    kp.nprim, cdv, tab, 0.0, work, out);
         for (int k = threadIdx.x; k < S::nsab * S::nscd; k += blockDim.x)
             a.out[(size_t)qi * S::nsab * S::nscd + k] = out[k];
         __syncthreads();
@@ -405,7 +409,7 @@ __global__ void __launch_bounds__(VLX_WAVE) diag_kernel_md(const DiagArgs a)
         const PairInfo pp = a.pairs[pi];
         const double abv[3] = {pp.abx, pp.aby, pp.abz};
         md::quartet<la, lb, la, lb>(a.prims + pp.prim_start, pp.nprim, abv, a.prims + pp.prim_start, pp.nprim, abv,
-                                    tab, work, out);
+                                    tab, 0.0, work, out);
         double mx = 0.0;
         for (int k = threadIdx.x; k < S::nsab * S::nscd; k += blockDim.x) mx = (mx < out[k]) ? out[k] : mx;
         for (int off = 32; off > 0; off >>= 1) {

@@ -133,6 +133,7 @@ struct ClassSchedule {
     long long lane_slots = 0, prim_slots = 0;  // schedule efficiency diagnostics
     int stream = 0;  // side stream the class runs on
     bool tiled = false;
+    int tiled_waves = 0;  // 4: 4-wave blocks; 0: the class's wide block
     int tile_lds = 0;  // bytes of dynamic LDS (tiled)
     std::vector<TileItem> titems;
     DevBuf<TileItem> d_titems;
@@ -150,13 +151,22 @@ constexpr int NSIDE = 4;  // side streams that run class kernels concurrently
 
 constexpr int NO_FP32 = 99;  // bin-sum threshold that sends no quartet to FP32
 
+// Kernel variant of a class: untiled (ket-/bra-stationary or MD), tiled with
+// 4-wave blocks, tiled with the class's wide block.
+enum { VARIANT_UNTILED = 0, VARIANT_TILED4 = 1, VARIANT_TILED_WIDE = 2, NVARIANTS = 3 };
+
 struct vlx_context {
     int device = 0;
     int nstreams = NSIDE;           // side streams in use (VLX_STREAMS, 1..NSIDE)
     int max_chunk = BRA_CHUNK;      // bras per work item upper bound (VLX_BRA_CHUNK)
     int min_items = 16384;          // work items a class launch aims for (VLX_MIN_ITEMS)
     int use_tiled = 1;              // LDS-tiled kernels where built (VLX_TILED)
-    int tiled_lmax = 4;             // largest la+lb+lc+ld that uses the tiled kernel (VLX_TILED_LMAX)
+    // largest la+lb+lc+ld that may use the tiled kernel (VLX_TILED_LMAX).  The L = 5 tiled
+    // kernels are built but off by default: with four concurrent streams they left non-finite
+    // J entries in the 32-water def2-SVP build (not with one stream); cause not found.
+    int tiled_lmax = 4;
+    int force_variant = -1;         // autotuning: every class runs this variant (VARIANT_*)
+    std::map<int, int> tuned_variant;  // class entry -> VARIANT_* of the last autotune (default for new screeners)
     int group_atoms = 12;           // atoms per group of the tiled kernels (VLX_GROUP_ATOMS)
     int tiled_waves = 0;            // 4: 4-wave blocks for every tiled class; 0: per-class default (VLX_TILED_WAVES)
     int max_tile_lds = 96 * 1024;   // larger tiles fall back to the untiled kernel
@@ -206,6 +216,7 @@ struct vlx_screener {
     };
     std::vector<PairRef> pair_ref;  // nshells x nshells -> position in all[]
     std::map<int, std::vector<std::unique_ptr<ClassSchedule>>> schedules;  // by (ithreshold, FP32 bin-sum threshold)
+    std::map<int, int> variant;  // class entry -> VARIANT_* chosen by vlx_fock_autotune
 };
 
 namespace {
@@ -369,21 +380,24 @@ const DiagLauncher* find_diag(int x)
     return nullptr;
 }
 
-int label_factors(const char* label, double xf, double* fj, double* fk)
+// Digestion factors of a label and whether it uses erf(omega r)/r.
+// src/t4c_func/T4CLocalDistributor.cpp: 2jk (4, -1), 2jkx (4, -xf),
+// j (2, 0), k (0, 1), kx (0, xf); then G = F' + F'^T.  The range-separated
+// labels j_rs, k_rs, kx_rs digest like j, k, kx on erf-attenuated integrals
+// (src/t4c_func/T4CMatrixDistributor.cpp:51-54); omega is ignored otherwise.
+int label_factors(const char* label, double xf, double* fj, double* fk, bool* range_separated)
 {
-    // src/t4c_func/T4CLocalDistributor.cpp: 2jk (4, -1), 2jkx (4, -xf),
-    // j (2, 0), k (0, 1), kx (0, xf); then G = F' + F'^T.
     if (!label) return fail(VLX_ERR_ARG, "label is NULL");
     const std::string s(label);
+    *range_separated = false;
     if (s == "2jk") { *fj = 4.0; *fk = -1.0; }
     else if (s == "2jkx") { *fj = 4.0; *fk = -xf; }
-    else if (s == "j") { *fj = 2.0; *fk = 0.0; }
-    else if (s == "k") { *fj = 0.0; *fk = 1.0; }
-    else if (s == "kx") { *fj = 0.0; *fk = xf; }
-    else if (s == "j_rs" || s == "k_rs" || s == "kx_rs")
-        return fail(VLX_ERR_UNSUPPORTED, "range-separated label '%s' is not built", label);
+    else if (s == "j" || s == "j_rs") { *fj = 2.0; *fk = 0.0; }
+    else if (s == "k" || s == "k_rs") { *fj = 0.0; *fk = 1.0; }
+    else if (s == "kx" || s == "kx_rs") { *fj = 0.0; *fk = xf; }
     else
         return fail(VLX_ERR_ARG, "unknown Fock label '%s'", label);
+    *range_separated = s.size() > 3 && s.compare(s.size() - 3, 3, "_rs") == 0;
     return VLX_OK;
 }
 
@@ -453,12 +467,28 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
         const int* gm = s->grp_max;
         const int tile_lds = 8 * (gm[ce.la] * gm[ce.lb] + gm[ce.lc] * gm[ce.ld] + gm[ce.la] * gm[ce.lc] +
                                    gm[ce.la] * gm[ce.ld] + gm[ce.lb] * gm[ce.lc] + gm[ce.lb] * gm[ce.ld]);
-        if (s->ctx->use_tiled && ce.tiled && ce.la + ce.lb + ce.lc + ce.ld <= s->ctx->tiled_lmax &&
-            tile_lds <= s->ctx->max_tile_lds) {
+        const int L = ce.la + ce.lb + ce.lc + ce.ld;
+        const int nsph = (2 * ce.la + 1) * (2 * ce.lb + 1) * (2 * ce.lc + 1) * (2 * ce.ld + 1);
+        const bool can_tile = s->ctx->use_tiled && ce.tiled && L <= s->ctx->tiled_lmax && tile_lds <= s->ctx->max_tile_lds;
+        // default: tiled for L <= 4 (and for L = 5 blocks of <= 75 integrals when tiled_lmax
+        // allows it); the autotuner's choice, or the one being timed, overrides it
+        int var = (can_tile && (L <= 4 || nsph <= 75)) ? (s->ctx->tiled_waves == 4 ? VARIANT_TILED4 : VARIANT_TILED_WIDE)
+                                                      : VARIANT_UNTILED;
+        if (s->ctx->force_variant >= 0) {
+            var = s->ctx->force_variant;
+        } else {
+            const auto vit = s->variant.find(e);
+            const auto cit = s->ctx->tuned_variant.find(e);
+            if (vit != s->variant.end()) var = vit->second;
+            else if (cit != s->ctx->tuned_variant.end()) var = cit->second;
+        }
+        if (!can_tile) var = VARIANT_UNTILED;
+        if (var != VARIANT_UNTILED) {
             // tiled items: (bra segment prefix, chunk of <= 64 kets of a ket segment)
             const PairSet& tb = unscreened ? s->all_tiled_bra[ce.x] : s->kept_tiled_bra[ce.x];
             const PairSet& tk = unscreened ? s->all_tiled[ce.y] : s->kept_tiled[ce.y];
             sch->tiled = true;
+            sch->tiled_waves = (var == VARIANT_TILED4) ? 4 : 0;
             sch->tile_lds = tile_lds;
             for (size_t sk = 0; sk < tk.segs.size(); sk++) {
                 const auto& ks = tk.segs[sk];
@@ -622,6 +652,7 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_TILED_LMAX")) ctx->tiled_lmax = std::atoi(e);
     if (const char* e = std::getenv("VLX_GROUP_ATOMS")) ctx->group_atoms = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("VLX_TILED_WAVES")) ctx->tiled_waves = std::atoi(e);
+    if (const char* e = std::getenv("VLX_VARIANT")) ctx->force_variant = std::max(-1, std::min(NVARIANTS - 1, std::atoi(e)));
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (int k = 0; k < NSIDE; k++) {
@@ -1016,6 +1047,75 @@ int vlx_context_class_times(vlx_context* ctx, int max_records, int* nrecords, in
     return VLX_OK;
 }
 
+int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_density, int ithreshold, int repeats,
+                      int* nchosen, int* classes, int* variants)
+{
+    if (!ctx || !scr || !d_density) return fail(VLX_ERR_ARG, "NULL argument to vlx_fock_autotune");
+    if (scr->ctx != ctx) return fail(VLX_ERR_ARG, "screener belongs to another context");
+    if (repeats < 1) repeats = 1;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t nn = (size_t)scr->nao * scr->nao;
+    HIP_TRY(ctx->d_fock.alloc(nn));
+    // time every class under each variant on one stream (no overlap between classes)
+    const int saved_streams = ctx->nstreams;
+    const bool saved_timing = ctx->timing;
+    const double saved_tau = ctx->mixed_tau;
+    const int saved_force = ctx->force_variant;
+    ctx->nstreams = 1;
+    ctx->mixed_tau = 0.0;
+    std::map<int, double> best_ms;
+    std::map<int, int> best_var;
+    int rc = VLX_OK;
+    for (int v = 0; v < NVARIANTS && rc == VLX_OK; v++) {
+        ctx->force_variant = v;
+        scr->schedules.clear();
+        ctx->timing = false;
+        rc = vlx_fock_compute_device(ctx, scr, d_density, "2jk", 1.0, 0.0, ithreshold, ctx->d_fock.ptr);  // warm-up
+        if (rc) break;
+        rc = drain_timing(ctx);
+        if (rc) break;
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        ctx->class_ms.clear();
+        ctx->timing = true;
+        for (int r = 0; r < repeats && rc == VLX_OK; r++)
+            rc = vlx_fock_compute_device(ctx, scr, d_density, "2jk", 1.0, 0.0, ithreshold, ctx->d_fock.ptr);
+        if (rc) break;
+        rc = drain_timing(ctx);
+        if (rc) break;
+        for (const auto& kv : ctx->class_ms) {
+            const double ms = kv.second.first / repeats;
+            auto it = best_ms.find(kv.first);
+            if (it == best_ms.end() || ms < it->second) {
+                best_ms[kv.first] = ms;
+                best_var[kv.first] = v;
+            }
+        }
+    }
+    ctx->force_variant = saved_force;
+    ctx->nstreams = saved_streams;
+    ctx->timing = saved_timing;
+    ctx->mixed_tau = saved_tau;
+    ctx->class_ms.clear();
+    scr->schedules.clear();
+    if (rc) return rc;
+    scr->variant = best_var;
+    for (const auto& kv : best_var) ctx->tuned_variant[kv.first] = kv.second;
+    if (nchosen) *nchosen = static_cast<int>(best_var.size());
+    int r = 0;
+    for (const auto& kv : best_var) {
+        const auto& ce = g_class_table[kv.first];
+        if (classes) {
+            classes[4 * r] = ce.la;
+            classes[4 * r + 1] = ce.lb;
+            classes[4 * r + 2] = ce.lc;
+            classes[4 * r + 3] = ce.ld;
+        }
+        if (variants) variants[r] = kv.second;
+        r++;
+    }
+    return VLX_OK;
+}
+
 int vlx_fock_work(vlx_screener* scr, int ithreshold, int max_records, int* nrecords, int* classes,
                   long long* quartets, long long* prim_quartets, double* flops)
 {
@@ -1051,11 +1151,15 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
 {
     if (!ctx || !scr || !d_density || !d_fock_out) return fail(VLX_ERR_ARG, "NULL argument to vlx_fock_compute_device");
     if (scr->ctx != ctx) return fail(VLX_ERR_ARG, "screener belongs to another context");
-    if (omega != 0.0) return fail(VLX_ERR_UNSUPPORTED, "omega != 0 (erf-attenuated operator) is not built");
     if (ithreshold > 30) return fail(VLX_ERR_ARG, "ithreshold %d outside the accepted range (< 0 or 0..30)", ithreshold);
     double fj = 0.0, fk = 0.0;
-    int rc = label_factors(label, exchange_factor, &fj, &fk);
+    bool range_separated = false;
+    int rc = label_factors(label, exchange_factor, &fj, &fk, &range_separated);
     if (rc) return rc;
+    if (range_separated && !std::isfinite(omega)) return fail(VLX_ERR_ARG, "omega must be finite");
+    // erf(0 r)/r = 0: a range-separated build with omega = 0 is zero, as in the reference
+    const double omega2 = range_separated ? omega * omega : 0.0;
+    if (range_separated && omega2 == 0.0) fj = fk = 0.0;
     HIP_TRY(hipSetDevice(ctx->device));
     const int n = scr->nao;
     const size_t nn = (size_t)n * n;
@@ -1088,6 +1192,7 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
     HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
     for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamWaitEvent(ctx->side[k], ctx->ev_start, 0));
     for (const auto& schp : *list) {
+        if (fj == 0.0 && fk == 0.0) break;  // nothing to digest: G = 0
         const ClassSchedule& sch = *schp;
         const auto& ce = g_class_table[sch.entry];
         const PairSet& bra = sch.tiled ? (unscreened ? scr->all_tiled_bra[ce.x] : scr->kept_tiled_bra[ce.x])
@@ -1110,7 +1215,7 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
         a.grp_n = scr->d_grp_n.ptr;
         a.grp_ao = scr->d_grp_ao.ptr;
         a.tile_lds_bytes = sch.tile_lds;
-        a.tiled_waves = ctx->tiled_waves;
+        a.tiled_waves = sch.tiled_waves;
         a.sc.nitems = static_cast<long long>(sch.tiled ? sch.titems.size() : sch.items.size());
         a.sc.nket = ket.size();
         a.sc.ithreshold = ithreshold;
@@ -1118,6 +1223,7 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
         a.sc.nao = n;
         a.sc.fj = fj;
         a.sc.fk = fk;
+        a.sc.omega2 = omega2;
         const long long waves = std::min<long long>(a.sc.nitems, 8192);
         const int grid = sch.tiled ? static_cast<int>(std::min<long long>(a.sc.nitems, 4096))
                                    : static_cast<int>((waves + 3) / 4);

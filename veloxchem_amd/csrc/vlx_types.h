@@ -56,6 +56,7 @@ struct FockScalars {
     int nao;
     double fj;                     // coefficient of the Coulomb terms (0: none)
     double fk;                     // coefficient of the exchange terms (0: none)
+    double omega2;                 // omega^2 of the erf(omega r)/r operator (range-separated labels), 0: 1/r
 };
 
 struct FockArgs {

@@ -13,13 +13,16 @@ Mirrors the Python facade src/pymodule/fockdriver.py:37-135 and the pybind11
 * ``_set_block_size_factor(f)``.
 
 Labels: '2jk' -> 2J - K, '2jkx' -> 2J - xf K, 'j' -> J, 'k' -> K,
-'kx' -> xf K, with J_pq = sum_rs (pq|rs) D_rs and K_pr = sum_qs (pq|rs) D_qs.
+'kx' -> xf K, with J_pq = sum_rs (pq|rs) D_rs and K_pr = sum_qs (pq|rs) D_qs;
+'j_rs', 'k_rs', 'kx_rs' the same on erf(omega r)/r integrals
+(src/t4c_func/T4CMatrixDistributor.cpp:51-54).  omega is ignored by the
+other labels, as in the reference.
 
 Deviations from the reference, by design: an unknown label raises
 ValueError (the reference silently returns zeros,
-src/t4c_func/T4CUtils.cpp:814-1062); omega != 0 and non-symmetric densities
-raise NotImplementedError/ValueError instead of computing; ithreshold must be
-< 0 (unscreened) or 0..30.
+src/t4c_func/T4CUtils.cpp:814-1062); non-symmetric densities raise
+NotImplementedError instead of computing; ithreshold must be < 0 (unscreened)
+or 0..30.
 """
 
 import ctypes
@@ -76,6 +79,25 @@ class FockDriver:
             raise ValueError('FockDriver._set_block_size_factor: Invalid factor')
         self._block_size_factor = int(factor)
         _native.check(_native.lib().vlx_set_block_size_factor(_native.context(self._device), int(factor)))
+
+    VARIANTS = ('untiled', 'tiled4', 'tiled-wide')
+
+    def _autotune(self, screener, density, ithreshold=12, repeats=2):
+        """Times each angular class under every kernel variant on this density and keeps
+        the fastest per class (vlx_fock_autotune); returns {(la, lb, lc, ld): variant}."""
+        dm = density.to_numpy() if hasattr(density, 'to_numpy') else np.asarray(density, dtype=np.float64)
+        buf = _native.DeviceArray(dm.shape, self._device)
+        try:
+            buf.upload(dm)
+            n = ctypes.c_int()
+            cls = np.zeros(4 * 64, dtype=np.int32)
+            var = np.zeros(64, dtype=np.int32)
+            _native.check(_native.lib().vlx_fock_autotune(
+                _native.context(self._device), screener.handle, buf.ptr, int(ithreshold), int(repeats),
+                ctypes.byref(n), cls.ctypes.data_as(_native._c_int_p), var.ctypes.data_as(_native._c_int_p)))
+        finally:
+            buf.free()
+        return {tuple(int(x) for x in cls[4 * r:4 * r + 4]): self.VARIANTS[int(var[r])] for r in range(n.value)}
 
     def compute_eri(self, molecule, basis, eri_thresh=1.0e-12):
         """Full AO ERI tensor (pq|rs) (src/t4c_func/FockDriver.cpp:76-117).
