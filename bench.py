@@ -27,6 +27,16 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 FP64_VALU_PEAK_TFLOPS = 78.6
+# HBM bytes per launch per kernel, from FETCH_SIZE/WRITE_SIZE passes scaled by a copy calibration
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'fock_build_water32_svp_traffic.json')
+
+
+def kernel_traffic(name):
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            return json.load(fh)['bytes_per_launch'].get(name)
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def build_workload(nmol, box, seed, basis):
@@ -178,6 +188,7 @@ def main():
     dom_ms = times[dom][0] / times[dom][1]
     dom_flops = by_class[dom][3]
     achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    dom_kernel = _native.class_kernel(scr, args.ithreshold, dom)
     build_tflops = flops_build / (ms_per_step * 1e-3) / 1e12
 
     result = {
@@ -205,12 +216,12 @@ def main():
         },
         'roofline': {
             'bound': 'valu',
-            'kernel': 'fock_kernel<%d,%d,%d,%d>' % dom,
+            'kernel': dom_kernel,
             'achieved': achieved,
             'peak': FP64_VALU_PEAK_TFLOPS,
             'unit': 'TFLOP/s',
             'frac': achieved / FP64_VALU_PEAK_TFLOPS,
-            'traffic': None,
+            'traffic': kernel_traffic(dom_kernel),
             'kernel_ms': dom_ms,
             'kernel_share_of_build': times[dom][0] / sum(v[0] for v in times.values()),
             'build_achieved': build_tflops,

@@ -148,6 +148,11 @@ int vlx_fock_schedule_stats(vlx_screener* scr, int ithreshold, int max_records, 
 int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_density, int ithreshold, int repeats,
                       int* nchosen, int* classes, int* variants);
 
+/* Kernel (template name as profilers report it) that a class (la lb|lc ld)
+ * runs with at ithreshold, written to name[0..len). */
+int vlx_fock_class_kernel(vlx_context* ctx, vlx_screener* scr, int ithreshold, int la, int lb, int lc, int ld,
+                          char* name, int len);
+
 /* Per-class kernel timing with HIP events on the context's stream
  * (off by default).  vlx_context_class_times synchronises the stream and
  * returns, per class (la, lb, lc, ld), the summed kernel time in ms and the

@@ -73,6 +73,8 @@ def lib():
         L.vlx_eri_quartets.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_double_p]
         L.vlx_fock_autotune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                         _c_int_p, _c_int_p, _c_int_p]
+        L.vlx_fock_class_kernel.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.vlx_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         L.vlx_context_last_fp32_bin_sum.argtypes = [ctypes.c_void_p, _c_int_p]
         L.vlx_fock_precision_split.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_ll_p, _c_ll_p]
@@ -123,6 +125,14 @@ def last_fp32_bin_sum(device=0):
     k = ctypes.c_int()
     check(lib().vlx_context_last_fp32_bin_sum(context(device), ctypes.byref(k)))
     return k.value
+
+
+def class_kernel(screener, ithreshold, cls, device=0):
+    """Name of the kernel that class cls = (la, lb, lc, ld) runs with, as profilers report it."""
+    buf = ctypes.create_string_buffer(128)
+    check(lib().vlx_fock_class_kernel(context(device), screener.handle, int(ithreshold), *[int(x) for x in cls],
+                                      buf, len(buf)))
+    return buf.value.decode()
 
 
 def class_times(device=0):

@@ -1082,7 +1082,14 @@ int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_densi
         if (rc) break;
         rc = drain_timing(ctx);
         if (rc) break;
+        // classes that cannot run this variant ran untiled: they only count for VARIANT_UNTILED
+        std::map<int, bool> ran_tiled;
+        std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
+        rc = get_schedules(scr, ithreshold, NO_FP32, &list);
+        if (rc) break;
+        for (const auto& sp : *list) ran_tiled[sp->entry] = sp->tiled;
         for (const auto& kv : ctx->class_ms) {
+            if ((v != VARIANT_UNTILED) != ran_tiled[kv.first]) continue;
             const double ms = kv.second.first / repeats;
             auto it = best_ms.find(kv.first);
             if (it == best_ms.end() || ms < it->second) {
@@ -1114,6 +1121,32 @@ int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_densi
         r++;
     }
     return VLX_OK;
+}
+
+int vlx_fock_class_kernel(vlx_context* ctx, vlx_screener* scr, int ithreshold, int la, int lb, int lc, int ld,
+                          char* name, int len)
+{
+    if (!ctx || !scr || !name || len < 1) return fail(VLX_ERR_ARG, "NULL argument");
+    std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
+    int rc = get_schedules(scr, ithreshold, NO_FP32, &list);
+    if (rc) return rc;
+    for (const auto& sp : *list) {
+        const auto& ce = g_class_table[sp->entry];
+        if (ce.la != la || ce.lb != lb || ce.lc != lc || ce.ld != ld) continue;
+        if (sp->tiled) {
+            const int L = la + lb + lc + ld;
+            const int wide = (L <= 2) ? 16 : (L == 3 ? 8 : 4);
+            std::snprintf(name, len, "fock_kernel_tiled<%d,%d,%d,%d,double,%d>", la, lb, lc, ld,
+                          sp->tiled_waves == 4 ? 4 : wide);
+        } else {
+            const char* k = (ce.variant == VLX_KET_STATIONARY) ? "fock_kernel"
+                            : (ce.variant == VLX_WAVE_MD)       ? "fock_kernel_md"
+                                                                : "fock_kernel_bs";
+            std::snprintf(name, len, "%s<%d,%d,%d,%d>", k, la, lb, lc, ld);
+        }
+        return VLX_OK;
+    }
+    return fail(VLX_ERR_ARG, "class (%d%d|%d%d) has no work at this threshold", la, lb, lc, ld);
 }
 
 int vlx_fock_work(vlx_screener* scr, int ithreshold, int max_records, int* nrecords, int* classes,
