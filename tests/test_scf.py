@@ -131,3 +131,24 @@ def test_scf_dimer_gpu():
     drv = ScfRestrictedDriver()
     res = drv.compute(mol, MolecularBasis.read(mol, 'def2-svp'))
     assert abs(res['scf_energy'] - (-151.9220860932)) < 1.0e-8
+
+
+def test_canonical_purification_matches_eigh_projector():
+    import torch
+    from veloxchem_amd.scfdriver import canonical_purification
+    rng = np.random.default_rng(3)
+    m, nocc = 40, 11
+    q, _ = np.linalg.qr(rng.standard_normal((m, m)))
+    evals = np.concatenate([np.sort(rng.uniform(-20.0, -0.5, nocc)), np.sort(rng.uniform(0.2, 15.0, m - nocc))])
+    fo = (q * evals) @ q.T
+    d = canonical_purification(torch.from_numpy(fo), nocc).numpy()
+    ref = q[:, :nocc] @ q[:, :nocc].T
+    assert np.max(np.abs(d - ref)) < 1e-10
+    assert abs(np.trace(d) - nocc) < 1e-10
+    # degenerate HOMO/LUMO: either no projector (the caller diagonalises) or, once rounding has
+    # split the pair, a valid idempotent projector with the right trace
+    evals[nocc] = evals[nocc - 1]
+    d = canonical_purification(torch.from_numpy((q * evals) @ q.T), nocc)
+    if d is not None:
+        d = d.numpy()
+        assert np.max(np.abs(d @ d - d)) < 1e-8 and abs(np.trace(d) - nocc) < 1e-8

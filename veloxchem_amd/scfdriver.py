@@ -71,6 +71,40 @@ def c2diis_weights(errors):
     return best
 
 
+def canonical_purification(fo, nocc, tol=1e-13, max_iter=200):
+    """Occupied-space projector of a symmetric matrix (orthonormal basis) without diagonalisation.
+
+    Trace-conserving canonical purification (A. H. R. Palser, D. E. Manolopoulos, Phys. Rev. B 58,
+    12704 (1998)): start from a linear map of fo with trace nocc and spectrum in [0, 1] (Gershgorin
+    bounds), then iterate the cubic that fixes the trace, two matrix products per step, until
+    tr(D - D^2) ~ 0.  Returns None when it does not converge (no gap), so the caller falls back to
+    eigh.
+    """
+    m = fo.shape[0]
+    if nocc <= 0 or nocc >= m:
+        return None
+    diag = torch.diagonal(fo)
+    rad = fo.abs().sum(dim=1) - diag.abs()
+    bounds = torch.stack([(diag - rad).min(), (diag + rad).max(), torch.trace(fo) / m]).tolist()
+    emin, emax, mu = bounds
+    lam = min(nocc / (emax - mu), (m - nocc) / (mu - emin))
+    eye = torch.eye(m, dtype=fo.dtype, device=fo.device)
+    d = (lam / m) * (mu * eye - fo) + (nocc / m) * eye
+    for _ in range(max_iter):
+        d2 = d @ d
+        d3 = d2 @ d
+        t = torch.stack([torch.trace(d), torch.trace(d2), torch.trace(d3)]).tolist()
+        denom = t[0] - t[1]
+        if abs(denom) <= tol * m:
+            return d
+        c = (t[1] - t[2]) / denom
+        if c >= 0.5:
+            d = ((1.0 + c) * d2 - d3) / c
+        else:
+            d = ((1.0 - 2.0 * c) * d + (1.0 + c) * d2 - d3) / (1.0 - c)
+    return None
+
+
 class _DeviceFock:
     """G = 2J - K of a device density through vlx_fock_compute_device (same buffers, no host copies)."""
 
@@ -102,6 +136,9 @@ class ScfRestrictedDriver:
         self.ovl_thresh = 1.0e-6
         self.diis_thresh = 1000.0
         self.eri_thresh = 1.0e-12
+        # density from the effective Fock matrix: 'purification' (matrix products only, eigh once
+        # per stage for the orbitals) or 'eigh' every iteration (as the reference)
+        self.density_method = 'purification'
         # fock_2e(screener, D, thresh_int) -> G on D's device; default: the GPU build
         self.fock_2e = None
         # 'fp64', or 'mixed': FP32 recursions where Schwarz bound x max|D| < mixed_threshold
@@ -228,10 +265,11 @@ class ScfRestrictedDriver:
         self.timings['screener'] += time.perf_counter() - t
         nocc = molecule.number_of_alpha_electrons()
         focks, errs = deque(), deque()
+        bmat = np.zeros((0, 0))
         e_grad = None
         e_old = 0.0
         self._is_converged = False
-        C = eps = F = None
+        C = eps = F = last_f_eff = None
         fmax = 1.0 / math.sqrt(self.ovl_thresh)
         for it in range(max_iter + 1):
             self._num_iter = it
@@ -259,28 +297,48 @@ class ScfRestrictedDriver:
                 if len(focks) == self.max_err_vecs:
                     focks.popleft()
                     errs.popleft()
-                # the error vector of a stored (F, D) pair is this iteration's err
+                    bmat = bmat[1:, 1:]
+                # the error vector of a stored (F, D) pair is this iteration's err; the B matrix
+                # grows by one row of dot products (elementwise products and sums: a k x N^2 GEMM
+                # with k <= 10 is a pathological BLAS shape)
+                dots = torch.stack([torch.sum(e * err) for e in errs] + [torch.sum(err * err)]).cpu().numpy()
+                k = len(errs)
+                grown = np.zeros((k + 1, k + 1))
+                grown[:k, :k] = bmat
+                grown[k, :] = dots
+                grown[:, k] = dots
+                bmat = grown
                 focks.append(F)
                 errs.append(err)
             if len(focks) > 1:
-                E = torch.stack([e.reshape(-1) for e in errs])
-                wts = c2diis_weights({'bmat': (E @ E.T).cpu().numpy()})
-                f_eff = torch.tensordot(torch.from_numpy(np.ascontiguousarray(wts)).to(dev),
-                                        torch.stack(list(focks)), dims=1)
+                wts = c2diis_weights({'bmat': bmat})
+                f_eff = torch.zeros_like(F)
+                for wi, fi in zip(wts, focks):
+                    f_eff.add_(fi, alpha=float(wi))
             elif len(focks) == 1:
                 f_eff = focks[0]
             else:
                 f_eff = F
-            eps, vecs = torch.linalg.eigh(X.T @ f_eff @ X)
-            C = X @ vecs
-            # drop MOs with coefficients above 1/sqrt(ovl_thresh) (scfdriver.py:2724-2747)
-            big = C.abs().amax(dim=0) > fmax
-            if bool(big.any()):
-                C, eps = C[:, ~big], eps[~big]
-            den = (C[:, :nocc] @ C[:, :nocc].T).contiguous()
+            fo = X.T @ f_eff @ X
+            dp = canonical_purification(fo, nocc) if self.density_method == 'purification' else None
+            if dp is not None:
+                den = (X @ dp @ X.T).contiguous()
+                C = None
+            else:
+                eps, vecs = torch.linalg.eigh(fo)
+                C = X @ vecs
+                # drop MOs with coefficients above 1/sqrt(ovl_thresh) (scfdriver.py:2724-2747)
+                big = C.abs().amax(dim=0) > fmax
+                if bool(big.any()):
+                    C, eps = C[:, ~big], eps[~big]
+                den = (C[:, :nocc] @ C[:, :nocc].T).contiguous()
+            last_f_eff = f_eff
             self.timings['linear_algebra'] += time.perf_counter() - t
         if C is None:
-            eps, vecs = torch.linalg.eigh(X.T @ F @ X)
+            # orbitals of the last effective Fock matrix (the one that produced the density)
+            t = time.perf_counter()
+            eps, vecs = torch.linalg.eigh(X.T @ (F if last_f_eff is None else last_f_eff) @ X)
             C = X @ vecs
+            self.timings['linear_algebra'] += time.perf_counter() - t
         return {'scf_energy': self._scf_energy, 'C_alpha': C.cpu().numpy(), 'E_alpha': eps.cpu().numpy(),
                 'D_alpha': den.cpu().numpy(), 'F_alpha': F.cpu().numpy(), 'S': S_np}
