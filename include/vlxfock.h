@@ -53,6 +53,7 @@ int vlx_set_block_size_factor(vlx_context* ctx, int factor);
  * `threshold` (decided per tile of kets at the screener's decimal bins:
  * bin_ab + bin_cd >= ceil(2 + log10(max|D| / threshold))), for the classes
  * with the LDS-tiled kernel (la+lb+lc+ld <= 4); J/K accumulation stays FP64.
+ * Range-separated labels (*_rs) always run in FP64.
  */
 enum { VLX_PRECISION_FP64 = 0, VLX_PRECISION_MIXED = 1 };
 int vlx_set_precision(vlx_context* ctx, int mode, double threshold);
@@ -88,11 +89,13 @@ int vlx_screener_number_of_pairs(const vlx_screener* scr, long long* out);
 /*
  * CFockDriver::compute(screener, D, label, exchange_factor, omega, ithreshold)
  * (src/t4c_func/FockDriver.cpp:120-171; rank/nodes variant :234-289).
- * label: "2jk" (2J - K), "2jkx" (2J - xf K), "j" (J), "k" (K), "kx" (xf K).
+ * label: "2jk" (2J - K), "2jkx" (2J - xf K), "j" (J), "k" (K), "kx" (xf K),
+ * and the range-separated "j_rs", "k_rs", "kx_rs" over erf(omega r)/r
+ * (omega = 0 gives a zero matrix there; omega is ignored by the other labels).
  * ithreshold >= 0 applies the binned Schwarz selection bin_bra + bin_ket <=
  * ithreshold; ithreshold < 0 computes every shell quartet (the reference's
- * unscreened _compute_fock_omp(basis, molecule, ...) path).  omega must be 0
- * (range-separated labels are not built).  D must be symmetric.
+ * unscreened _compute_fock_omp(basis, molecule, ...) path).  D must be
+ * symmetric.
  * Host buffers: density in, G out (both nao x nao).
  */
 int vlx_fock_compute(vlx_context* ctx, vlx_screener* scr, const double* density, const char* label,

@@ -72,7 +72,7 @@ struct EriClass;
 // F_{L+j}(T_k) (-1)^j / j!), then downward recursion
 // F_m = (2T F_{m+1} + e^-T) / (2m+1).  T >= BOYS_TMAX: F_0 = sqrt(pi/T)/2
 // (erf(sqrt(T)) = 1 to double precision there) and upward recursion
-// F_{m+1} = ((2m+1) F_m - e^-T) / (2T), which is stable for T > L.
+// F_{m+1} = ((2m+1) F_m - e^-T) / (2T), stable for T > L.
 // (-1)^n / n!
 constexpr double exp_series_coef(int n)
 {
@@ -105,24 +105,40 @@ __device__ __forceinline__ void boys_eval(const R T, const R* __restrict__ tab, 
             for (int m = L - 1; m >= 0; m--) F[m] = (R(2.0) * T * F[m + 1] + et) * R(1.0 / (2 * m + 1));
         }
     } else {
-        const R et = vexp<R>(-T);
         const R rs = vrsqrt<R>(T);
         const R it = rs * rs;
         F[0] = R(0.88622692545275801365) * rs;  // sqrt(pi) / 2 / sqrt(T)
+        if constexpr (L > 0) {
+            // F_{m+1} = ((2m+1) F_m - e^-T) / (2T).  Next to (2m+1) F_m the e^-T term
+            // weighs at most e^-36 / ((2m+1) F_m(36)): 3e-10 for m < 6, 1e-6 for m = 11.
+            // L <= 6: e^-T to ~1e-7 relative is enough, as 2^x with x = -T log2(e)
+            // split in FP64 into n + f (0 <= f < 1) and 2^f from the hardware FP32
+            // exp2; in FP32 the term is below rounding and is dropped.  Higher L
+            // takes the full-precision exponential.
+            R et = R(0);
+            if constexpr (L > 6) {
+                et = vexp<R>(-T);
+            } else if constexpr (sizeof(R) == 8) {
+                const double x = -T * 1.4426950408889634;
+                const double nf = floor(x);
+                const float f = static_cast<float>(x - nf);
+                et = ldexp(static_cast<double>(__builtin_amdgcn_exp2f(f)), static_cast<int>(nf));
+            }
 #pragma unroll
-        for (int m = 0; m < L; m++) F[m + 1] = (R(2 * m + 1) * F[m] - et) * (R(0.5) * it);
+            for (int m = 0; m < L; m++) F[m + 1] = (R(2 * m + 1) * F[m] - et) * (R(0.5) * it);
+        }
     }
 }
 
-// FP64 quartet (all FP64 kernels)
-template <int la, int lb, int lc, int ld>
+// FP64 quartet (all FP64 kernels); RS selects the erf(omega r)/r operator
+template <int la, int lb, int lc, int ld, bool RS = false>
 __device__ __forceinline__ void eri_quartet(const PrimPair* __restrict__ bra_prims, const int bra_np, const double ABx,
                                             const double ABy, const double ABz, const PrimPair* __restrict__ ket_prims,
                                             const int ket_np, const double CDx, const double CDy, const double CDz,
                                             const double* __restrict__ boys_tab, const double omega2,
                                             double* __restrict__ out)
 {
-    EriClass<la, lb, lc, ld>::template quartet<double>(bra_prims, bra_np, ABx, ABy, ABz, ket_prims, ket_np, CDx, CDy,
+    EriClass<la, lb, lc, ld>::template quartet<double, RS>(bra_prims, bra_np, ABx, ABy, ABz, ket_prims, ket_np, CDx, CDy,
                                                        CDz, boys_tab, omega2, out);
 }
 
@@ -200,7 +216,7 @@ __device__ __forceinline__ void flush_block(double* __restrict__ F, const int n,
 
 // Pointers are separate __restrict__ kernel arguments so that wave-uniform
 // loads (work items, bra pairs and primitives, D_ab) become scalar loads.
-template <int la, int lb, int lc, int ld>
+template <int la, int lb, int lc, int ld, bool RS>
 __global__ void __launch_bounds__(VLX_BLOCK)
     fock_kernel(const PairInfo* __restrict__ bra, const PrimPair* __restrict__ bra_prims,
                 const PairInfo* __restrict__ ket, const PrimPair* __restrict__ ket_prims,
@@ -270,9 +286,9 @@ __global__ void __launch_bounds__(VLX_BLOCK)
             if (kp.same) s *= 0.5;
             if (a.same_class && bp.rank == j) s *= 0.5;
             double I[nsph];
-            eri_quartet<la, lb, lc, ld>(bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
-                                        ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz,
-                                        tab, a.omega2, I);
+            eri_quartet<la, lb, lc, ld, RS>(bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
+                                            ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz,
+                                            tab, a.omega2, I);
 #pragma unroll
             for (int x = 0; x < nsph; x++) I[x] *= s;
             int q[nb];
@@ -371,7 +387,7 @@ __global__ void __launch_bounds__(VLX_BLOCK)
 // {j0, i0, i1}: kets [j0, j0 + 64) of bra i0, i1 = number of kets of bra i0.
 // F'_ab is reduced over the wavefront; F'_cd, F'_ac, F'_ad, F'_bc and F'_bd
 // are per-lane atomics.
-template <int la, int lb, int lc, int ld>
+template <int la, int lb, int lc, int ld, bool RS>
 __global__ void __launch_bounds__(VLX_BLOCK)
     fock_kernel_bs(const PairInfo* __restrict__ bra, const PrimPair* __restrict__ bra_prims,
                    const PairInfo* __restrict__ ket, const PrimPair* __restrict__ ket_prims,
@@ -401,9 +417,9 @@ __global__ void __launch_bounds__(VLX_BLOCK)
         if (kp.same) s *= 0.5;
         if (a.same_class && bp.rank == j) s *= 0.5;
         double I[nsph];
-        eri_quartet<la, lb, lc, ld>(bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
-                                    ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz, tab,
-                                    a.omega2, I);
+        eri_quartet<la, lb, lc, ld, RS>(bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
+                                        ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz, tab,
+                                        a.omega2, I);
 #pragma unroll
         for (int x = 0; x < nsph; x++) I[x] *= s;
         int p[na], q[nb], r[nc], t[nd];
@@ -513,7 +529,7 @@ __device__ __forceinline__ void flush_tile(double* __restrict__ F, const int n, 
     }
 }
 
-template <int la, int lb, int lc, int ld, typename R, int NW>
+template <int la, int lb, int lc, int ld, typename R, int NW, bool RS>
 __global__ void __launch_bounds__(NW * VLX_WAVE)
     fock_kernel_tiled(const PairInfo* __restrict__ pairs_bra, const PrimPair* __restrict__ bra_prims,
                       const PairInfo* __restrict__ pairs_ket, const PrimPair* __restrict__ ket_prims,
@@ -569,6 +585,14 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
         for (int k = 0; k < nc; k++)
 #pragma unroll
             for (int l = 0; l < nd; l++) dcd[k][l] = R(D[(size_t)r[k] * n + t[l]]);
+        // J_CD depends on the lane's ket only: small ket blocks accumulate it in
+        // registers over all bras of the item (one LDS atomic per element at the end)
+        constexpr bool jcd_in_regs = nc * nd <= 9;
+        double jcd[jcd_in_regs ? nc : 1][jcd_in_regs ? nd : 1];
+#pragma unroll
+        for (int k = 0; k < (jcd_in_regs ? nc : 1); k++)
+#pragma unroll
+            for (int l = 0; l < (jcd_in_regs ? nd : 1); l++) jcd[k][l] = 0.0;
 
         for (int ib = ti.i0 + wave; ib < ti.i1; ib += NW) {
             const PairInfo bp = pairs_bra[ib];
@@ -580,7 +604,7 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
             if (kp.same) s *= R(0.5);
             if (a.same_class && bp.rank == kp.rank) s *= R(0.5);
             R I[nsph];
-            EriClass<la, lb, lc, ld>::template quartet<R>(bra_prims + bp.prim_start, bp.nprim, R(bp.abx), R(bp.aby),
+            EriClass<la, lb, lc, ld>::template quartet<R, RS>(bra_prims + bp.prim_start, bp.nprim, R(bp.abx), R(bp.aby),
                                                           R(bp.abz), ket_prims + kp.prim_start + ke.y, active ? 1 : 0,
                                                           R(kp.abx), R(kp.aby), R(kp.abz), tab, R(a.omega2), I);
 #pragma unroll
@@ -604,7 +628,7 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                         // one wave owns bra ib within the item: a plain store, no atomic
                         if (lane == 0) t_jab[(bp.loc_a + i) * nB + bp.loc_b + jb] = static_cast<double>(v);
                     }
-                if (active) {
+                if (jcd_in_regs || active) {
                     R vcd[nc][nd];
 #pragma unroll
                     for (int k = 0; k < nc; k++)
@@ -621,11 +645,19 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                                 for (int l = 0; l < nd; l++)
                                     vcd[k][l] = fma(I[((i * nb + jb) * nc + k) * nd + l], dab, vcd[k][l]);
                         }
+                    if constexpr (jcd_in_regs) {
+                        // inactive lanes add zeros (I is scaled by 0)
 #pragma unroll
-                    for (int k = 0; k < nc; k++)
+                        for (int k = 0; k < nc; k++)
 #pragma unroll
-                        for (int l = 0; l < nd; l++)
-                            unsafeAtomicAdd(&t_jcd[rl[k] * nD + tl[l]], static_cast<double>(vcd[k][l]));
+                            for (int l = 0; l < nd; l++) jcd[k][l] += static_cast<double>(vcd[k][l]);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < nc; k++)
+#pragma unroll
+                            for (int l = 0; l < nd; l++)
+                                unsafeAtomicAdd(&t_jcd[rl[k] * nD + tl[l]], static_cast<double>(vcd[k][l]));
+                    }
                 }
             }
             if (do_k && active) {
@@ -675,6 +707,15 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                         unsafeAtomicAdd(&t_kbd[(bp.loc_b + jb) * nD + tl[l]], static_cast<double>(v));
                     }
                 }
+            }
+        }
+        if constexpr (jcd_in_regs) {
+            if (do_j && lane_ok) {
+#pragma unroll
+                for (int k = 0; k < nc; k++)
+#pragma unroll
+                    for (int l = 0; l < nd; l++)
+                        if (jcd[k][l] != 0.0) unsafeAtomicAdd(&t_jcd[rl[k] * nD + tl[l]], jcd[k][l]);
             }
         }
         __syncthreads();
@@ -739,12 +780,18 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
     if (active) a.maxval[pi] = mx;
 }
 
-// kernel: fock_kernel (ket-stationary) or fock_kernel_bs (bra-stationary)
+// kernel: fock_kernel (ket-stationary) or fock_kernel_bs (bra-stationary);
+// the erf(omega r)/r instantiation runs when FockScalars::omega2 > 0
 #define VLX_INSTANTIATE_CLASS(la, lb, lc, ld, kernel)                                                   \
     void launch_fock_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                       \
     {                                                                                                   \
-        hipLaunchKernelGGL((kernel<la, lb, lc, ld>), dim3(grid), dim3(VLX_BLOCK), 0, s, a.bra,          \
-                           a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F, a.sc);       \
+        if (a.sc.omega2 > 0.0)                                                                          \
+            hipLaunchKernelGGL((kernel<la, lb, lc, ld, true>), dim3(grid), dim3(VLX_BLOCK), 0, s, a.bra, \
+                               a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F, a.sc);   \
+        else                                                                                            \
+            hipLaunchKernelGGL((kernel<la, lb, lc, ld, false>), dim3(grid), dim3(VLX_BLOCK), 0, s,      \
+                               a.bra, a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F,   \
+                               a.sc);                                                                   \
     }                                                                                                   \
     void launch_eri_##la##lb##lc##ld(const EriArgs& a, hipStream_t s)                                   \
     {                                                                                                   \
@@ -754,21 +801,30 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
 
 // LDS-tiled launcher (dynamic LDS = a.tile_lds_bytes); grid-stride over items
 // Tiled kernels: NWB waves per block (the item's bras split over them; the
-// LDS tiles are shared), or 4 waves when FockArgs::tiled_waves == 4.
-#define VLX_TILED_LAUNCH(la, lb, lc, ld, R, NW, tab)                                                    \
-    hipLaunchKernelGGL((fock_kernel_tiled<la, lb, lc, ld, R, NW>), dim3(grid), dim3((NW) * VLX_WAVE),  \
+// LDS tiles are shared), or 4 waves when FockArgs::tiled_waves == 4.  The
+// FP64 launcher picks the erf(omega r)/r instantiation when omega2 > 0; the
+// FP32 launcher has the Coulomb one only (range-separated builds run no FP32
+// items).
+#define VLX_TILED_LAUNCH(la, lb, lc, ld, R, NW, RS, tab)                                                \
+    hipLaunchKernelGGL((fock_kernel_tiled<la, lb, lc, ld, R, NW, RS>), dim3(grid), dim3((NW) * VLX_WAVE), \
                        a.tile_lds_bytes, s, a.bra, a.bra_prims, a.ket, a.ket_prims, a.titems, a.kentries,  \
                        tab, a.D, a.F, a.grp_off, a.grp_n, a.grp_ao, a.sc)
 #define VLX_INSTANTIATE_TILED(la, lb, lc, ld, NWB)                                                      \
     void launch_tiled_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                      \
     {                                                                                                   \
-        if (a.tiled_waves == 4 || (NWB) == 4) VLX_TILED_LAUNCH(la, lb, lc, ld, double, 4, a.boys_tab);  \
-        else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, a.boys_tab);                                 \
+        const bool w4 = a.tiled_waves == 4 || (NWB) == 4;                                               \
+        if (a.sc.omega2 > 0.0) {                                                                        \
+            if (w4) VLX_TILED_LAUNCH(la, lb, lc, ld, double, 4, true, a.boys_tab);                      \
+            else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, true, a.boys_tab);                       \
+        } else {                                                                                        \
+            if (w4) VLX_TILED_LAUNCH(la, lb, lc, ld, double, 4, false, a.boys_tab);                     \
+            else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, false, a.boys_tab);                      \
+        }                                                                                               \
     }                                                                                                   \
     void launch_tiled32_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                    \
     {                                                                                                   \
-        if (a.tiled_waves == 4 || (NWB) == 4) VLX_TILED_LAUNCH(la, lb, lc, ld, float, 4, a.boys_tab_f); \
-        else VLX_TILED_LAUNCH(la, lb, lc, ld, float, NWB, a.boys_tab_f);                                \
+        if (a.tiled_waves == 4 || (NWB) == 4) VLX_TILED_LAUNCH(la, lb, lc, ld, float, 4, false, a.boys_tab_f); \
+        else VLX_TILED_LAUNCH(la, lb, lc, ld, float, NWB, false, a.boys_tab_f);                         \
     }
 
 #define VLX_INSTANTIATE_DIAG(la, lb)                                                                    \

@@ -156,7 +156,9 @@ def gen_class(la, lb, lc, ld):
       f'{len(vrr.lines)} VRR intermediates per primitive quartet')
     w('template <>')
     w(f'struct EriClass<{la}, {lb}, {lc}, {ld}> {{')
-    w('template <typename R>')
+    # RS: erf(omega r)/r (range-separated) attenuation, a compile-time switch so
+    # that plain Coulomb builds carry no per-primitive attenuation arithmetic
+    w('template <typename R, bool RS>')
     w('static __device__ __forceinline__ void quartet(')
     w('    const PrimPair* __restrict__ bra_prims, const int bra_np, const R ABx, const R ABy, const R ABz,')
     w('    const PrimPair* __restrict__ ket_prims, const int ket_np, const R CDx, const R CDy, const R CDz,')
@@ -181,16 +183,18 @@ def gen_class(la, lb, lc, ld):
     w('            const R pref = R(bp.k * kp.k) * rs;')
     w(f'            R F[{L + 1}];')
     # erf(omega r)/r: F_m(T) -> s^(m+1/2) F_m(s T), s = omega^2 / (omega^2 + rho)
-    w('            const bool attenuated = omega2 > R(0);')
-    w('            const R sfac = attenuated ? omega2 * vrcp<R>(omega2 + rho) : R(1);')
-    w(f'            boys_eval<{L}, R>(attenuated ? T * sfac : T, boys_tab, F);')
-    w('            if (attenuated) {')
+    w('            if constexpr (RS) {')
+    w('                const R sfac = omega2 * vrcp<R>(omega2 + rho);')
+    w(f'                boys_eval<{L}, R>(T * sfac, boys_tab, F);')
     w('                R fm = vsqrt<R>(sfac);')
     w('#pragma unroll')
     w(f'                for (int m = 0; m <= {L}; m++) {{')
     w('                    F[m] *= fm;')
     w('                    fm *= sfac;')
     w('                }')
+    w('            } else {')
+    w(f'                boys_eval<{L}, R>(T, boys_tab, F);')
+    w('                (void)omega2;')
     w('            }')
     for m in range(L + 1):
         w(f'            const R b{m} = pref * F[{m}];')

@@ -1200,9 +1200,10 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
     zero_kernel<<<std::min<size_t>((nn + 255) / 256, 4096), 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, nn);
     HIP_TRY(hipGetLastError());
     // mixed precision: FP32 where Schwarz bound x max|D| < tau, i.e. for bin sums
-    // >= k32 = ceil(2 + log10(max|D| / tau)) (bins: sqrt(max) < 10^(1 - bin))
+    // >= k32 = ceil(2 + log10(max|D| / tau)) (bins: sqrt(max) < 10^(1 - bin));
+    // range-separated builds stay in FP64 (the FP32 kernels are Coulomb-only)
     int k32 = NO_FP32;
-    if (ctx->mixed_tau > 0.0 && ithreshold >= 0) {
+    if (ctx->mixed_tau > 0.0 && ithreshold >= 0 && omega2 == 0.0) {
         HIP_TRY(ctx->d_dmax.alloc(1));
         clear_u64_kernel<<<1, 1, 0, ctx->stream>>>(ctx->d_dmax.ptr);
         absmax_kernel<<<std::min<size_t>((nn + 255) / 256, 1024), 256, 0, ctx->stream>>>(d_density, nn, ctx->d_dmax.ptr);
