@@ -563,11 +563,13 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
         for (int e = threadIdx.x; e < ntile; e += blockDim.x) tiles[e] = 0.0;
         __syncthreads();
 
-        // lane = one primitive pair of a ket pair: the contraction over ket
-        // primitives happens in the LDS tiles (every step after the VRR is linear)
+        // lane = a run of primitive pairs of one ket pair (first | count << 16):
+        // the contraction over the pair's runs happens in the LDS tiles (every
+        // step after the VRR is linear)
         const int e = ti.j0 + lane;
         const bool lane_ok = e < ti.j1;
         const int2 ke = kentries[lane_ok ? e : ti.j0];
+        const int kq0 = ke.y & 0xffff, kqn = ke.y >> 16;
         const PairInfo kp = pairs_ket[ke.x];
         int r[nc], t[nd], rl[nc], tl[nd];
 #pragma unroll
@@ -605,7 +607,7 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
             if (a.same_class && bp.rank == kp.rank) s *= R(0.5);
             R I[nsph];
             EriClass<la, lb, lc, ld>::template quartet<R, RS>(bra_prims + bp.prim_start, bp.nprim, R(bp.abx), R(bp.aby),
-                                                          R(bp.abz), ket_prims + kp.prim_start + ke.y, active ? 1 : 0,
+                                                          R(bp.abz), ket_prims + kp.prim_start + kq0, active ? kqn : 0,
                                                           R(kp.abx), R(kp.aby), R(kp.abz), tab, R(a.omega2), I);
 #pragma unroll
             for (int x = 0; x < nsph; x++) I[x] *= s;

@@ -120,7 +120,7 @@ struct PairSet {
         int cnt_le[NBINS];  // pairs of the segment with bin <= b
     };
     std::vector<Segment> segs;  // tiled order only: runs of equal (group A, group B)
-    std::vector<int2> kentries;           // tiled ket order: (pair, primitive) entries
+    std::vector<int2> kentries;           // tiled ket order: (pair, first primitive | count << 16) entries
     std::vector<int> seg_e0, seg_e1;      // entry range of each segment
     DevBuf<int2> d_kentries;
     DevBuf<PairInfo> d_pairs;
@@ -170,6 +170,9 @@ struct vlx_context {
     int group_atoms = 12;           // atoms per group of the tiled kernels (VLX_GROUP_ATOMS)
     int tiled_waves = 0;            // 4: 4-wave blocks for every tiled class; 0: per-class default (VLX_TILED_WAVES)
     int max_tile_lds = 96 * 1024;   // larger tiles fall back to the untiled kernel
+    // primitives of one ket pair per lane of the tiled kernels (VLX_KET_PRIMS, 1..16): a ket
+    // pair of n primitives takes ceil(n / ket_prims) lanes with near-equal shares
+    int ket_prims = 3;
     hipStream_t stream = nullptr;
     hipStream_t side[NSIDE] = {};
     hipEvent_t ev_start = nullptr;
@@ -307,8 +310,9 @@ void reorder_set(const vlx_screener& s, const PairSet& src, const std::vector<in
 
 // Tiled orders of `src`, both by (group A, group B, bin, nprim): the bras a
 // ket chunk's bins allow are a prefix of each segment.  Kets are expanded to
-// (pair, primitive) entries, one per lane.  Segments are the runs of equal
-// (group A, group B).
+// lane entries of at most ket_prims consecutive primitives of one pair (the
+// contraction over the rest of the pair's primitives happens in the LDS
+// tiles).  Segments are the runs of equal (group A, group B).
 void build_tiled(const vlx_screener& s, const PairSet& src, PairSet& ket, PairSet& bra)
 {
     std::vector<int> order(src.size());
@@ -336,8 +340,18 @@ void build_tiled(const vlx_screener& s, const PairSet& src, PairSet& ket, PairSe
     reorder_set(s, src, border, bra);
     for (int k = 0; k < bra.size(); k++) bra.pairs[k].rank = rank_of[border[k]];
     ket.kentries.clear();
-    for (int k = 0; k < ket.size(); k++)
-        for (int q = 0; q < ket.pairs[k].nprim; q++) ket.kentries.push_back(make_int2(k, q));
+    const int pmax = s.ctx->ket_prims;
+    std::vector<long long> eoff(ket.size() + 1, 0);
+    for (int k = 0; k < ket.size(); k++) {
+        const int np = ket.pairs[k].nprim;
+        const int m = (np + pmax - 1) / pmax;
+        for (int t = 0, q = 0; t < m; t++) {
+            const int cnt = np / m + (t < np % m ? 1 : 0);
+            ket.kentries.push_back(make_int2(k, q | (cnt << 16)));
+            q += cnt;
+        }
+        eoff[k + 1] = eoff[k] + m;
+    }
     for (PairSet* ps : {&ket, &bra}) {
         ps->segs.clear();
         for (int k = 0; k < ps->size(); k++) {
@@ -355,8 +369,6 @@ void build_tiled(const vlx_screener& s, const PairSet& src, PairSet& ket, PairSe
         }
     }
     // entry ranges of the ket segments
-    std::vector<long long> eoff(ket.size() + 1, 0);
-    for (int k = 0; k < ket.size(); k++) eoff[k + 1] = eoff[k] + ket.pairs[k].nprim;
     ket.seg_e0.clear();
     ket.seg_e1.clear();
     for (const auto& sg : ket.segs) {
@@ -561,14 +573,19 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
         if (sch->tiled) {
             const PairSet& tb = unscreened ? s->all_tiled_bra[ce.x] : s->kept_tiled_bra[ce.x];
             const PairSet& tk = unscreened ? s->all_tiled[ce.y] : s->kept_tiled[ce.y];
-            (void)tk;
+            // a chunk's lanes loop over the largest primitive count among its entries
+            auto kmax = [&](const TileItem& ti) {
+                int m = 1;
+                for (int j = ti.j0; j < ti.j1; j++) m = std::max(m, tk.kentries[j].y >> 16);
+                return static_cast<long long>(m);
+            };
             for (const auto& ti : sch->titems) {
                 sch->lane_slots += 64LL * (ti.i1 - ti.i0);
-                sch->prim_slots += (tb.nprim_prefix[ti.i1] - tb.nprim_prefix[ti.i0]) * 64;
+                sch->prim_slots += (tb.nprim_prefix[ti.i1] - tb.nprim_prefix[ti.i0]) * 64 * kmax(ti);
             }
             for (const auto& ti : sch->titems32) {
                 sch->lane_slots += 64LL * (ti.i1 - ti.i0);
-                sch->prim_slots32 += (tb.nprim_prefix[ti.i1] - tb.nprim_prefix[ti.i0]) * 64;
+                sch->prim_slots32 += (tb.nprim_prefix[ti.i1] - tb.nprim_prefix[ti.i0]) * 64 * kmax(ti);
             }
             sch->prim_slots += sch->prim_slots32;
         } else if (ce.variant == VLX_KET_STATIONARY) {
@@ -652,6 +669,7 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_TILED_LMAX")) ctx->tiled_lmax = std::atoi(e);
     if (const char* e = std::getenv("VLX_GROUP_ATOMS")) ctx->group_atoms = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("VLX_TILED_WAVES")) ctx->tiled_waves = std::atoi(e);
+    if (const char* e = std::getenv("VLX_KET_PRIMS")) ctx->ket_prims = std::max(1, std::min(16, std::atoi(e)));
     if (const char* e = std::getenv("VLX_VARIANT")) ctx->force_variant = std::max(-1, std::min(NVARIANTS - 1, std::atoi(e)));
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));

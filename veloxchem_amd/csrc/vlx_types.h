@@ -66,7 +66,7 @@ struct FockArgs {
     const PrimPair* ket_prims;
     const WorkItem* items;
     const TileItem* titems;        // tiled variant
-    const int2* kentries;          // tiled variant: (ket pair, primitive) per lane entry
+    const int2* kentries;          // tiled variant: (ket pair, first primitive | count << 16) per lane
     const int* grp_off;            // (ngroups x 4): offset of the (group, l) AO list in grp_ao
     const int* grp_n;              // (ngroups x 4): number of AOs in the (group, l) block
     const int* grp_ao;             // global AO index of each local AO
