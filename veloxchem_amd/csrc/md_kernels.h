@@ -339,40 +339,200 @@ __device__ void digest(const PairInfo& bp, const PairInfo& kp, double s, const d
 
 }  // namespace md
 
-// Work item {j0, i0, i1}: bra i0 with kets [j0, i1) (processed one at a time).
-template <int la, int lb, int lc, int ld>
-__global__ void __launch_bounds__(VLX_WAVE)
-    fock_kernel_md(const PairInfo* __restrict__ bra, const PrimPair* __restrict__ bra_prims,
-                   const PairInfo* __restrict__ ket, const PrimPair* __restrict__ ket_prims,
-                   const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
-                   const double* __restrict__ D, double* __restrict__ F, const FockScalars a)
+// ------------------------------------------------------------------------
+// Matrix-core McMurchie-Davidson Fock kernel.
+//
+// The Hermite-to-spherical matrices of every primitive pair are built once
+// per screener (u_build_kernel) and stored in HBM as
+//   U[prim][h][s] = k sum_{cart a, b} S_a S_b E^x E^y E^z
+// (h < nherm(la + lb) Hermite index, s < (2la+1)(2lb+1) spherical index, both
+// padded to multiples of 16 with zeros).  One wavefront takes one shell
+// quartet at a time; per primitive quartet it builds R^0[tuv] (scaled by
+// 1/sqrt(zeta + eta)) in LDS and runs two FP64 matrix-core products
+//   X[p][cd]    = sum_q (-1)^{|q|} R^0[p + q] U_CD[q][cd]       (v_mfma_f64_16x16x4)
+//   out[ab][cd] += sum_p U_AB[p][ab] X[p][cd]
+// one 16-wide cd tile at a time; out stays in the matrix-core accumulators
+// over all primitive quartets of the shell quartet.  Operand layouts of
+// v_mfma_f64_16x16x4f64 (wave64): A[i][k] in lane i + 16k, B[k][j] in lane
+// j + 16k, D[i][j] in register i / 4 of lane j + 16 (i % 4).
+
+using md::hidx;
+using md::nherm;
+
+struct HermDecode {
+    int e[nherm(VLX_MAX_LTOT)];  // t | u << 8 | v << 16 of Hermite index idx
+};
+constexpr HermDecode make_herm_decode()
 {
-    using S = md::Sizes<la, lb, lc, ld>;
-    static_assert((S::lds + S::nsab * S::nscd + (S::tab_in_lds ? BOYS_TAB_SIZE : 0)) * 8 <= 160 * 1024,
-                  "LDS budget");
-    __shared__ double tab_lds[S::tab_in_lds ? BOYS_TAB_SIZE : 1];
-    __shared__ double work[S::lds];
-    __shared__ double out[S::nsab * S::nscd];
-    const double* tab = S::tab_in_lds ? tab_lds : boys_tab;
-    if constexpr (S::tab_in_lds) stage_boys(tab_lds, boys_tab);
+    HermDecode d{};
+    int i = 0;
+    for (int n = 0; n <= VLX_MAX_LTOT; n++)
+        for (int s = 0; s <= n; s++)
+            for (int v = 0; v <= s; v++) d.e[i++] = (n - s) | ((s - v) << 8) | (v << 16);
+    return d;
+}
+static __device__ constexpr HermDecode g_herm = make_herm_decode();
+
+__device__ __forceinline__ int hidx_packed(int tuv)
+{
+    return hidx(tuv & 0xff, (tuv >> 8) & 0xff, tuv >> 16);
+}
+
+typedef double vlx_d4 __attribute__((ext_vector_type(4)));
+
+template <int la, int lb, int lc, int ld>
+struct MmdSizes {
+    static constexpr int L = la + lb + lc + ld;
+    static constexpr int nhab = nherm(la + lb), nhcd = nherm(lc + ld), nr = nherm(L);
+    static constexpr int nsab = (2 * la + 1) * (2 * lb + 1), nscd = (2 * lc + 1) * (2 * ld + 1);
+    static constexpr int HA = vlx_u_rows(la, lb), SA = vlx_u_cols(la, lb);  // bra U: HA x SA
+    static constexpr int HC = vlx_u_rows(lc, ld), SC = vlx_u_cols(lc, ld);  // ket U: HC x SC
+    static constexpr int MT = SA / 16, NT = SC / 16, MX = HA / 16;
+    static constexpr int KX = (nhcd + 3) / 4, KO = (nhab + 3) / 4;
+};
+
+// Work item {j0, i0, i1}: bra i0 with kets [j0, i1), one shell quartet at a time.
+template <int la, int lb, int lc, int ld, bool RS>
+__global__ void __launch_bounds__(VLX_WAVE)
+    fock_kernel_mmd(const PairInfo* __restrict__ bra, const PrimPair* __restrict__ bra_prims,
+                    const double* __restrict__ bra_u, const PairInfo* __restrict__ ket,
+                    const PrimPair* __restrict__ ket_prims, const double* __restrict__ ket_u,
+                    const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
+                    const double* __restrict__ D, double* __restrict__ F, const FockScalars a)
+{
+    using S = MmdSizes<la, lb, lc, ld>;
+    constexpr int L = S::L;
+    __shared__ double rbuf[2 * S::nr];
+    __shared__ double xt[S::HA * 16];
+    __shared__ double outs[S::nsab * S::nscd];
+    const int lane = threadIdx.x;
+    const int lr = lane & 15, lk = lane >> 4;
+    // per-lane gather offsets of the X product: row p = 16 mx + lr, column q = 4 kq + lk
     for (long long it = blockIdx.x; it < a.nitems; it += gridDim.x) {
         const WorkItem wi = items[it];
         const PairInfo bp = bra[wi.i0];
-        const double abv[3] = {bp.abx, bp.aby, bp.abz};
         for (int j = wi.j0; j < wi.i1; j++) {
             const PairInfo kp = ket[j];
-            const double cdv[3] = {kp.abx, kp.aby, kp.abz};
-            md::quartet<la, lb, lc, ld>(bra_prims + bp.prim_start, bp.nprim, abv, ket_prims + kp.prim_start,
-                                        kp.nprim, cdv, tab, a.omega2, work, out);
-            double s = 1.0;
-            if (bp.same) s *= 0.5;
-            if (kp.same) s *= 0.5;
-            if (a.same_class && bp.rank == j) s *= 0.5;
-            md::digest<la, lb, lc, ld>(bp, kp, s, out, D, F, a);
+            vlx_d4 acc[S::MT][S::NT];
+#pragma unroll
+            for (int m = 0; m < S::MT; m++)
+#pragma unroll
+                for (int t = 0; t < S::NT; t++) acc[m][t] = vlx_d4{0.0, 0.0, 0.0, 0.0};
+            for (int ib = 0; ib < bp.nprim; ib++) {
+                const PrimPair pb = bra_prims[bp.prim_start + ib];
+                const double* ua = bra_u + (size_t)(bp.prim_start + ib) * (S::HA * S::SA);
+                for (int ik = 0; ik < kp.nprim; ik++) {
+                    const PrimPair pk = ket_prims[kp.prim_start + ik];
+                    const double* uc = ket_u + (size_t)(kp.prim_start + ik) * (S::HC * S::SC);
+                    const double sz = pb.zeta + pk.zeta;
+                    const double rho_c = pb.zeta * pk.zeta / sz;
+                    double rho = rho_c, w = vrsqrt<double>(sz);
+                    if constexpr (RS) {
+                        const double sfac = a.omega2 / (a.omega2 + rho_c);
+                        rho = rho_c * sfac;
+                        w *= sqrt(sfac);
+                    }
+                    const double pqx = pb.px - pk.px, pqy = pb.py - pk.py, pqz = pb.pz - pk.pz;
+                    double Fm[L + 1];
+                    boys_eval<L, double>(rho * (pqx * pqx + pqy * pqy + pqz * pqz), boys_tab, Fm);
+                    // R levels n = L .. 0, ping-pong in rbuf; R^n_000 = w (-2 rho)^n F_n
+                    double seed[L + 1];
+                    {
+                        double f = w;
+#pragma unroll
+                        for (int n = 0; n <= L; n++) {
+                            seed[n] = f * Fm[n];
+                            f *= -2.0 * rho;
+                        }
+                    }
+                    double* cur = rbuf + ((L & 1) ? S::nr : 0);
+                    if (lane == 0) cur[0] = seed[L];
+#pragma unroll
+                    for (int n = L - 1; n >= 0; n--) {
+                        double* prev = cur;
+                        cur = (cur == rbuf) ? rbuf + S::nr : rbuf;
+                        __syncthreads();
+                        for (int idx = lane; idx < nherm(L - n); idx += VLX_WAVE) {
+                            const int tuv = g_herm.e[idx];
+                            const int t = tuv & 0xff, u = (tuv >> 8) & 0xff, v = tuv >> 16;
+                            double val;
+                            if (t > 0) {
+                                val = pqx * prev[hidx(t - 1, u, v)];
+                                if (t > 1) val = fma(double(t - 1), prev[hidx(t - 2, u, v)], val);
+                            } else if (u > 0) {
+                                val = pqy * prev[hidx(t, u - 1, v)];
+                                if (u > 1) val = fma(double(u - 1), prev[hidx(t, u - 2, v)], val);
+                            } else if (v > 0) {
+                                val = pqz * prev[hidx(t, u, v - 1)];
+                                if (v > 1) val = fma(double(v - 1), prev[hidx(t, u, v - 2)], val);
+                            } else {
+                                val = seed[n];
+                            }
+                            cur[idx] = val;
+                        }
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int nt = 0; nt < S::NT; nt++) {
+                        // X tile (rows p, 16 columns cd of tile nt) = sum_q (-1)^|q| R^0[p + q] U_CD[q][cd]
+#pragma unroll
+                        for (int mx = 0; mx < S::MX; mx++) {
+                            vlx_d4 x = {0.0, 0.0, 0.0, 0.0};
+                            const int p = 16 * mx + lr;
+                            const int tp = p < S::nhab ? g_herm.e[p] : 0;
+#pragma unroll 4
+                            for (int kq = 0; kq < S::KX; kq++) {
+                                const int q = 4 * kq + lk;
+                                double av = 0.0;
+                                if (p < S::nhab && q < S::nhcd) {
+                                    const int tq = g_herm.e[q];
+                                    const double r0 = cur[hidx_packed(tp + tq)];
+                                    const int deg = (tq & 0xff) + ((tq >> 8) & 0xff) + (tq >> 16);
+                                    av = (deg & 1) ? -r0 : r0;
+                                }
+                                const double bv = uc[q * S::SC + 16 * nt + lr];
+                                x = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, x, 0, 0, 0);
+                            }
+#pragma unroll
+                            for (int r = 0; r < 4; r++) xt[(16 * mx + lk + 4 * r) * 16 + lr] = x[r];
+                        }
+                        __syncthreads();
+                        // out[ab][cd] += sum_p U_AB[p][ab] X[p][cd]
+#pragma unroll
+                        for (int ms = 0; ms < S::MT; ms++) {
+#pragma unroll 4
+                            for (int kp2 = 0; kp2 < S::KO; kp2++) {
+                                const int pr = 4 * kp2 + lk;
+                                const double av = ua[pr * S::SA + 16 * ms + lr];
+                                const double bv = xt[pr * 16 + lr];
+                                acc[ms][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[ms][nt], 0, 0, 0);
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
+            }
+            // contracted block -> LDS [ab][cd], then digestion
+#pragma unroll
+            for (int ms = 0; ms < S::MT; ms++)
+#pragma unroll
+                for (int nt = 0; nt < S::NT; nt++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int ab = 16 * ms + lk + 4 * r, cd = 16 * nt + lr;
+                        if (ab < S::nsab && cd < S::nscd) outs[ab * S::nscd + cd] = acc[ms][nt][r];
+                    }
+            __syncthreads();
+            double sf = 1.0;
+            if (bp.same) sf *= 0.5;
+            if (kp.same) sf *= 0.5;
+            if (a.same_class && bp.rank == j) sf *= 0.5;
+            md::digest<la, lb, lc, ld>(bp, kp, sf, outs, D, F, a);
             __syncthreads();
         }
     }
 }
+
 
 template <int la, int lb, int lc, int ld>
 __global__ void __launch_bounds__(VLX_WAVE) eri_out_kernel_md(const EriArgs a)
@@ -421,13 +581,19 @@ __global__ void __launch_bounds__(VLX_WAVE) diag_kernel_md(const DiagArgs a)
     }
 }
 
+// Fock: the matrix-core kernel (U arrays of both pair sets in FockArgs); the
+// cooperative LDS kernel serves explicit quartets and the screener.
+#define VLX_MMD_LAUNCH(la, lb, lc, ld, RS)                                                              \
+    hipLaunchKernelGGL((fock_kernel_mmd<la, lb, lc, ld, RS>), dim3(g), dim3(VLX_WAVE), 0, s, a.bra,     \
+                       a.bra_prims, a.bra_u, a.ket, a.ket_prims, a.ket_u, a.items, a.boys_tab, a.D, a.F, \
+                       a.sc)
 #define VLX_INSTANTIATE_MD_CLASS(la, lb, lc, ld)                                                        \
     void launch_fock_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                       \
     {                                                                                                   \
         (void)grid;                                                                                     \
-        const int g = static_cast<int>(std::min<long long>(a.sc.nitems, 4096));                         \
-        hipLaunchKernelGGL((fock_kernel_md<la, lb, lc, ld>), dim3(g), dim3(VLX_WAVE), 0, s, a.bra,      \
-                           a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F, a.sc);       \
+        const int g = static_cast<int>(std::min<long long>(a.sc.nitems, 16384));                        \
+        if (a.sc.omega2 > 0.0) VLX_MMD_LAUNCH(la, lb, lc, ld, true);                                    \
+        else VLX_MMD_LAUNCH(la, lb, lc, ld, false);                                                     \
     }                                                                                                   \
     void launch_eri_##la##lb##lc##ld(const EriArgs& a, hipStream_t s)                                   \
     {                                                                                                   \

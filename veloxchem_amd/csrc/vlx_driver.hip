@@ -125,6 +125,8 @@ struct PairSet {
     DevBuf<int2> d_kentries;
     DevBuf<PairInfo> d_pairs;
     DevBuf<PrimPair> d_prims;
+    DevBuf<double> d_u;   // matrix-core MD kernel: U of every primitive pair (built on first use)
+    bool u_ready = false;
     int size() const { return static_cast<int>(pairs.size()); }
 };
 
@@ -377,6 +379,19 @@ void build_tiled(const vlx_screener& s, const PairSet& src, PairSet& ket, PairSe
     }
 }
 
+// U arrays of a pair set for the matrix-core MD Fock kernel (once per set).
+int ensure_u(vlx_context* ctx, PairSet& ps)
+{
+    if (ps.u_ready || ps.size() == 0) return VLX_OK;
+    const size_t per = (size_t)vlx_u_rows(ps.la, ps.lb) * vlx_u_cols(ps.la, ps.lb);
+    HIP_TRY(ps.d_u.alloc(per * ps.prims.size()));
+    vlx_launch_u_build(ps.d_pairs.ptr, ps.d_prims.ptr, ps.size(), ps.la, ps.lb, ps.d_u.ptr, ctx->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ps.u_ready = true;
+    return VLX_OK;
+}
+
 int upload_set(PairSet& ps)
 {
     HIP_TRY(ps.d_pairs.upload(ps.pairs));
@@ -606,6 +621,11 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
             }
         }
         if (sch->items.empty() && sch->titems.empty() && sch->titems32.empty()) continue;
+        if (!sch->tiled && ce.variant == VLX_WAVE_MD) {
+            int rc = ensure_u(s->ctx, unscreened ? s->all_bra[ce.x] : s->kept_bra[ce.x]);
+            if (!rc) rc = ensure_u(s->ctx, unscreened ? s->all[ce.y] : s->kept[ce.y]);
+            if (rc) return rc;
+        }
         if (sch->tiled) {
             if (!sch->titems.empty()) HIP_TRY(sch->d_titems.upload(sch->titems));
             if (!sch->titems32.empty()) HIP_TRY(sch->d_titems32.upload(sch->titems32));
@@ -1154,13 +1174,13 @@ int vlx_fock_class_kernel(vlx_context* ctx, vlx_screener* scr, int ithreshold, i
         if (sp->tiled) {
             const int L = la + lb + lc + ld;
             const int wide = (L <= 2) ? 16 : (L == 3 ? 8 : 4);
-            std::snprintf(name, len, "fock_kernel_tiled<%d,%d,%d,%d,double,%d>", la, lb, lc, ld,
+            std::snprintf(name, len, "fock_kernel_tiled<%d,%d,%d,%d,double,%d,false>", la, lb, lc, ld,
                           sp->tiled_waves == 4 ? 4 : wide);
         } else {
             const char* k = (ce.variant == VLX_KET_STATIONARY) ? "fock_kernel"
-                            : (ce.variant == VLX_WAVE_MD)       ? "fock_kernel_md"
+                            : (ce.variant == VLX_WAVE_MD)       ? "fock_kernel_mmd"
                                                                 : "fock_kernel_bs";
-            std::snprintf(name, len, "%s<%d,%d,%d,%d>", k, la, lb, lc, ld);
+            std::snprintf(name, len, "%s<%d,%d,%d,%d,false>", k, la, lb, lc, ld);
         }
         return VLX_OK;
     }
@@ -1256,6 +1276,8 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
         a.bra_prims = bra.d_prims.ptr;
         a.ket = ket.d_pairs.ptr;
         a.ket_prims = ket.d_prims.ptr;
+        a.bra_u = bra.d_u.ptr;
+        a.ket_u = ket.d_u.ptr;
         a.items = sch.d_items.ptr;
         a.boys_tab = ctx->d_boys.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;
         a.boys_tab_f = ctx->d_boys_f.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;

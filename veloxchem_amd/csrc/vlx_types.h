@@ -74,10 +74,24 @@ struct FockArgs {
     int tiled_waves;               // 4: force 4-wave blocks for the tiled kernel; else the class default
     const double* boys_tab;        // BOYS_TAB_SIZE doubles for L = la+lb+lc+ld
     const float* boys_tab_f;       // the same table in FP32 (mixed-precision tiled kernel)
+    const double* bra_u;           // matrix-core MD kernel: Hermite-to-spherical U of every bra primitive pair
+    const double* ket_u;           // ... and of every ket primitive pair
     const double* D;               // nao x nao, row-major
     double* F;                     // nao x nao accumulator F' (G = F' + F'^T)
     FockScalars sc;
 };
+
+// Hermite functions of total degree <= l, and the padded U array of a pair
+// class (la, lb) of the matrix-core MD kernel: rows = Hermite index, columns
+// = spherical index of the pair, both padded to multiples of 16.
+__host__ __device__ constexpr int vlx_nherm(int l) { return (l + 1) * (l + 2) * (l + 3) / 6; }
+__host__ __device__ constexpr int vlx_pad16(int x) { return (x + 15) / 16 * 16; }
+__host__ __device__ constexpr int vlx_u_rows(int la, int lb) { return vlx_pad16(vlx_nherm(la + lb)); }
+__host__ __device__ constexpr int vlx_u_cols(int la, int lb) { return vlx_pad16((2 * la + 1) * (2 * lb + 1)); }
+
+// U arrays ([prim][vlx_u_rows][vlx_u_cols]) of the npairs pairs of one pair set.
+void vlx_launch_u_build(const PairInfo* pairs, const PrimPair* prims, int npairs, int la, int lb, double* u,
+                        hipStream_t s);
 
 struct EriArgs {
     const PairInfo* bra;
