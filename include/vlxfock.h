@@ -142,7 +142,8 @@ int vlx_fock_schedule_stats(vlx_screener* scr, int ithreshold, int max_records, 
  * Autotuner (no reference counterpart): times every angular class of one
  * 2jk build at ithreshold on `d_density` under each kernel variant
  * (0 = untiled, 1 = LDS-tiled with 4-wave blocks, 2 = LDS-tiled with the
- * class's wide block), `repeats` builds each on a single stream, and keeps
+ * class's wide block, 3 = matrix-core McMurchie-Davidson, for classes with
+ * la+lb+lc+ld >= 4), `repeats` builds each on a single stream, and keeps
  * the fastest variant per class for later builds with this screener (and as
  * the default of screeners created later on this context).
  * Optional outputs: *nchosen records of classes (4 ints each) and variants

@@ -306,6 +306,15 @@ def tiled_waves(la, lb, lc, ld):
     return 16 if L <= 2 else (8 if L == 3 else 4)
 
 
+# classes with la+lb+lc+ld >= MMD_LMIN also get the matrix-core MD Fock kernel as
+# an alternative variant (the autotuner picks it per class)
+MMD_LMIN = 4
+
+
+def mmd_eligible(la, lb, lc, ld, var):
+    return var == 'md' or la + lb + lc + ld >= MMD_LMIN
+
+
 def tiled_eligible(la, lb, lc, ld):
     """Classes that also get the LDS-tiled Fock kernel.  L <= 4 use it by
     default; for L = 5 it is built and chosen at run time (VLX_TILED_LMAX)."""
@@ -340,8 +349,11 @@ def main(outdir, lmax):
                 src.append(f'VLX_INSTANTIATE_MD_DIAG({la}, {lb})')
         else:
             kern = 'fock_kernel' if var == 'ks' else 'fock_kernel_bs'
-            src = ['// generated by gen_eri.py -- do not edit', '#include "../eri_kernels.h"', '', body, '',
+            hdr = '../md_kernels.h' if mmd_eligible(la, lb, lc, ld, var) else '../eri_kernels.h'
+            src = ['// generated by gen_eri.py -- do not edit', f'#include "{hdr}"', '', body, '',
                    f'VLX_INSTANTIATE_CLASS({la}, {lb}, {lc}, {ld}, {kern})']
+            if mmd_eligible(la, lb, lc, ld, var):
+                src.append(f'VLX_INSTANTIATE_MMD({la}, {lb}, {lc}, {ld})')
             if (la, lb) == (lc, ld):
                 src.append(f'VLX_INSTANTIATE_DIAG({la}, {lb})')
             if tiled_eligible(la, lb, lc, ld):
@@ -357,6 +369,8 @@ def main(outdir, lmax):
         d.append(f'VLX_DECLARE_CLASS({la}, {lb}, {lc}, {ld})')
         if tiled_eligible(la, lb, lc, ld) and variants[(la, lb, lc, ld)] != 'md':
             d.append(f'VLX_DECLARE_TILED({la}, {lb}, {lc}, {ld})')
+        if mmd_eligible(la, lb, lc, ld, variants[(la, lb, lc, ld)]) and variants[(la, lb, lc, ld)] != 'md':
+            d.append(f'VLX_DECLARE_MMD({la}, {lb}, {lc}, {ld})')
         if (la, lb) == (lc, ld):
             d.append(f'VLX_DECLARE_DIAG({la}, {lb})')
     d.append('static const ClassLaunchers g_class_table[] = {')
@@ -368,8 +382,11 @@ def main(outdir, lmax):
         has_tiled = tiled_eligible(la, lb, lc, ld) and variants[(la, lb, lc, ld)] != 'md'
         tl = f'&launch_tiled_{la}{lb}{lc}{ld}' if has_tiled else 'nullptr'
         tl32 = f'&launch_tiled32_{la}{lb}{lc}{ld}' if has_tiled else 'nullptr'
+        var = variants[(la, lb, lc, ld)]
+        mm = (f'&launch_fock_{la}{lb}{lc}{ld}' if var == 'md' else
+              f'&launch_mmd_{la}{lb}{lc}{ld}' if mmd_eligible(la, lb, lc, ld, var) else 'nullptr')
         d.append(f'    {{{x}, {y}, {la}, {lb}, {lc}, {ld}, {nv}, {nh}, {ks}, &launch_fock_{la}{lb}{lc}{ld}, '
-                 f'&launch_eri_{la}{lb}{lc}{ld}, {tl}, {tl32}}},')
+                 f'&launch_eri_{la}{lb}{lc}{ld}, {tl}, {tl32}, {mm}}},')
     d.append('};')
     d.append('static const DiagLauncher g_diag_table[] = {')
     for (la, lb) in pcs:

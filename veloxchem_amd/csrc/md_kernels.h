@@ -389,11 +389,12 @@ struct MmdSizes {
     static constexpr int HC = vlx_u_rows(lc, ld), SC = vlx_u_cols(lc, ld);  // ket U: HC x SC
     static constexpr int MT = SA / 16, NT = SC / 16, MX = HA / 16;
     static constexpr int KX = (nhcd + 3) / 4, KO = (nhab + 3) / 4;
+    static constexpr int XC = MX < 2 ? MX : 2;  // X row tiles per chunk
 };
 
 // Work item {j0, i0, i1}: bra i0 with kets [j0, i1), one shell quartet at a time.
 template <int la, int lb, int lc, int ld, bool RS>
-__global__ void __launch_bounds__(VLX_WAVE)
+__global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4)))
     fock_kernel_mmd(const PairInfo* __restrict__ bra, const PrimPair* __restrict__ bra_prims,
                     const double* __restrict__ bra_u, const PairInfo* __restrict__ ket,
                     const PrimPair* __restrict__ ket_prims, const double* __restrict__ ket_u,
@@ -403,125 +404,138 @@ __global__ void __launch_bounds__(VLX_WAVE)
     using S = MmdSizes<la, lb, lc, ld>;
     constexpr int L = S::L;
     __shared__ double rbuf[2 * S::nr];
-    __shared__ double xt[S::HA * 16];
+    __shared__ double seeds[L + 1];
+    __shared__ double xt[S::XC * 16 * 16];
     __shared__ double outs[S::nsab * S::nscd];
     const int lane = threadIdx.x;
     const int lr = lane & 15, lk = lane >> 4;
-    // per-lane gather offsets of the X product: row p = 16 mx + lr, column q = 4 kq + lk
     for (long long it = blockIdx.x; it < a.nitems; it += gridDim.x) {
         const WorkItem wi = items[it];
         const PairInfo bp = bra[wi.i0];
         for (int j = wi.j0; j < wi.i1; j++) {
             const PairInfo kp = ket[j];
-            vlx_d4 acc[S::MT][S::NT];
+            // one 16-wide cd tile at a time: its accumulators persist over all primitive quartets
+            for (int nt = 0; nt < S::NT; nt++) {
+                vlx_d4 acc[S::MT];
 #pragma unroll
-            for (int m = 0; m < S::MT; m++)
-#pragma unroll
-                for (int t = 0; t < S::NT; t++) acc[m][t] = vlx_d4{0.0, 0.0, 0.0, 0.0};
-            for (int ib = 0; ib < bp.nprim; ib++) {
-                const PrimPair pb = bra_prims[bp.prim_start + ib];
-                const double* ua = bra_u + (size_t)(bp.prim_start + ib) * (S::HA * S::SA);
-                for (int ik = 0; ik < kp.nprim; ik++) {
-                    const PrimPair pk = ket_prims[kp.prim_start + ik];
-                    const double* uc = ket_u + (size_t)(kp.prim_start + ik) * (S::HC * S::SC);
-                    const double sz = pb.zeta + pk.zeta;
-                    const double rho_c = pb.zeta * pk.zeta / sz;
-                    double rho = rho_c, w = vrsqrt<double>(sz);
-                    if constexpr (RS) {
-                        const double sfac = a.omega2 / (a.omega2 + rho_c);
-                        rho = rho_c * sfac;
-                        w *= sqrt(sfac);
-                    }
-                    const double pqx = pb.px - pk.px, pqy = pb.py - pk.py, pqz = pb.pz - pk.pz;
-                    double Fm[L + 1];
-                    boys_eval<L, double>(rho * (pqx * pqx + pqy * pqy + pqz * pqz), boys_tab, Fm);
-                    // R levels n = L .. 0, ping-pong in rbuf; R^n_000 = w (-2 rho)^n F_n
-                    double seed[L + 1];
-                    {
-                        double f = w;
-#pragma unroll
-                        for (int n = 0; n <= L; n++) {
-                            seed[n] = f * Fm[n];
-                            f *= -2.0 * rho;
+                for (int m = 0; m < S::MT; m++) acc[m] = vlx_d4{0.0, 0.0, 0.0, 0.0};
+                for (int ib = 0; ib < bp.nprim; ib++) {
+                    const PrimPair pb = bra_prims[bp.prim_start + ib];
+                    const double* ua = bra_u + (size_t)(bp.prim_start + ib) * (S::HA * S::SA);
+                    for (int ik = 0; ik < kp.nprim; ik++) {
+                        const PrimPair pk = ket_prims[kp.prim_start + ik];
+                        const double* uc = ket_u + (size_t)(kp.prim_start + ik) * (S::HC * S::SC) + 16 * nt + lr;
+                        const double sz = pb.zeta + pk.zeta;
+                        const double rho_c = pb.zeta * pk.zeta / sz;
+                        double rho = rho_c, w = vrsqrt<double>(sz);
+                        if constexpr (RS) {
+                            const double sfac = a.omega2 / (a.omega2 + rho_c);
+                            rho = rho_c * sfac;
+                            w *= sqrt(sfac);
                         }
-                    }
-                    double* cur = rbuf + ((L & 1) ? S::nr : 0);
-                    if (lane == 0) cur[0] = seed[L];
+                        const double pqx = pb.px - pk.px, pqy = pb.py - pk.py, pqz = pb.pz - pk.pz;
+                        double Fm[L + 1];
+                        boys_eval<L, double>(rho * (pqx * pqx + pqy * pqy + pqz * pqz), boys_tab, Fm);
+                        // R levels n = L .. 0, ping-pong in rbuf; R^n_000 = w (-2 rho)^n F_n, lane n
+                        // keeps level n's seed in LDS (the level loop then needs no register array)
+                        {
+                            double f = w, mine = 0.0;
 #pragma unroll
-                    for (int n = L - 1; n >= 0; n--) {
-                        double* prev = cur;
-                        cur = (cur == rbuf) ? rbuf + S::nr : rbuf;
-                        __syncthreads();
-                        for (int idx = lane; idx < nherm(L - n); idx += VLX_WAVE) {
-                            const int tuv = g_herm.e[idx];
-                            const int t = tuv & 0xff, u = (tuv >> 8) & 0xff, v = tuv >> 16;
-                            double val;
-                            if (t > 0) {
-                                val = pqx * prev[hidx(t - 1, u, v)];
-                                if (t > 1) val = fma(double(t - 1), prev[hidx(t - 2, u, v)], val);
-                            } else if (u > 0) {
-                                val = pqy * prev[hidx(t, u - 1, v)];
-                                if (u > 1) val = fma(double(u - 1), prev[hidx(t, u - 2, v)], val);
-                            } else if (v > 0) {
-                                val = pqz * prev[hidx(t, u, v - 1)];
-                                if (v > 1) val = fma(double(v - 1), prev[hidx(t, u, v - 2)], val);
-                            } else {
-                                val = seed[n];
+                            for (int n = 0; n <= L; n++) {
+                                if (lane == n) mine = f * Fm[n];
+                                f *= -2.0 * rho;
                             }
-                            cur[idx] = val;
+                            if (lane <= L) seeds[lane] = mine;
+                            if (lane == L) rbuf[(L & 1) ? S::nr : 0] = mine;
                         }
-                    }
-                    __syncthreads();
+                        double* cur = rbuf + ((L & 1) ? S::nr : 0);
+#pragma unroll 1
+                        for (int n = L - 1; n >= 0; n--) {
+                            double* prev = cur;
+                            cur = (cur == rbuf) ? rbuf + S::nr : rbuf;
+                            __syncthreads();
+                            const int nn = nherm(L - n);
+                            for (int idx = lane; idx < nn; idx += VLX_WAVE) {
+                                const int tuv = g_herm.e[idx];
+                                const int t = tuv & 0xff, u = (tuv >> 8) & 0xff, v = tuv >> 16;
+                                double val;
+                                if (t > 0) {
+                                    val = pqx * prev[hidx(t - 1, u, v)];
+                                    if (t > 1) val = fma(double(t - 1), prev[hidx(t - 2, u, v)], val);
+                                } else if (u > 0) {
+                                    val = pqy * prev[hidx(t, u - 1, v)];
+                                    if (u > 1) val = fma(double(u - 1), prev[hidx(t, u - 2, v)], val);
+                                } else if (v > 0) {
+                                    val = pqz * prev[hidx(t, u, v - 1)];
+                                    if (v > 1) val = fma(double(v - 1), prev[hidx(t, u, v - 2)], val);
+                                } else {
+                                    val = seeds[n];
+                                }
+                                cur[idx] = val;
+                            }
+                        }
+                        __syncthreads();
+                        // X = sum_q (-1)^|q| R^0[p + q] U_CD[q][cd] for the 16 columns cd of tile nt, in
+                        // chunks of XC row tiles; each chunk feeds its K range of out += U_AB^T X at once
+#pragma unroll 1
+                        for (int pc = 0; pc < S::MX; pc += S::XC) {
+                            vlx_d4 x[S::XC];
+                            int prow[S::XC];
+                            bool prow_ok[S::XC];
 #pragma unroll
-                    for (int nt = 0; nt < S::NT; nt++) {
-                        // X tile (rows p, 16 columns cd of tile nt) = sum_q (-1)^|q| R^0[p + q] U_CD[q][cd]
-#pragma unroll
-                        for (int mx = 0; mx < S::MX; mx++) {
-                            vlx_d4 x = {0.0, 0.0, 0.0, 0.0};
-                            const int p = 16 * mx + lr;
-                            const int tp = p < S::nhab ? g_herm.e[p] : 0;
-#pragma unroll 4
+                            for (int i = 0; i < S::XC; i++) {
+                                x[i] = vlx_d4{0.0, 0.0, 0.0, 0.0};
+                                const int p = 16 * (pc + i) + lr;
+                                prow_ok[i] = p < S::nhab;
+                                prow[i] = prow_ok[i] ? g_herm.e[p] : 0;
+                            }
+#pragma unroll 1
                             for (int kq = 0; kq < S::KX; kq++) {
                                 const int q = 4 * kq + lk;
-                                double av = 0.0;
-                                if (p < S::nhab && q < S::nhcd) {
-                                    const int tq = g_herm.e[q];
-                                    const double r0 = cur[hidx_packed(tp + tq)];
-                                    const int deg = (tq & 0xff) + ((tq >> 8) & 0xff) + (tq >> 16);
-                                    av = (deg & 1) ? -r0 : r0;
+                                const bool qok = q < S::nhcd;
+                                const int tq = qok ? g_herm.e[q] : 0;
+                                const bool neg = ((tq & 0xff) + ((tq >> 8) & 0xff) + (tq >> 16)) & 1;
+                                const double bv = uc[q * S::SC];
+#pragma unroll
+                                for (int i = 0; i < S::XC; i++) {
+                                    double av = 0.0;
+                                    if (qok && prow_ok[i]) {
+                                        const double r0 = cur[hidx_packed(prow[i] + tq)];
+                                        av = neg ? -r0 : r0;
+                                    }
+                                    x[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, x[i], 0, 0, 0);
                                 }
-                                const double bv = uc[q * S::SC + 16 * nt + lr];
-                                x = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, x, 0, 0, 0);
                             }
 #pragma unroll
-                            for (int r = 0; r < 4; r++) xt[(16 * mx + lk + 4 * r) * 16 + lr] = x[r];
-                        }
-                        __syncthreads();
-                        // out[ab][cd] += sum_p U_AB[p][ab] X[p][cd]
+                            for (int i = 0; i < S::XC; i++)
 #pragma unroll
-                        for (int ms = 0; ms < S::MT; ms++) {
-#pragma unroll 4
-                            for (int kp2 = 0; kp2 < S::KO; kp2++) {
+                                for (int r = 0; r < 4; r++) xt[(16 * i + lk + 4 * r) * 16 + lr] = x[i][r];
+                            __syncthreads();
+                            // out[ab][cd] += sum_p U_AB[p][ab] X[p][cd] over this chunk's rows p
+                            const int k0 = 4 * pc, k1 = std::min(S::KO, 4 * (pc + S::XC));
+#pragma unroll 1
+                            for (int kp2 = k0; kp2 < k1; kp2++) {
                                 const int pr = 4 * kp2 + lk;
-                                const double av = ua[pr * S::SA + 16 * ms + lr];
-                                const double bv = xt[pr * 16 + lr];
-                                acc[ms][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[ms][nt], 0, 0, 0);
+                                const double bv = xt[(pr - 16 * pc) * 16 + lr];
+#pragma unroll
+                                for (int ms = 0; ms < S::MT; ms++) {
+                                    const double av = ua[pr * S::SA + 16 * ms + lr];
+                                    acc[ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[ms], 0, 0, 0);
+                                }
                             }
+                            __syncthreads();
                         }
-                        __syncthreads();
                     }
                 }
-            }
-            // contracted block -> LDS [ab][cd], then digestion
+                // contracted tile -> LDS [ab][cd]
 #pragma unroll
-            for (int ms = 0; ms < S::MT; ms++)
-#pragma unroll
-                for (int nt = 0; nt < S::NT; nt++)
+                for (int ms = 0; ms < S::MT; ms++)
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         const int ab = 16 * ms + lk + 4 * r, cd = 16 * nt + lr;
-                        if (ab < S::nsab && cd < S::nscd) outs[ab * S::nscd + cd] = acc[ms][nt][r];
+                        if (ab < S::nsab && cd < S::nscd) outs[ab * S::nscd + cd] = acc[ms][r];
                     }
+            }
             __syncthreads();
             double sf = 1.0;
             if (bp.same) sf *= 0.5;
@@ -587,6 +601,16 @@ __global__ void __launch_bounds__(VLX_WAVE) diag_kernel_md(const DiagArgs a)
     hipLaunchKernelGGL((fock_kernel_mmd<la, lb, lc, ld, RS>), dim3(g), dim3(VLX_WAVE), 0, s, a.bra,     \
                        a.bra_prims, a.bra_u, a.ket, a.ket_prims, a.ket_u, a.items, a.boys_tab, a.D, a.F, \
                        a.sc)
+// launch_mmd_: the matrix-core kernel as an alternative variant of a class
+// whose default kernel is generated (same MD work items).
+#define VLX_INSTANTIATE_MMD(la, lb, lc, ld)                                                             \
+    void launch_mmd_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                        \
+    {                                                                                                   \
+        (void)grid;                                                                                     \
+        const int g = static_cast<int>(std::min<long long>(a.sc.nitems, 16384));                        \
+        if (a.sc.omega2 > 0.0) VLX_MMD_LAUNCH(la, lb, lc, ld, true);                                    \
+        else VLX_MMD_LAUNCH(la, lb, lc, ld, false);                                                     \
+    }
 #define VLX_INSTANTIATE_MD_CLASS(la, lb, lc, ld)                                                        \
     void launch_fock_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                       \
     {                                                                                                   \

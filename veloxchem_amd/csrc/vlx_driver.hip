@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -134,7 +135,9 @@ struct ClassSchedule {
     int entry = -1;  // index into g_class_table
     long long lane_slots = 0, prim_slots = 0;  // schedule efficiency diagnostics
     int stream = 0;  // side stream the class runs on
+    int variant = 0;      // VARIANT_* the schedule runs
     bool tiled = false;
+    bool mmd = false;     // matrix-core MD kernel (MD work items)
     int tiled_waves = 0;  // 4: 4-wave blocks; 0: the class's wide block
     int tile_lds = 0;  // bytes of dynamic LDS (tiled)
     std::vector<TileItem> titems;
@@ -153,9 +156,10 @@ constexpr int NSIDE = 4;  // side streams that run class kernels concurrently
 
 constexpr int NO_FP32 = 99;  // bin-sum threshold that sends no quartet to FP32
 
-// Kernel variant of a class: untiled (ket-/bra-stationary or MD), tiled with
-// 4-wave blocks, tiled with the class's wide block.
-enum { VARIANT_UNTILED = 0, VARIANT_TILED4 = 1, VARIANT_TILED_WIDE = 2, NVARIANTS = 3 };
+// Kernel variant of a class: untiled (generated ket-/bra-stationary kernel),
+// tiled with 4-wave blocks, tiled with the class's wide block, matrix-core MD.
+// Classes without generated recursions run the matrix-core MD kernel only.
+enum { VARIANT_UNTILED = 0, VARIANT_TILED4 = 1, VARIANT_TILED_WIDE = 2, VARIANT_MMD = 3, NVARIANTS = 4 };
 
 struct vlx_context {
     int device = 0;
@@ -169,7 +173,9 @@ struct vlx_context {
     int tiled_lmax = 4;
     int force_variant = -1;         // autotuning: every class runs this variant (VARIANT_*)
     std::map<int, int> tuned_variant;  // class entry -> VARIANT_* of the last autotune (default for new screeners)
-    int group_atoms = 12;           // atoms per group of the tiled kernels (VLX_GROUP_ATOMS)
+    // atoms per group of the tiled kernels (VLX_GROUP_ATOMS); 0: the largest of 12, 8, 6, 4, 3,
+    // 2, 1 whose (ps|ss) tiles fit max_tile_lds (12 for def2-SVP waters, 8 for def2-TZVP)
+    int group_atoms = 0;
     int tiled_waves = 0;            // 4: 4-wave blocks for every tiled class; 0: per-class default (VLX_TILED_WAVES)
     int max_tile_lds = 96 * 1024;   // larger tiles fall back to the untiled kernel
     // primitives of one ket pair per lane of the tiled kernels (VLX_KET_PRIMS, 1..16): a ket
@@ -212,6 +218,7 @@ struct vlx_screener {
     // atom groups of the tiled kernels
     std::vector<int> shell_group, shell_loc;  // per shell: group, local AO index of component 0
     int ngroups = 0;
+    int group_atoms = 0;  // atoms per group in use
     std::vector<int> grp_off, grp_n, grp_ao;  // (ngroups x 4) offsets/sizes, flat global AO lists
     int grp_max[4] = {0, 0, 0, 0};           // largest (group, l) block per l
     DevBuf<int> d_grp_off, d_grp_n, d_grp_ao;
@@ -509,8 +516,12 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
             if (vit != s->variant.end()) var = vit->second;
             else if (cit != s->ctx->tuned_variant.end()) var = cit->second;
         }
-        if (!can_tile) var = VARIANT_UNTILED;
-        if (var != VARIANT_UNTILED) {
+        if ((var == VARIANT_TILED4 || var == VARIANT_TILED_WIDE) && !can_tile) var = VARIANT_UNTILED;
+        if (var == VARIANT_MMD && !ce.mmd) var = VARIANT_UNTILED;
+        if (ce.variant == VLX_WAVE_MD) var = VARIANT_MMD;  // no generated kernel
+        sch->variant = var;
+        sch->mmd = var == VARIANT_MMD;
+        if (var == VARIANT_TILED4 || var == VARIANT_TILED_WIDE) {
             // tiled items: (bra segment prefix, chunk of <= 64 kets of a ket segment)
             const PairSet& tb = unscreened ? s->all_tiled_bra[ce.x] : s->kept_tiled_bra[ce.x];
             const PairSet& tk = unscreened ? s->all_tiled[ce.y] : s->kept_tiled[ce.y];
@@ -543,7 +554,7 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
                     }
                 }
             }
-        } else if (ce.variant == VLX_KET_STATIONARY) {
+        } else if (!sch->mmd && ce.variant == VLX_KET_STATIONARY) {
             // work items: 64 consecutive kets x a chunk of bras; the chunk shrinks
             // for classes with little work so that the launch still has enough items
             std::vector<std::pair<int, int>> ranges;
@@ -569,8 +580,8 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
                     sch->items.push_back({static_cast<int>(64 * t), i0, std::min(ranges[t].second, i0 + chunk), 0});
         } else {
             // work items: one bra x a run of the ket prefix its bins allow
-            // (64 kets, one per lane; or a few kets for the cooperative kernel)
-            const int kstep = (ce.variant == VLX_WAVE_MD) ? 4 : 64;
+            // (64 kets, one per lane; or a few kets for the matrix-core MD kernel)
+            const int kstep = sch->mmd ? 4 : 64;
             for (int i = 0; i < bra.size(); i++) {
                 int nk;
                 if (unscreened) {
@@ -581,7 +592,7 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
                 }
                 if (same) nk = std::min(nk, bra.pairs[i].rank + 1);
                 for (int j0 = 0; j0 < nk; j0 += kstep)
-                    sch->items.push_back({j0, i, (ce.variant == VLX_WAVE_MD) ? std::min(nk, j0 + kstep) : nk, 0});
+                    sch->items.push_back({j0, i, sch->mmd ? std::min(nk, j0 + kstep) : nk, 0});
             }
         }
         // schedule efficiency: slots the wavefronts execute, masked lanes included
@@ -603,7 +614,7 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
                 sch->prim_slots32 += (tb.nprim_prefix[ti.i1] - tb.nprim_prefix[ti.i0]) * 64 * kmax(ti);
             }
             sch->prim_slots += sch->prim_slots32;
-        } else if (ce.variant == VLX_KET_STATIONARY) {
+        } else if (!sch->mmd && ce.variant == VLX_KET_STATIONARY) {
             for (const auto& wi : sch->items) {
                 int kmax = 0;
                 for (int j = wi.j0; j < std::min(wi.j0 + 64, ket.size()); j++) kmax = std::max(kmax, ket.pairs[j].nprim);
@@ -612,16 +623,16 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
             }
         } else {
             for (const auto& wi : sch->items) {
-                const int j1 = (ce.variant == VLX_WAVE_MD) ? wi.i1 : std::min(wi.j0 + 64, wi.i1);
+                const int j1 = sch->mmd ? wi.i1 : std::min(wi.j0 + 64, wi.i1);
                 int kmax = 0;
                 for (int j = wi.j0; j < j1; j++) kmax = std::max(kmax, ket.pairs[j].nprim);
-                const int lanes = (ce.variant == VLX_WAVE_MD) ? (j1 - wi.j0) : 64;
+                const int lanes = sch->mmd ? (j1 - wi.j0) : 64;
                 sch->lane_slots += lanes;
                 sch->prim_slots += (long long)kmax * bra.pairs[wi.i0].nprim * lanes;
             }
         }
         if (sch->items.empty() && sch->titems.empty() && sch->titems32.empty()) continue;
-        if (!sch->tiled && ce.variant == VLX_WAVE_MD) {
+        if (sch->mmd) {
             int rc = ensure_u(s->ctx, unscreened ? s->all_bra[ce.x] : s->kept_bra[ce.x]);
             if (!rc) rc = ensure_u(s->ctx, unscreened ? s->all[ce.y] : s->kept[ce.y]);
             if (rc) return rc;
@@ -687,7 +698,7 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_MIN_ITEMS")) ctx->min_items = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("VLX_TILED")) ctx->use_tiled = std::atoi(e) != 0;
     if (const char* e = std::getenv("VLX_TILED_LMAX")) ctx->tiled_lmax = std::atoi(e);
-    if (const char* e = std::getenv("VLX_GROUP_ATOMS")) ctx->group_atoms = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("VLX_GROUP_ATOMS")) ctx->group_atoms = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("VLX_TILED_WAVES")) ctx->tiled_waves = std::atoi(e);
     if (const char* e = std::getenv("VLX_KET_PRIMS")) ctx->ket_prims = std::max(1, std::min(16, std::atoi(e)));
     if (const char* e = std::getenv("VLX_VARIANT")) ctx->force_variant = std::max(-1, std::min(NVARIANTS - 1, std::atoi(e)));
@@ -849,6 +860,34 @@ int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* 
     // atoms: runs of shells with identical centres; groups: consecutive atoms
     s->shell_group.assign(nshells, 0);
     s->shell_loc.assign(nshells, 0);
+    int group_atoms = ctx->group_atoms;
+    if (group_atoms <= 0) {
+        // functions per (atom, l), atoms = runs of shells on one centre
+        std::vector<std::array<int, 4>> per_atom;
+        for (int sh = 0; sh < nshells; sh++) {
+            const bool new_atom = sh == 0 || centers[3 * sh] != centers[3 * sh - 3] ||
+                                  centers[3 * sh + 1] != centers[3 * sh - 2] ||
+                                  centers[3 * sh + 2] != centers[3 * sh - 1];
+            if (new_atom) per_atom.push_back({0, 0, 0, 0});
+            per_atom.back()[l[sh]] += 2 * l[sh] + 1;
+        }
+        for (int g : {12, 8, 6, 4, 3, 2, 1}) {
+            group_atoms = g;
+            int gm0 = 0, gm1 = 0;
+            for (size_t a0 = 0; a0 < per_atom.size(); a0 += g) {
+                int n0 = 0, n1 = 0;
+                for (size_t a = a0; a < std::min(per_atom.size(), a0 + g); a++) {
+                    n0 += per_atom[a][0];
+                    n1 += per_atom[a][1];
+                }
+                gm0 = std::max(gm0, n0);
+                gm1 = std::max(gm1, n1);
+            }
+            // tiles of (ps|ss): AB, AC, AD = p x s; CD, BC, BD = s x s
+            if (8LL * (3LL * gm1 * gm0 + 3LL * gm0 * gm0) <= ctx->max_tile_lds) break;
+        }
+    }
+    s->group_atoms = group_atoms;
     {
         int atom = 0, group = 0, atoms_in_group = 0;
         std::vector<int> count(4, 0);
@@ -862,7 +901,7 @@ int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* 
             if (new_atom) {
                 atom++;
                 atoms_in_group++;
-                if (atoms_in_group >= ctx->group_atoms) {
+                if (atoms_in_group >= group_atoms) {
                     for (int l = 0; l < 4; l++) all_lists.push_back(ao_lists[l]);
                     ao_lists.assign(4, {});
                     group++;
@@ -1120,14 +1159,14 @@ int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_densi
         if (rc) break;
         rc = drain_timing(ctx);
         if (rc) break;
-        // classes that cannot run this variant ran untiled: they only count for VARIANT_UNTILED
-        std::map<int, bool> ran_tiled;
+        // classes that cannot run this variant ran another one: they count only for the one they ran
+        std::map<int, int> ran;
         std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
         rc = get_schedules(scr, ithreshold, NO_FP32, &list);
         if (rc) break;
-        for (const auto& sp : *list) ran_tiled[sp->entry] = sp->tiled;
+        for (const auto& sp : *list) ran[sp->entry] = sp->variant;
         for (const auto& kv : ctx->class_ms) {
-            if ((v != VARIANT_UNTILED) != ran_tiled[kv.first]) continue;
+            if (ran[kv.first] != v) continue;
             const double ms = kv.second.first / repeats;
             auto it = best_ms.find(kv.first);
             if (it == best_ms.end() || ms < it->second) {
@@ -1177,8 +1216,8 @@ int vlx_fock_class_kernel(vlx_context* ctx, vlx_screener* scr, int ithreshold, i
             std::snprintf(name, len, "fock_kernel_tiled<%d,%d,%d,%d,double,%d,false>", la, lb, lc, ld,
                           sp->tiled_waves == 4 ? 4 : wide);
         } else {
-            const char* k = (ce.variant == VLX_KET_STATIONARY) ? "fock_kernel"
-                            : (ce.variant == VLX_WAVE_MD)       ? "fock_kernel_mmd"
+            const char* k = sp->mmd                             ? "fock_kernel_mmd"
+                            : (ce.variant == VLX_KET_STATIONARY) ? "fock_kernel"
                                                                 : "fock_kernel_bs";
             std::snprintf(name, len, "%s<%d,%d,%d,%d,false>", k, la, lb, lc, ld);
         }
@@ -1322,6 +1361,8 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
                 const int grid32 = static_cast<int>(std::min<long long>(a32.sc.nitems, 4096));
                 ce.tiled32(a32, grid32, ctx->side[sch.stream]);
             }
+        } else if (sch.mmd) {
+            ce.mmd(a, grid, ctx->side[sch.stream]);
         } else {
             ce.fock(a, grid, ctx->side[sch.stream]);
         }

@@ -128,6 +128,7 @@ struct ClassLaunchers {
     EriLaunchFn eri;
     FockLaunchFn tiled;   // LDS-tiled Fock kernel, or nullptr
     FockLaunchFn tiled32; // the tiled kernel with FP32 recursions (mixed precision), or nullptr
+    FockLaunchFn mmd;     // matrix-core McMurchie-Davidson Fock kernel (MD work items), or nullptr
 };
 
 struct DiagLauncher {
@@ -143,3 +144,4 @@ struct DiagLauncher {
     void launch_tiled_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s); \
     void launch_tiled32_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);
 #define VLX_DECLARE_DIAG(la, lb) void launch_diag_##la##lb(const DiagArgs& a, hipStream_t s);
+#define VLX_DECLARE_MMD(la, lb, lc, ld) void launch_mmd_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);
