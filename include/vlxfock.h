@@ -95,7 +95,7 @@ int vlx_screener_number_of_pairs(const vlx_screener* scr, long long* out);
  * ithreshold >= 0 applies the binned Schwarz selection bin_bra + bin_ket <=
  * ithreshold; ithreshold < 0 computes every shell quartet (the reference's
  * unscreened _compute_fock_omp(basis, molecule, ...) path).  D must be
- * symmetric.
+ * symmetric (vlx_fock_compute_general takes any D).
  * Host buffers: density in, G out (both nao x nao).
  */
 int vlx_fock_compute(vlx_context* ctx, vlx_screener* scr, const double* density, const char* label,
@@ -105,6 +105,22 @@ int vlx_fock_compute(vlx_context* ctx, vlx_screener* scr, const double* density,
  * the context's stream and returns without synchronising. */
 int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,
                             double exchange_factor, double omega, int ithreshold, double* d_fock_out);
+
+/*
+ * CFockDriver::compute for a general (non-symmetric) density, the reference's
+ * mat_t.general path (src/t4c_func/FockDriver.cpp:120-171 with the general
+ * distributors of src/t4c_func/T4CLocalDistributor.cpp; its tests compare K
+ * with the transposed k.gen fixture, tests/test_fock_driver.py:320-360).
+ * J(D) = J(S) and K(D) = K(S) + K(A) for S = (D + D^T)/2, A = (D - D^T)/2:
+ * one build of S plus, for labels with exchange, an exchange-only build of A
+ * whose F' enters antisymmetrised.  Same labels and arguments as
+ * vlx_fock_compute; host buffers (nao x nao).
+ */
+int vlx_fock_compute_general(vlx_context* ctx, vlx_screener* scr, const double* density, const char* label,
+                             double exchange_factor, double omega, int ithreshold, double* fock_out);
+/* The same on device buffers (context stream, no synchronisation). */
+int vlx_fock_compute_general_device(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,
+                                    double exchange_factor, double omega, int ithreshold, double* d_fock_out);
 
 /* Device memory helpers (HBM of the context's device) for callers that keep
  * D and G resident between builds. */

@@ -441,9 +441,9 @@ static void digest(const oracle_basis_t *bs, int sa, int sb, int sc, int sd, con
     }
 }
 
-/* internal: build with an optional bra-pair stride (1 = everything) */
+/* internal: build with an optional bra-pair stride (1 = everything); antisym: G = F' - F'^T */
 static long fock_impl(const oracle_basis_t *bs, const double *dm, int label, double xf, double omega, int ithreshold,
-                      int nthreads, int stride, const int *bins_in, double *gout, long *total);
+                      int nthreads, int stride, const int *bins_in, double *gout, long *total, int antisym);
 
 /*
  * Bounded CPU-baseline sample: only bra pairs ip with ip % stride == 0 are
@@ -453,7 +453,7 @@ static long fock_impl(const oracle_basis_t *bs, const double *dm, int label, dou
 long oracle_fock_sampled(const oracle_basis_t *bs, const double *dm, int label, double xf, int ithreshold,
                          int nthreads, int stride, const int *bins, double *gout, long *total)
 {
-    return fock_impl(bs, dm, label, xf, 0.0, ithreshold, nthreads, stride, bins, gout, total);
+    return fock_impl(bs, dm, label, xf, 0.0, ithreshold, nthreads, stride, bins, gout, total, 0);
 }
 
 /*
@@ -467,7 +467,7 @@ long oracle_fock(const oracle_basis_t *bs, const double *dm, int label, double x
                  double *gout)
 {
     long total = 0;
-    return fock_impl(bs, dm, label, xf, 0.0, ithreshold, nthreads, 1, NULL, gout, &total);
+    return fock_impl(bs, dm, label, xf, 0.0, ithreshold, nthreads, 1, NULL, gout, &total, 0);
 }
 
 /*
@@ -480,12 +480,57 @@ long oracle_fock_erf(const oracle_basis_t *bs, const double *dm, int label, doub
 {
     long total = 0;
     if (label != LBL_J && label != LBL_K && label != LBL_KX) return -1;
-    return fock_impl(bs, dm, label, xf, omega, ithreshold, nthreads, 1, NULL, gout, &total);
+    return fock_impl(bs, dm, label, xf, omega, ithreshold, nthreads, 1, NULL, gout, &total, 0);
+}
+
+/*
+ * General (non-symmetric) density, as the reference digests mat_t.general
+ * densities (its tests: tests/test_fock_driver.py:320-360, K compared with
+ * the transposed k.gen fixture).  With S = (D + D^T)/2 and A = (D - D^T)/2:
+ * J(D) = J(S) because (ab|cd) = (ab|dc), and K(D) = K(S) + K(A): with the
+ * reference's K_ac = sum_bd (ab|cd) D_bd and the unique-quartet digestion
+ * F'_ac += (ab|cd) A_bd of A, K(A) = F'_A - F'_A^T (antisymmetric, zero
+ * diagonal).  Returns the quartets of the S build.
+ */
+long oracle_fock_general(const oracle_basis_t *bs, const double *dm, int label, double xf, int ithreshold,
+                         int nthreads, double *gout)
+{
+    const int n = bs->nao;
+    const size_t nn = (size_t)n * n;
+    double kc;
+    switch (label) {
+    case LBL_2JK: kc = -1.0; break;
+    case LBL_2JKX: kc = -xf; break;
+    case LBL_J: kc = 0.0; break;
+    case LBL_K: kc = 1.0; break;
+    case LBL_KX: kc = xf; break;
+    default: return -1;
+    }
+    double *s = calloc(nn, sizeof(double)), *a = calloc(nn, sizeof(double));
+    for (int p = 0; p < n; p++)
+        for (int q = 0; q < n; q++) {
+            const double x = dm[(size_t)p * n + q], y = dm[(size_t)q * n + p];
+            s[(size_t)p * n + q] = 0.5 * (x + y);
+            a[(size_t)p * n + q] = 0.5 * (x - y);
+        }
+    long total = 0;
+    long nq = fock_impl(bs, s, label, xf, 0.0, ithreshold, nthreads, 1, NULL, gout, &total, 0);
+    if (nq >= 0 && kc != 0.0) {
+        double *ga = malloc(sizeof(double) * nn);
+        const long na = fock_impl(bs, a, LBL_K, 1.0, 0.0, ithreshold, nthreads, 1, NULL, ga, &total, 1);
+        if (na < 0) nq = -1;
+        else
+            for (size_t i = 0; i < nn; i++) gout[i] += kc * ga[i];  /* ga = F'_A - F'_A^T */
+        free(ga);
+    }
+    free(s);
+    free(a);
+    return nq;
 }
 
 /* bins_in: optional nshells x nshells pair bins from oracle_pair_bins */
 static long fock_impl(const oracle_basis_t *bs, const double *dm, int label, double xf, double omega, int ithreshold,
-                      int nthreads, int stride, const int *bins_in, double *gout, long *total)
+                      int nthreads, int stride, const int *bins_in, double *gout, long *total, int antisym)
 {
     const int ns = bs->nshells, n = bs->nao;
     if (stride < 1) return -1;
@@ -538,12 +583,12 @@ static long fock_impl(const oracle_basis_t *bs, const double *dm, int label, dou
         free(fp);
         free(buf);
     }
-    /* symmetrize: G = F' + F'^T */
+    /* symmetrize: G = F' + F'^T (antisym: G = F' - F'^T) */
     for (int p = 0; p < n; p++)
         for (int q = p; q < n; q++) {
-            const double v = gout[(size_t)p * n + q] + gout[(size_t)q * n + p];
-            gout[(size_t)p * n + q] = v;
-            gout[(size_t)q * n + p] = v;
+            const double x = gout[(size_t)p * n + q], y = gout[(size_t)q * n + p];
+            gout[(size_t)p * n + q] = antisym ? x - y : x + y;
+            gout[(size_t)q * n + p] = antisym ? y - x : x + y;
         }
     free(maxv);
     free(bins);

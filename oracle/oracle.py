@@ -122,6 +122,24 @@ def fock(ob, density, label='2jk', exchange_factor=1.0, ithreshold=15, nthreads=
     return g, int(nq)
 
 
+def fock_general(ob, density, label='2jk', exchange_factor=1.0, ithreshold=15, nthreads=0):
+    """Build for a general (non-symmetric) density: J of its symmetric part plus the exchange
+    of the whole density (oracle_fock_general); returns (G, number of shell quartets)."""
+    n = ob.struct.nao
+    dm = np.ascontiguousarray(density, dtype=np.float64)
+    assert dm.shape == (n, n)
+    g = np.zeros((n, n))
+    L = lib()
+    L.oracle_fock_general.restype = ctypes.c_long
+    nq = L.oracle_fock_general(ctypes.byref(ob.struct), dm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                               ctypes.c_int(LABELS[label]), ctypes.c_double(exchange_factor),
+                               ctypes.c_int(ithreshold), ctypes.c_int(nthreads),
+                               g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if nq < 0:
+        raise ValueError('oracle_fock_general: bad input')
+    return g, int(nq)
+
+
 def fock_erf(ob, density, label='k', exchange_factor=1.0, omega=0.0, ithreshold=15, nthreads=0):
     """Range-separated build (labels j/k/kx on erf(omega r)/r integrals): (G, quartets)."""
     n = ob.struct.nao

@@ -123,6 +123,52 @@ def test_dimer_svpd_fixtures(label):
     assert reference_equal(g, ref)
 
 
+@pytest.mark.parametrize('label,xf', [('j', 0.0), ('k', 0.0), ('2jk', 0.0), ('2jkx', 0.31), ('kx', 0.27)])
+def test_dimer_svpd_general_density_fixtures(label, xf):
+    """mat_t.general density: reference tests/test_fock_driver.py:320-360 (J = j.gen, K = k.gen^T)."""
+    mol, bas = dimer('def2-svpd')
+    scr = _screened(mol, bas)
+    dm = make_matrix(bas, mat_t.general)
+    dm.set_values(load('h2o.dimer.svpd.density.gen.npy'))
+    g = FockDriver().compute(scr, dm, label, xf, 0.0, 15)
+    assert g.get_type() == mat_t.general
+    j, kt = load('h2o.dimer.svpd.j.gen.npy'), load('h2o.dimer.svpd.k.gen.npy').T
+    ref = {'j': j, 'k': kt, '2jk': 2 * j - kt, '2jkx': 2 * j - xf * kt, 'kx': xf * kt}[label]
+    assert reference_equal(g.to_numpy(), ref)
+
+
+def test_general_density_matrices_overload():
+    """Matrices + label list with general densities (reference tests/test_fock_driver.py:60-140)."""
+    from veloxchem_amd import Matrices
+    mol, bas = dimer('def2-svpd')
+    scr = _screened(mol, bas)
+    dm = make_matrix(bas, mat_t.general)
+    dm.set_values(load('h2o.dimer.svpd.density.gen.npy'))
+    dms = Matrices()
+    dms.add(dm, '0')
+    dms.add(dm, '1')
+    out = FockDriver().compute(scr, dms, ['j', 'k'], 0.0, 0.0, 15)
+    assert reference_equal(out.matrix('0').to_numpy(), load('h2o.dimer.svpd.j.gen.npy'))
+    assert reference_equal(out.matrix('1').to_numpy(), load('h2o.dimer.svpd.k.gen.npy').T)
+
+
+def test_general_density_cluster_vs_oracle_1e10():
+    mol = water_cluster(4, 6.0, 7)
+    bas = MolecularBasis.read(mol, 'def2-svp')
+    scr = _screened(mol, bas)
+    n = bas.get_dimensions_of_basis()
+    rng = np.random.default_rng(5)
+    d = pseudo_density(n, 20, seed=3) + 0.05 * rng.standard_normal((n, n))
+    dm = make_matrix(bas, mat_t.general)
+    dm.set_values(d)
+    ob = orc.basis_from(bas, mol)
+    for label, xf in (('2jk', 0.0), ('2jkx', 0.4)):
+        g = FockDriver().compute(scr, dm, label, xf, 0.0, 12).to_numpy()
+        ref, _ = orc.fock_general(ob, d, label, xf, 12)
+        assert np.max(np.abs(g - ref)) < 1e-10
+        assert np.max(np.abs(g - g.T)) > 1e-3  # the exchange of the antisymmetric part is there
+
+
 def test_dimer_tzvp_fixture_2jk():
     from veloxchem_amd import _native
     if _native.lib().vlx_max_angular_momentum() < 3:

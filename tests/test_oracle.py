@@ -113,6 +113,30 @@ def test_fock_dimer_fixtures(basis, tag):
     assert reference_equal(g, 0.23 * k)
 
 
+@pytest.mark.parametrize('label,xf', [('j', 0.0), ('k', 0.0), ('2jk', 0.0), ('2jkx', 0.31), ('kx', 0.27)])
+def test_fock_general_density_fixtures(label, xf):
+    """Non-symmetric density: reference tests/test_fock_driver.py:60-140 and :320-360 (def2-SVPD
+    dimer, ithreshold 15) compare J with j.gen and K with the transposed k.gen fixture."""
+    mol, bas = dimer('def2-svpd')
+    ob = orc.basis_from(bas, mol)
+    dm = load('h2o.dimer.svpd.density.gen.npy')
+    j = load('h2o.dimer.svpd.j.gen.npy')
+    kt = load('h2o.dimer.svpd.k.gen.npy').T
+    ref = {'j': j, 'k': kt, '2jk': 2.0 * j - kt, '2jkx': 2.0 * j - xf * kt, 'kx': xf * kt}[label]
+    g, _ = orc.fock_general(ob, dm, label, xf, 15)
+    assert reference_equal(g, ref)
+
+
+def test_fock_general_reduces_to_symmetric():
+    # a symmetric density through the general path gives the symmetric build exactly
+    mol, bas = dimer('def2-svpd')
+    ob = orc.basis_from(bas, mol)
+    dm = load('h2o.dimer.svpd.density.npy')
+    g1, _ = orc.fock(ob, dm, '2jk', 0.0, 15)
+    g2, _ = orc.fock_general(ob, dm, '2jk', 0.0, 15)
+    assert np.max(np.abs(g1 - g2)) < 1e-13
+
+
 def test_planted_mistake_is_caught():
     # a dropped exchange factor in the label handling must fail the fixture check
     mol, bas = sto3g_water()

@@ -108,6 +108,24 @@ def test_scf_dimer_oracle():
     assert abs(res['scf_energy'] - (-151.9220860932)) < 1.0e-8
 
 
+def test_scf_incremental_fock_oracle():
+    # difference-density builds reach the full-build energy; the last energy is from a full build
+    mol = dimer()
+    bas, min_bas = MolecularBasis.read(mol, 'def2-svp'), MolecularBasis.read(mol, 'ao-start-guess')
+    ref = _oracle_driver()
+    e_full = ref.compute(mol, bas, min_bas)['scf_energy']
+    drv = _oracle_driver()
+    drv.incremental_fock = True
+    res = drv.compute(mol, bas, min_bas)
+    assert drv.is_converged
+    inc = [h for h in drv.history if h['incremental']]
+    assert len(inc) >= 3
+    assert all(h['build_threshold'] < h['eri_threshold'] for h in inc)
+    assert not drv.history[-1]['incremental']
+    assert abs(res['scf_energy'] - e_full) < 1.0e-9
+    assert abs(res['scf_energy'] - (-151.9220860932)) < 1.0e-8
+
+
 def test_scf_rejects_open_shell():
     mol = water()
     mol.set_charge(1)
@@ -130,6 +148,20 @@ def test_scf_dimer_gpu():
     mol = dimer()
     drv = ScfRestrictedDriver()
     res = drv.compute(mol, MolecularBasis.read(mol, 'def2-svp'))
+    assert abs(res['scf_energy'] - (-151.9220860932)) < 1.0e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_scf_dimer_gpu_incremental(precision):
+    mol = dimer()
+    drv = ScfRestrictedDriver()
+    drv.incremental_fock = True
+    drv.precision = precision
+    drv.mixed_threshold = 1.0e-8
+    res = drv.compute(mol, MolecularBasis.read(mol, 'def2-svp'))
+    assert drv.is_converged
+    assert sum(h['incremental'] for h in drv.history) >= 3
     assert abs(res['scf_energy'] - (-151.9220860932)) < 1.0e-8
 
 

@@ -188,6 +188,7 @@ struct vlx_context {
     DevBuf<double> d_boys;
     DevBuf<float> d_boys_f;         // the Boys table in FP32 (mixed precision)
     DevBuf<double> d_density, d_fock, d_faccum;
+    DevBuf<double> d_sym, d_anti;   // symmetric and antisymmetric parts of a general density
     DevBuf<unsigned long long> d_dmax;  // max |D| of the current build (bit pattern)
     double mixed_tau = 0.0;         // mixed precision: FP32 below this Schwarz x max|D| bound; 0 = FP64 only
     int last_k32 = NO_FP32;         // bin-sum threshold of the last build
@@ -460,6 +461,26 @@ __global__ void symmetrize_kernel(const double* __restrict__ f, double* __restri
     if (q < n) g[(size_t)p * n + q] = f[(size_t)p * n + q] + f[(size_t)q * n + p];
 }
 
+// G += c (F' - F'^T)
+__global__ void antisym_add_kernel(const double* __restrict__ f, double* __restrict__ g, int n, double c)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (q < n) g[(size_t)p * n + q] += c * (f[(size_t)p * n + q] - f[(size_t)q * n + p]);
+}
+
+// S = (D + D^T) / 2, A = (D - D^T) / 2
+__global__ void split_kernel(const double* __restrict__ d, double* __restrict__ s, double* __restrict__ a, int n)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (q < n) {
+        const double x = d[(size_t)p * n + q], y = d[(size_t)q * n + p];
+        s[(size_t)p * n + q] = 0.5 * (x + y);
+        a[(size_t)p * n + q] = 0.5 * (x - y);
+    }
+}
+
 // Work items of every class at ithreshold.  k32 < NO_FP32 also splits the
 // tiled items: the bras of an item whose bin plus the smallest ket bin of
 // its chunk reaches k32 form an FP32 item (every quartet in it has
@@ -670,7 +691,7 @@ int check_symmetric_host(const double* d, int n)
         for (int q = p + 1; q < n; q++) {
             const double a = d[(size_t)p * n + q], b = d[(size_t)q * n + p];
             if (std::fabs(a - b) > 1.0e-12 * std::max(1.0, std::max(std::fabs(a), std::fabs(b))))
-                return fail(VLX_ERR_ARG, "density is not symmetric at (%d, %d): general densities are not built", p, q);
+                return fail(VLX_ERR_ARG, "density is not symmetric at (%d, %d): use vlx_fock_compute_general", p, q);
         }
     return VLX_OK;
 }
@@ -1256,20 +1277,15 @@ int vlx_fock_work(vlx_screener* scr, int ithreshold, int max_records, int* nreco
     return VLX_OK;
 }
 
-int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,
-                            double exchange_factor, double omega, int ithreshold, double* d_fock_out)
+namespace {
+
+// One build on a device density: F' by the class kernels, then
+// G = F' + F'^T into d_out (anti_coef == 0), or d_out += anti_coef (F' - F'^T)
+// (the exchange of the antisymmetric part of a general density).
+int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, double fj, double fk, double omega2,
+               int ithreshold, double* d_fock_out, double anti_coef)
 {
-    if (!ctx || !scr || !d_density || !d_fock_out) return fail(VLX_ERR_ARG, "NULL argument to vlx_fock_compute_device");
-    if (scr->ctx != ctx) return fail(VLX_ERR_ARG, "screener belongs to another context");
-    if (ithreshold > 30) return fail(VLX_ERR_ARG, "ithreshold %d outside the accepted range (< 0 or 0..30)", ithreshold);
-    double fj = 0.0, fk = 0.0;
-    bool range_separated = false;
-    int rc = label_factors(label, exchange_factor, &fj, &fk, &range_separated);
-    if (rc) return rc;
-    if (range_separated && !std::isfinite(omega)) return fail(VLX_ERR_ARG, "omega must be finite");
-    // erf(0 r)/r = 0: a range-separated build with omega = 0 is zero, as in the reference
-    const double omega2 = range_separated ? omega * omega : 0.0;
-    if (range_separated && omega2 == 0.0) fj = fk = 0.0;
+    int rc = VLX_OK;
     HIP_TRY(hipSetDevice(ctx->device));
     const int n = scr->nao;
     const size_t nn = (size_t)n * n;
@@ -1377,9 +1393,57 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
         HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_done[k], 0));
     }
     dim3 sg((n + 255) / 256, n);
-    symmetrize_kernel<<<sg, 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, d_fock_out, n);
+    if (anti_coef == 0.0) symmetrize_kernel<<<sg, 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, d_fock_out, n);
+    else antisym_add_kernel<<<sg, 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, d_fock_out, n, anti_coef);
     HIP_TRY(hipGetLastError());
     return VLX_OK;
+}
+
+int parse_build(vlx_context* ctx, vlx_screener* scr, const void* d_density, const void* d_out, const char* label,
+                double exchange_factor, double omega, int ithreshold, double* fj, double* fk, double* omega2)
+{
+    if (!ctx || !scr || !d_density || !d_out) return fail(VLX_ERR_ARG, "NULL argument to the Fock build");
+    if (scr->ctx != ctx) return fail(VLX_ERR_ARG, "screener belongs to another context");
+    if (ithreshold > 30) return fail(VLX_ERR_ARG, "ithreshold %d outside the accepted range (< 0 or 0..30)", ithreshold);
+    bool range_separated = false;
+    int rc = label_factors(label, exchange_factor, fj, fk, &range_separated);
+    if (rc) return rc;
+    if (range_separated && !std::isfinite(omega)) return fail(VLX_ERR_ARG, "omega must be finite");
+    // erf(0 r)/r = 0: a range-separated build with omega = 0 is zero, as in the reference
+    *omega2 = range_separated ? omega * omega : 0.0;
+    if (range_separated && *omega2 == 0.0) *fj = *fk = 0.0;
+    return VLX_OK;
+}
+
+}  // namespace
+
+int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,
+                            double exchange_factor, double omega, int ithreshold, double* d_fock_out)
+{
+    double fj = 0.0, fk = 0.0, omega2 = 0.0;
+    int rc = parse_build(ctx, scr, d_density, d_fock_out, label, exchange_factor, omega, ithreshold, &fj, &fk, &omega2);
+    if (rc) return rc;
+    return fock_build(ctx, scr, d_density, fj, fk, omega2, ithreshold, d_fock_out, 0.0);
+}
+
+int vlx_fock_compute_general_device(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,
+                                    double exchange_factor, double omega, int ithreshold, double* d_fock_out)
+{
+    double fj = 0.0, fk = 0.0, omega2 = 0.0;
+    int rc = parse_build(ctx, scr, d_density, d_fock_out, label, exchange_factor, omega, ithreshold, &fj, &fk, &omega2);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int n = scr->nao;
+    const size_t nn = (size_t)n * n;
+    HIP_TRY(ctx->d_sym.alloc(nn));
+    HIP_TRY(ctx->d_anti.alloc(nn));
+    dim3 sg((n + 255) / 256, n);
+    split_kernel<<<sg, 256, 0, ctx->stream>>>(d_density, ctx->d_sym.ptr, ctx->d_anti.ptr, n);
+    HIP_TRY(hipGetLastError());
+    // J(D) = J(S); K(D) = K(S) + K(A) with K(A) = F'_A - F'_A^T
+    rc = fock_build(ctx, scr, ctx->d_sym.ptr, fj, fk, omega2, ithreshold, d_fock_out, 0.0);
+    if (rc || fk == 0.0) return rc;
+    return fock_build(ctx, scr, ctx->d_anti.ptr, 0.0, 1.0, omega2, ithreshold, d_fock_out, fk);
 }
 
 int vlx_fock_compute(vlx_context* ctx, vlx_screener* scr, const double* density, const char* label,
@@ -1396,6 +1460,24 @@ int vlx_fock_compute(vlx_context* ctx, vlx_screener* scr, const double* density,
     HIP_TRY(hipMemcpyAsync(ctx->d_density.ptr, density, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     rc = vlx_fock_compute_device(ctx, scr, ctx->d_density.ptr, label, exchange_factor, omega, ithreshold,
                                  ctx->d_fock.ptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(fock_out, ctx->d_fock.ptr, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VLX_OK;
+}
+
+int vlx_fock_compute_general(vlx_context* ctx, vlx_screener* scr, const double* density, const char* label,
+                             double exchange_factor, double omega, int ithreshold, double* fock_out)
+{
+    if (!ctx || !scr || !density || !fock_out) return fail(VLX_ERR_ARG, "NULL argument to vlx_fock_compute_general");
+    const int n = scr->nao;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t nn = (size_t)n * n;
+    HIP_TRY(ctx->d_density.alloc(nn));
+    HIP_TRY(ctx->d_fock.alloc(nn));
+    HIP_TRY(hipMemcpyAsync(ctx->d_density.ptr, density, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int rc = vlx_fock_compute_general_device(ctx, scr, ctx->d_density.ptr, label, exchange_factor, omega, ithreshold,
+                                             ctx->d_fock.ptr);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(fock_out, ctx->d_fock.ptr, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -18,10 +18,14 @@ Labels: '2jk' -> 2J - K, '2jkx' -> 2J - xf K, 'j' -> J, 'k' -> K,
 (src/t4c_func/T4CMatrixDistributor.cpp:51-54).  omega is ignored by the
 other labels, as in the reference.
 
+A density Matrix of type mat_t.general (non-symmetric) takes
+vlx_fock_compute_general: J of its symmetric part plus the exchange of the
+whole density, K_pr = sum_qs (pq|rs) D_qs, as the reference's general path
+(its tests: tests/test_fock_driver.py:320-360).
+
 Deviations from the reference, by design: an unknown label raises
 ValueError (the reference silently returns zeros,
-src/t4c_func/T4CUtils.cpp:814-1062); non-symmetric densities raise
-NotImplementedError instead of computing; ithreshold must be < 0 (unscreened)
+src/t4c_func/T4CUtils.cpp:814-1062); ithreshold must be < 0 (unscreened)
 or 0..30.
 """
 
@@ -189,15 +193,15 @@ class FockDriver:
             raise ValueError(f'FockDriver: density has shape {dm.shape}, basis has {n} AOs')
         if not isinstance(label, str):
             raise TypeError('label must be a string')
-        if mat is not None and mat.get_type() == mat_t.general:
-            raise NotImplementedError('FockDriver: general (non-symmetric) densities are not built')
         if ithreshold > 30:
             raise ValueError(f'FockDriver: ithreshold {ithreshold} outside 0..30')
         if nodes != 1 or rank != 0:
             raise NotImplementedError('FockDriver: one process per GPU; rank/nodes partitioning is not built')
         dm = np.ascontiguousarray(dm)
         g = np.zeros((n, n))
-        _native.check(_native.lib().vlx_fock_compute(
+        general = mat is not None and mat.get_type() == mat_t.general
+        fn = _native.lib().vlx_fock_compute_general if general else _native.lib().vlx_fock_compute
+        _native.check(fn(
             _native.context(self._device), screener.handle, dm.ctypes.data_as(_native._c_double_p),
             label.encode(), ctypes.c_double(xf), ctypes.c_double(omega), int(ithreshold),
             g.ctypes.data_as(_native._c_double_p)))

@@ -144,6 +144,11 @@ class ScfRestrictedDriver:
         # 'fp64', or 'mixed': FP32 recursions where Schwarz bound x max|D| < mixed_threshold
         self.precision = 'fp64'
         self.mixed_threshold = 1.0e-8
+        # incremental (difference-density) Fock builds: G_n = G_{n-1} + G(D_n - D_{n-1}) with the
+        # screening threshold tightened by max|D_n - D_{n-1}|; a full build every full_fock_every
+        # iterations, when the ERI threshold changes, and to confirm convergence
+        self.incremental_fock = False
+        self.full_fock_every = 8
         self.screener_factory = None
         self.device = None
         self.history = []
@@ -271,12 +276,28 @@ class ScfRestrictedDriver:
         self._is_converged = False
         C = eps = F = last_f_eff = None
         fmax = 1.0 / math.sqrt(self.ovl_thresh)
+        g_prev = d_prev = None
+        full_thresh, since_full, confirm = None, 0, False
         for it in range(max_iter + 1):
             self._num_iter = it
             thresh = self.eri_thresh if e_grad is None else self._get_dyn_threshold(e_grad)
             thresh_int = int(-math.log10(thresh))
             t = time.perf_counter()
-            G = self.fock_2e(screener, den, thresh_int)
+            incremental = (self.incremental_fock and g_prev is not None and not confirm and
+                           thresh_int == full_thresh and since_full < self.full_fock_every)
+            build_thresh = thresh_int
+            if incremental:
+                # quartets with Schwarz bound x max|dD| below what a full build drops are skipped:
+                # bin sums > thresh_int + ceil(log10 max|dD|)
+                dd = (den - d_prev).contiguous()
+                dmax = float(dd.abs().max())
+                build_thresh = thresh_int + math.ceil(math.log10(dmax)) if dmax > 0.0 else -1
+                G = g_prev + self.fock_2e(screener, dd, build_thresh) if build_thresh >= 0 else g_prev.clone()
+                since_full += 1
+            else:
+                G = self.fock_2e(screener, den, thresh_int)
+                full_thresh, since_full = thresh_int, 0
+            g_prev, d_prev = G, den
             self.timings['fock'] += time.perf_counter() - t
             t = time.perf_counter()
             e_el = float(torch.sum(den * G) + 2.0 * torch.sum(den * hcore))
@@ -286,13 +307,18 @@ class ScfRestrictedDriver:
             e_grad = 2.0 * float(torch.linalg.norm(err))
             e_scf = e_el + e_nuc
             self.history.append({'energy': e_scf, 'gradient_norm': e_grad, 'max_gradient': float(err.abs().max()),
-                                 'diff_energy': e_scf - e_old, 'eri_threshold': thresh_int})
+                                 'diff_energy': e_scf - e_old, 'eri_threshold': thresh_int,
+                                 'incremental': incremental, 'build_threshold': build_thresh})
             e_old = e_scf
             self._scf_energy = e_scf
             if it > 0 and e_grad < conv_thresh:
-                self._is_converged = True
                 self.timings['linear_algebra'] += time.perf_counter() - t
+                if incremental:
+                    confirm = True  # the same density once more with a full build
+                    continue
+                self._is_converged = True
                 break
+            confirm = False
             if e_grad < self.diis_thresh:
                 if len(focks) == self.max_err_vecs:
                     focks.popleft()
