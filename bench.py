@@ -53,21 +53,82 @@ def build_workload(nmol, box, seed, basis):
     return mol, bas, scr, dm, t_screen
 
 
-def run_scf(mol, bas, conv_thresh, precision='fp64', tau=0.0):
+def run_scf(mol, bas, conv_thresh, precision='fp64', tau=0.0, incremental=False):
     from veloxchem_amd.scfdriver import ScfRestrictedDriver
     drv = ScfRestrictedDriver()
     drv.conv_thresh = conv_thresh
     drv.precision = precision
     drv.mixed_threshold = tau
+    drv.incremental_fock = incremental
     res = drv.compute(mol, bas)
     tm = res['timings']
     out = {'scf_time_s': round(tm['total'], 3), 'iterations': res['num_iter'], 'converged': res['is_converged'],
            'energy': res['scf_energy'], 'conv_thresh': conv_thresh, 'fock_time_s': round(tm['fock'], 3),
            'one_electron_s': round(tm['one_electron'], 3), 'linear_algebra_s': round(tm['linear_algebra'], 3),
-           'screener_s': round(tm['screener'], 3), 'guess_s': round(tm['guess'], 3), 'precision': precision}
+           'screener_s': round(tm['screener'], 3), 'guess_s': round(tm['guess'], 3), 'precision': precision,
+           'incremental_fock': incremental}
+    if incremental:
+        out['incremental_builds'] = sum(1 for h in drv.history if h['incremental'])
     if precision == 'mixed':
         out['mixed_threshold'] = tau
     return out
+
+
+def measure_builds(scr, dm, ithr, steps, warmup, taus, label=b'2jk', precision='fp64', tau=0.0):
+    """Timed builds on device-resident D (HIP events per class kernel), then the mixed sweep on the
+    same density: rate and error of each threshold against the FP64 build."""
+    from veloxchem_amd import _native
+    lib = _native.lib()
+    ctx = _native.context(0)
+    n = dm.shape[0]
+    d_dm = _native.DeviceArray((n, n))
+    d_g = _native.DeviceArray((n, n))
+    d_dm.upload(dm)
+
+    def one_build():
+        _native.check(lib.vlx_fock_compute_device(ctx, scr.handle, d_dm.ptr, label, 1.0, 0.0, ithr, d_g.ptr))
+
+    def timed(k):
+        for _ in range(max(1, warmup)):
+            one_build()
+        _native.check(lib.vlx_context_synchronize(ctx))
+        t0 = time.perf_counter()
+        for _ in range(k):
+            one_build()
+        _native.check(lib.vlx_context_synchronize(ctx))
+        return (time.perf_counter() - t0) * 1e3 / k
+
+    try:
+        _native.set_precision(precision, tau)
+        _native.check(lib.vlx_context_set_timing(ctx, 1))
+        _native.check(lib.vlx_context_reset_times(ctx))
+        ms = timed(steps)
+        _native.check(lib.vlx_context_set_timing(ctx, 0))
+        times = _native.class_times(0)
+        g = d_g.download()
+        _native.set_precision('fp64')
+        if precision == 'fp64':
+            g64, ms64 = g, ms
+        else:  # the FP64 reference of the sweep
+            ms64 = timed(steps)
+            g64 = d_g.download()
+        sweep = []
+        for t in taus:
+            _native.set_precision('mixed', t)
+            ms_m = timed(steps)
+            gm = d_g.download()
+            k32 = _native.last_fp32_bin_sum()
+            p64, p32 = scr.precision_split(ithr, k32)
+            entry = {'threshold': t, 'builds_per_s': 1e3 / ms_m, 'ms_per_step': ms_m, 'fp32_bin_sum': k32,
+                     'fp32_prim_fraction': p32 / max(1, p64 + p32), 'max_abs_dG': float(np.max(np.abs(gm - g64))),
+                     'dE2_vs_fp64': float(np.sum(dm * (gm - g64)))}
+            entry['speedup_vs_fp64'] = ms64 / ms_m
+            sweep.append(entry)
+    finally:
+        _native.set_precision('fp64')
+        d_dm.free()
+        d_g.free()
+    return ms, times, g, sweep
 
 
 def cpu_baseline(mol, bas, dm, ithr, target_s):
@@ -92,6 +153,20 @@ def cpu_baseline(mol, bas, dm, ithr, target_s):
                       f'{dt:.1f} s), extrapolated by quartet count'}
 
 
+def tzvp_builds(args, taus):
+    """Mixed-precision workload: seeded 64-water cluster (12.8 A box, seed 64), def2-TZVP
+    (2752 AOs, f shells): FP64 build rate and the mixed sweep with its error against FP64."""
+    mol, bas, scr, dm, t_screen = build_workload(64, 12.8, 64, 'def2-tzvp')
+    ms, times, _, sweep = measure_builds(scr, dm, args.ithreshold, args.tzvp_steps, 1, taus)
+    work = scr.work(args.ithreshold)
+    flops = sum(w[3] for w in work)
+    return {'workload': '64-water cluster (seed 64, 12.8 A box), def2-TZVP, '
+                        f'{bas.get_dimensions_of_basis()} AOs, 2jk on the SAD density, ithreshold {args.ithreshold}',
+            'fp64_builds_per_s': 1e3 / ms, 'fp64_ms_per_step': ms, 'steps': args.tzvp_steps,
+            'build_achieved_tflops': flops / (ms * 1e-3) / 1e12, 'screener_setup_s': round(t_screen, 3),
+            'shell_quartets': int(sum(w[1] for w in work)), 'mixed_sweep': sweep}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -113,6 +188,9 @@ def main():
                     help='Schwarz x max|D| thresholds of the mixed-precision sweep (comma list, empty: none)')
     ap.add_argument('--scf-conv', type=float, default=1.0e-8)
     ap.add_argument('--label', default='2jk', choices=['2jk', 'j', 'k'])
+    ap.add_argument('--no-tzvp', action='store_true',
+                    help='skip the 64-water def2-TZVP FP64 build and mixed-precision sweep')
+    ap.add_argument('--tzvp-steps', type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU; the multi-GPU sharding is design only '
@@ -120,18 +198,9 @@ def main():
         sys.exit(2)
 
     from veloxchem_amd import _native
-    lib = _native.lib()
-    ctx = _native.context(0)
+    _native.lib()
     mol, bas, scr, dm, t_screen = build_workload(args.nmol, args.box, args.seed, args.basis)
     n = bas.get_dimensions_of_basis()
-    d_dm = _native.DeviceArray((n, n))
-    d_g = _native.DeviceArray((n, n))
-    d_dm.upload(dm)
-    label = args.label.encode()
-
-    def one_build():
-        _native.check(lib.vlx_fock_compute_device(ctx, scr.handle, d_dm.ptr, label, 1.0, 0.0, args.ithreshold,
-                                                  d_g.ptr))
 
     tuned, t_tune = None, 0.0
     if not args.no_autotune:
@@ -140,47 +209,10 @@ def main():
         tuned = FockDriver()._autotune(scr, dm, args.ithreshold, repeats=2)
         t_tune = time.perf_counter() - t0
 
-    if args.precision == 'mixed':
-        _native.set_precision('mixed', args.tau)
-    for _ in range(args.warmup):
-        one_build()
-    _native.check(lib.vlx_context_synchronize(ctx))
-    _native.check(lib.vlx_context_set_timing(ctx, 1))
-    _native.check(lib.vlx_context_reset_times(ctx))
-    _native.check(lib.vlx_context_synchronize(ctx))
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        one_build()
-    _native.check(lib.vlx_context_synchronize(ctx))
-    t1 = time.perf_counter()
-    _native.check(lib.vlx_context_set_timing(ctx, 0))
-    times = _native.class_times(0)
-    g = d_g.download()
-    _native.set_precision('fp64')
-
-    # mixed-precision sweep on the same density: build rate and error vs the FP64 build
-    sweep = []
     taus = [float(x) for x in args.mixed.split(',') if x.strip()]
-    for tau in taus:
-        _native.set_precision('mixed', tau)
-        for _ in range(max(1, args.warmup)):
-            one_build()
-        _native.check(lib.vlx_context_synchronize(ctx))
-        t0m = time.perf_counter()
-        for _ in range(args.steps):
-            one_build()
-        _native.check(lib.vlx_context_synchronize(ctx))
-        ms_m = (time.perf_counter() - t0m) * 1e3 / args.steps
-        gm = d_g.download()
-        k32 = _native.last_fp32_bin_sum()
-        p64, p32 = scr.precision_split(args.ithreshold, k32)
-        sweep.append({'threshold': tau, 'builds_per_s': 1e3 / ms_m, 'ms_per_step': ms_m, 'fp32_bin_sum': k32,
-                      'fp32_prim_fraction': p32 / max(1, p64 + p32),
-                      'max_abs_dG': float(np.max(np.abs(gm - g))),
-                      'dE2_vs_fp64': float(np.sum(dm * (gm - g)))})
-    _native.set_precision('fp64')
-
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    ms_per_step, times, g, sweep = measure_builds(scr, dm, args.ithreshold, args.steps, args.warmup, taus,
+                                                  args.label.encode(), args.precision,
+                                                  args.tau if args.precision == 'mixed' else 0.0)
     work = scr.work(args.ithreshold)
     flops_build = sum(w[3] for w in work)
     by_class = {w[0]: w for w in work}
@@ -230,19 +262,29 @@ def main():
         'class_ms': {'%d%d%d%d' % c: round(v[0] / v[1], 4) for c, v in sorted(times.items())},
         'checksum': float(np.sum(g * dm)),
     }
-    d_dm.free()
-    d_g.free()
     if tuned:
         result['autotune'] = {'%d%d%d%d' % c: v for c, v in sorted(tuned.items())}
     if sweep:
         result['mixed_sweep'] = sweep
     if not args.no_scf:
+        # the dense-algebra libraries initialise on first use (solver handles, kernels):
+        # warm them up outside the timed SCF runs
+        import torch
+        w = torch.eye(64, dtype=torch.float64, device='cuda') + 0.01
+        torch.linalg.eigh(w @ w)
+        torch.cuda.synchronize()
         result['scf'] = run_scf(mol, bas, args.scf_conv)
+        inc = run_scf(mol, bas, args.scf_conv, incremental=True)
+        inc['dE_vs_fp64'] = inc['energy'] - result['scf']['energy']
+        result['scf_incremental'] = inc
         if taus:
             tau = max(taus)
-            mixed = run_scf(mol, bas, args.scf_conv, 'mixed', tau)
-            mixed['dE_vs_fp64'] = mixed['energy'] - result['scf']['energy']
-            result['scf_mixed'] = mixed
+            for key, incr in (('scf_mixed', False), ('scf_mixed_incremental', True)):
+                mixed = run_scf(mol, bas, args.scf_conv, 'mixed', tau, incremental=incr)
+                mixed['dE_vs_fp64'] = mixed['energy'] - result['scf']['energy']
+                result[key] = mixed
+    if not args.no_tzvp:
+        result['tzvp64'] = tzvp_builds(args, taus)
     if not args.no_cpu_baseline:
         result['cpu_baseline'] = cpu_baseline(mol, bas, dm, args.ithreshold, args.cpu_seconds)
     print(json.dumps(result))
