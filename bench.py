@@ -267,12 +267,9 @@ def main():
     if sweep:
         result['mixed_sweep'] = sweep
     if not args.no_scf:
-        # the dense-algebra libraries initialise on first use (solver handles, kernels):
-        # warm them up outside the timed SCF runs
-        import torch
-        w = torch.eye(64, dtype=torch.float64, device='cuda') + 0.01
-        torch.linalg.eigh(w @ w)
-        torch.cuda.synchronize()
+        # the dense-algebra libraries load their kernels for each matrix size on first use: one
+        # untimed SCF of the same molecule first (every run below builds its own screener)
+        run_scf(mol, bas, 1.0e-4)
         result['scf'] = run_scf(mol, bas, args.scf_conv)
         inc = run_scf(mol, bas, args.scf_conv, incremental=True)
         inc['dE_vs_fp64'] = inc['energy'] - result['scf']['energy']
