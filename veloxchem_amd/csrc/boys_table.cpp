@@ -37,7 +37,7 @@ void vlx_build_boys_table(double* tab)
                 if (j > 0) fact *= j;
                 const long double sgn = (j & 1) ? -1.0L : 1.0L;
                 row[j] = (double)(sgn * f[k][L + j] / fact);
+                row[BOYS_ORDER + 1 + j] = (L > 0) ? (double)(sgn * f[k][L - 1 + j] / fact) : 0.0;
             }
-            row[BOYS_ORDER + 1] = (double)expl(-(long double)k * BOYS_DT);
         }
 }

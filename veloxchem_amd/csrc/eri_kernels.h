@@ -67,20 +67,13 @@ template <> __device__ __forceinline__ float vrsqrt<float>(float x) { return __b
 template <int la, int lb, int lc, int ld>
 struct EriClass;
 
-// Boys function F_0..F_L(T).  T < BOYS_TMAX: F_L from the order-BOYS_ORDER
-// Taylor expansion around the nearest grid point (table row k holds
-// F_{L+j}(T_k) (-1)^j / j!), then downward recursion
-// F_m = (2T F_{m+1} + e^-T) / (2m+1).  T >= BOYS_TMAX: F_0 = sqrt(pi/T)/2
-// (erf(sqrt(T)) = 1 to double precision there) and upward recursion
-// F_{m+1} = ((2m+1) F_m - e^-T) / (2T), stable for T > L.
-// (-1)^n / n!
-constexpr double exp_series_coef(int n)
-{
-    double c = 1.0;
-    for (int m = 1; m <= n; m++) c /= m;
-    return (n & 1) ? -c : c;
-}
-
+// Boys function F_0..F_L(T).  T < BOYS_TMAX: F_L and F_{L-1} from their
+// order-BOYS_ORDER Taylor expansions around the nearest grid point (the two
+// blocks of table row k hold F_{L+j}(T_k) (-1)^j / j! and F_{L-1+j}(T_k)
+// (-1)^j / j!), then downward recursion F_m = (2T F_{m+1} + e^-T) / (2m+1).
+// T >= BOYS_TMAX: F_0 = sqrt(pi/T)/2 (erf(sqrt(T)) = 1 to double precision
+// there) and upward recursion F_{m+1} = ((2m+1) F_m - e^-T) / (2T), stable
+// for T > L.
 template <int L, typename R>
 __device__ __forceinline__ void boys_eval(const R T, const R* __restrict__ tab, R* __restrict__ F)
 {
@@ -96,13 +89,20 @@ __device__ __forceinline__ void boys_eval(const R T, const R* __restrict__ tab, 
         for (int j = ORD - 1; j >= 0; j--) f = fma(f, d, c[j]);
         F[L] = f;
         if constexpr (L > 0) {
-            // exp(-T) = exp(-T_k) * sum_{n <= ORD} (-d)^n / n!, Horner in d
-            R s = R(exp_series_coef(ORD));
+            // F_{L-1} from its own Taylor block of the row; then e^-T = (2L - 1) F_{L-1} - 2T F_L
+            // (the recursion at m = L - 1) for the downward steps.  At large T the difference
+            // cancels, but its error enters F_m with weight <= (2L - 1) F_{L-1} / ((2m + 1) F_m) <= 1
+            const R* c1 = c + (BOYS_ORDER + 1);
+            R g = c1[ORD];
 #pragma unroll
-            for (int n = ORD - 1; n >= 0; n--) s = fma(s, d, R(exp_series_coef(n)));
-            const R et = c[BOYS_ORDER + 1] * s;
+            for (int j = ORD - 1; j >= 0; j--) g = fma(g, d, c1[j]);
+            F[L - 1] = g;
+            if constexpr (L > 1) {
+                const R tt = R(2.0) * T;
+                const R et = R(2 * L - 1) * g - tt * f;
 #pragma unroll
-            for (int m = L - 1; m >= 0; m--) F[m] = (R(2.0) * T * F[m + 1] + et) * R(1.0 / (2 * m + 1));
+                for (int m = L - 2; m >= 0; m--) F[m] = (tt * F[m + 1] + et) * R(1.0 / (2 * m + 1));
+            }
         }
     } else {
         const R rs = vrsqrt<R>(T);
