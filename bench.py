@@ -223,6 +223,19 @@ def main():
     dom_kernel = _native.class_kernel(scr, args.ithreshold, dom)
     build_tflops = flops_build / (ms_per_step * 1e-3) / 1e12
 
+    # the same kernel timed alone: one stream serialises the class kernels (in the timed region
+    # above four streams share the chip, so each kernel's launch time includes its neighbours)
+    lib = _native.lib()
+    ctx = _native.context(0)
+    _native.check(lib.vlx_context_set_streams(ctx, 1))
+    try:
+        ms1, times1, _, _ = measure_builds(scr, dm, args.ithreshold, 2, 1, [], args.label.encode())
+    finally:
+        _native.check(lib.vlx_context_set_streams(ctx, max(1, min(4, int(os.environ.get('VLX_STREAMS', 4))))))
+    dom1 = max(times1, key=lambda c: times1[c][0])
+    dom1_ms = times1[dom1][0] / times1[dom1][1]
+    achieved1 = by_class[dom1][3] / (dom1_ms * 1e-3) / 1e12
+
     result = {
         'metric': 'J+K Fock builds/s + SCF time, mixed vs FP64 on GPU, at |\u0394E|<=1e-6 Eh',
         'mode': 'fp64 2jk build (value); fp64 RHF SCF (scf)',
@@ -258,6 +271,15 @@ def main():
             'kernel_share_of_build': times[dom][0] / sum(v[0] for v in times.values()),
             'build_achieved': build_tflops,
             'build_frac': build_tflops / FP64_VALU_PEAK_TFLOPS,
+        },
+        'roofline_single_stream': {
+            'kernel': _native.class_kernel(scr, args.ithreshold, dom1),
+            'achieved': achieved1,
+            'peak': FP64_VALU_PEAK_TFLOPS,
+            'unit': 'TFLOP/s',
+            'frac': achieved1 / FP64_VALU_PEAK_TFLOPS,
+            'kernel_ms': dom1_ms,
+            'build_ms_one_stream': ms1,
         },
         'class_ms': {'%d%d%d%d' % c: round(v[0] / v[1], 4) for c, v in sorted(times.items())},
         'checksum': float(np.sum(g * dm)),

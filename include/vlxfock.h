@@ -132,6 +132,12 @@ int vlx_copy_to_host(vlx_context* ctx, void* dst, const void* src, unsigned long
 /* Synchronise the context's stream. */
 int vlx_context_synchronize(vlx_context* ctx);
 
+/* Side streams the class kernels of later builds fan out to (1..4; the
+ * VLX_STREAMS environment variable sets the initial value, default 4).  No
+ * reference counterpart: one stream serialises the class kernels, for timing
+ * each in isolation. */
+int vlx_context_set_streams(vlx_context* ctx, int nstreams);
+
 /* Raw HIP stream of the context (hipStream_t), for event timing by callers. */
 void* vlx_context_stream(vlx_context* ctx);
 

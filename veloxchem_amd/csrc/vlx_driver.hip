@@ -378,12 +378,20 @@ void build_tiled(const vlx_screener& s, const PairSet& src, PairSet& ket, PairSe
             for (int b = std::max(pk.bin, 0); b < NBINS; b++) sg.cnt_le[b]++;
         }
     }
-    // entry ranges of the ket segments
+    // entry ranges of the ket segments; inside a segment the entries go by (bin, run length
+    // descending), so that a 64-entry chunk holds runs of equal length (its lanes loop to the
+    // chunk's longest run)
     ket.seg_e0.clear();
     ket.seg_e1.clear();
     for (const auto& sg : ket.segs) {
-        ket.seg_e0.push_back(static_cast<int>(eoff[sg.start]));
-        ket.seg_e1.push_back(static_cast<int>(eoff[sg.end]));
+        const int e0 = static_cast<int>(eoff[sg.start]), e1 = static_cast<int>(eoff[sg.end]);
+        ket.seg_e0.push_back(e0);
+        ket.seg_e1.push_back(e1);
+        std::stable_sort(ket.kentries.begin() + e0, ket.kentries.begin() + e1, [&](const int2& x, const int2& y) {
+            const int bx = ket.pairs[x.x].bin, by = ket.pairs[y.x].bin;
+            if (bx != by) return bx < by;
+            return (x.y >> 16) > (y.y >> 16);
+        });
     }
 }
 
@@ -553,22 +561,30 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
                 const auto& ks = tk.segs[sk];
                 for (int j0 = tk.seg_e0[sk]; j0 < tk.seg_e1[sk]; j0 += 64) {
                     const int j1 = std::min(tk.seg_e1[sk], j0 + 64);
-                    int bin_lo = NBINS;
-                    for (int j = j0; j < j1; j++) bin_lo = std::min(bin_lo, std::max(tk.pairs[tk.kentries[j].x].bin, 0));
+                    int bin_lo = NBINS, kmin = tk.kentries[j0].x;
+                    for (int j = j0; j < j1; j++) {
+                        bin_lo = std::min(bin_lo, std::max(tk.pairs[tk.kentries[j].x].bin, 0));
+                        kmin = std::min(kmin, tk.kentries[j].x);
+                    }
                     const int lim = unscreened ? NBINS - 1 : ithr - bin_lo;
                     if (lim < 0) continue;
                     // same class: bra segments at or after the ket segment (segment order is
                     // identical in both lists); the diagonal segment is masked by rank
                     for (size_t sb = same ? sk : 0; sb < tb.segs.size(); sb++) {
                         const auto& bs = tb.segs[sb];
-                        const int i0 = bs.start;
+                        int i0 = bs.start;
                         const int i1 = bs.start + (unscreened ? bs.end - bs.start : bs.cnt_le[std::min(lim, NBINS - 1)]);
+                        // diagonal segment: the tiled bra and ket orders are the same permutation
+                        // (position = rank), so bras before the chunk's lowest ket pair have a lower
+                        // rank than every ket of the chunk and would run fully masked
+                        if (same && sb == sk) i0 = std::max(i0, kmin);
                         if (i1 <= i0) continue;
                         // bras [im, i1) have bin >= k32 - bin_lo: FP32 for every lane of the chunk
                         int im = i1;
                         if (k32 < NO_FP32 && !unscreened) {
                             const int b = k32 - bin_lo - 1;
                             im = (b < 0) ? i0 : std::min(i1, bs.start + bs.cnt_le[std::min(b, NBINS - 1)]);
+                            im = std::max(im, i0);
                         }
                         if (im > i0) sch->titems.push_back({i0, im, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb});
                         if (i1 > im) sch->titems32.push_back({im, i1, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb});
@@ -1088,6 +1104,14 @@ int vlx_fock_schedule_stats(vlx_screener* scr, int ithreshold, int max_records, 
     return VLX_OK;
 }
 
+int vlx_context_set_streams(vlx_context* ctx, int nstreams)
+{
+    if (!ctx) return fail(VLX_ERR_ARG, "NULL context");
+    if (nstreams < 1 || nstreams > NSIDE) return fail(VLX_ERR_ARG, "nstreams %d outside 1..%d", nstreams, NSIDE);
+    ctx->nstreams = nstreams;
+    return VLX_OK;
+}
+
 int vlx_context_set_timing(vlx_context* ctx, int enable)
 {
     if (!ctx) return fail(VLX_ERR_ARG, "context is NULL");
@@ -1366,25 +1390,25 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
                     ctx->event_pool.pop_back();
                 }
             }
-            HIP_TRY(hipEventRecord(pend.start, ctx->side[sch.stream]));
+            HIP_TRY(hipEventRecord(pend.start, ctx->side[sch.stream % ctx->nstreams]));
         }
         if (sch.tiled) {
-            if (!sch.titems.empty()) ce.tiled(a, grid, ctx->side[sch.stream]);
+            if (!sch.titems.empty()) ce.tiled(a, grid, ctx->side[sch.stream % ctx->nstreams]);
             if (!sch.titems32.empty()) {
                 FockArgs a32 = a;
                 a32.titems = sch.d_titems32.ptr;
                 a32.sc.nitems = static_cast<long long>(sch.titems32.size());
                 const int grid32 = static_cast<int>(std::min<long long>(a32.sc.nitems, 4096));
-                ce.tiled32(a32, grid32, ctx->side[sch.stream]);
+                ce.tiled32(a32, grid32, ctx->side[sch.stream % ctx->nstreams]);
             }
         } else if (sch.mmd) {
-            ce.mmd(a, grid, ctx->side[sch.stream]);
+            ce.mmd(a, grid, ctx->side[sch.stream % ctx->nstreams]);
         } else {
-            ce.fock(a, grid, ctx->side[sch.stream]);
+            ce.fock(a, grid, ctx->side[sch.stream % ctx->nstreams]);
         }
         HIP_TRY(hipGetLastError());
         if (ctx->timing) {
-            HIP_TRY(hipEventRecord(pend.stop, ctx->side[sch.stream]));
+            HIP_TRY(hipEventRecord(pend.stop, ctx->side[sch.stream % ctx->nstreams]));
             ctx->pending.push_back(pend);
         }
     }
