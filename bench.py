@@ -131,11 +131,21 @@ def measure_builds(scr, dm, ithr, steps, warmup, taus, label=b'2jk', precision='
     return ms, times, g, sweep
 
 
-def cpu_baseline(mol, bas, dm, ithr, target_s):
-    """Oracle on all host cores over bra pairs ip % stride == 0, extrapolated."""
+def cpu_baseline(mol, bas, dm, ithr, target_s, basis_name, g_gpu):
+    """The reference's own C++ Fock path (oracle/_ref, CFockDriver::compute) on the host cores for
+    one full build, with its elementwise difference from the GPU build of the same D; where that
+    harness is not built, the oracle over every k-th bra pair, extrapolated."""
+    from oracle import refharness
+    cores = min(os.cpu_count() or 1, 16)
+    if refharness.available():
+        g_ref, info = refharness.fock(mol, basis_name, dm, '2jk', 1.0, 0.0, ithr, repeats=1, nthreads=cores)
+        return {'value': 1.0 / info['median_s'], 'unit': 'builds/s', 'cores': cores, 'kind': 'reference',
+                'sample': f'one full 2jk build of the same workload by the reference CFockDriver::compute '
+                          f'(OpenMP, {cores} threads): {info["median_s"]:.2f} s; its screener '
+                          f'{info["screener_s"]:.2f} s not included',
+                'max_abs_dG_gpu_vs_reference': float(np.max(np.abs(g_ref - g_gpu)))}
     from oracle import oracle as orc
     ob = orc.basis_from(bas, mol)
-    cores = min(os.cpu_count() or 1, 16)
     _, bins = orc.pair_maxima(ob)  # screening set-up, not timed
     # calibrate with a coarse sample, then size the sample to ~target_s
     stride = 1024
@@ -305,7 +315,7 @@ def main():
     if not args.no_tzvp:
         result['tzvp64'] = tzvp_builds(args, taus)
     if not args.no_cpu_baseline:
-        result['cpu_baseline'] = cpu_baseline(mol, bas, dm, args.ithreshold, args.cpu_seconds)
+        result['cpu_baseline'] = cpu_baseline(mol, bas, dm, args.ithreshold, args.cpu_seconds, args.basis, g)
     print(json.dumps(result))
 
 

@@ -1,0 +1,53 @@
+"""Reference C++ Fock path (oracle/_ref, built from the reference sources by
+oracle/ref.mk) as a second checker.
+
+CPU: the harness reproduces the reference's own fixtures and agrees with the
+oracle, which pins the harness itself.  GPU: the HIP build agrees with the
+reference's CFockDriver::compute elementwise on clusters the fixtures do not
+cover.  Skipped where the harness is not built.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from oracle import refharness
+from tests.common import dimer, load, reference_equal
+
+pytestmark = pytest.mark.skipif(not refharness.available(), reason='oracle/_ref/vlx_ref_fock is not built')
+
+
+def test_harness_reproduces_dimer_fixtures():
+    mol, _ = dimer('def2-svpd')
+    dm = load('h2o.dimer.svpd.density.npy')
+    j, k = load('h2o.dimer.svpd.j.npy'), load('h2o.dimer.svpd.k.npy')
+    for label, ref in (('2jk', 2.0 * j - k), ('j', j), ('k', k)):
+        g, info = refharness.fock(mol, 'def2-svpd', dm, label, 0.0, 0.0, 15)
+        assert reference_equal(g, ref), label
+    assert info['nao'] == dm.shape[0]
+
+
+def test_harness_matches_oracle_on_cluster():
+    from veloxchem_amd import MolecularBasis
+    from veloxchem_amd.workloads import pseudo_density, water_cluster
+    mol = water_cluster(3, 5.5, 9)
+    bas = MolecularBasis.read(mol, 'def2-svp')
+    d = pseudo_density(bas.get_dimensions_of_basis(), 15, seed=4)
+    g, _ = refharness.fock(mol, 'def2-svp', d, '2jk', 1.0, 0.0, 12)
+    ref, _ = orc.fock(orc.basis_from(bas, mol), d, '2jk', 1.0, 12, 0)
+    assert np.max(np.abs(g - ref)) < 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('nmol,box,seed,basis', [(4, 6.0, 7, 'def2-svp'), (3, 5.5, 3, 'def2-tzvp')])
+def test_gpu_build_matches_reference_cpu_path(nmol, box, seed, basis):
+    from veloxchem_amd import FockDriver, MolecularBasis, T4CScreener
+    from veloxchem_amd.sadguess import sad_density
+    from veloxchem_amd.workloads import water_cluster
+    mol = water_cluster(nmol, box, seed)
+    bas = MolecularBasis.read(mol, basis)
+    scr = T4CScreener()
+    scr.partition(bas, mol, 'eri')
+    d = sad_density(mol, MolecularBasis.read(mol, 'ao-start-guess'), bas)
+    g = FockDriver().compute(scr, d, '2jk', 1.0, 0.0, 12)
+    ref, _ = refharness.fock(mol, basis, d, '2jk', 1.0, 0.0, 12)
+    assert np.max(np.abs(np.asarray(g) - ref)) < 1e-10
