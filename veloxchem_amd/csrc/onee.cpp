@@ -165,6 +165,9 @@ void cart_block(const Shells& s, int sa, int sb, double* cs, double* ct, double*
         for (int j = 0; j < s.nprim[sb]; j++) {
             const double a = s.exps[s.poff[sa] + i], b = s.exps[s.poff[sb] + j];
             const double p = a + b, mu = a * b / p;
+            // every S, T and V term of the primitive pair carries e^(-mu |AB|^2): below e^-60
+            // (~1e-26) the pair adds nothing at double precision to the matrices it feeds
+            if (mu * ab2 > 60.0) continue;
             const double k = s.norms[s.poff[sa] + i] * s.norms[s.poff[sb] + j] * std::exp(-mu * ab2);
             double P[3];
             for (int x = 0; x < 3; x++) P[x] = (a * A[x] + b * B[x]) / p;
