@@ -390,6 +390,10 @@ struct MmdSizes {
     static constexpr int MT = SA / 16, NT = SC / 16, MX = HA / 16;
     static constexpr int KX = (nhcd + 3) / 4, KO = (nhab + 3) / 4;
     static constexpr int XC = MX < 2 ? MX : 2;  // X row tiles per chunk
+    // operands preloaded into registers: the X product's U_CD column when its K loop is short,
+    // and the whole U_AB operand of the out product when one chunk covers all its K steps
+    static constexpr bool PRE_X = KX <= 8;
+    static constexpr bool PRE_O = MX <= XC && KO * MT <= 16;
 };
 
 // Work item {j0, i0, i1}: bra i0 with kets [j0, i1), one shell quartet at a time.
@@ -425,6 +429,19 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
                     for (int ik = 0; ik < kp.nprim; ik++) {
                         const PrimPair pk = ket_prims[kp.prim_start + ik];
                         const double* uc = ket_u + (size_t)(kp.prim_start + ik) * (S::HC * S::SC) + 16 * nt + lr;
+                        // short K loops: both products' U operands go to registers here, so that
+                        // their memory latency overlaps the Boys function and the R levels
+                        double ucv[S::PRE_X ? S::KX : 1], uav[S::PRE_O ? S::KO : 1][S::PRE_O ? S::MT : 1];
+                        if constexpr (S::PRE_X) {
+#pragma unroll
+                            for (int kq = 0; kq < S::KX; kq++) ucv[kq] = uc[(4 * kq + lk) * S::SC];
+                        }
+                        if constexpr (S::PRE_O) {
+#pragma unroll
+                            for (int kp2 = 0; kp2 < S::KO; kp2++)
+#pragma unroll
+                                for (int ms = 0; ms < S::MT; ms++) uav[kp2][ms] = ua[(4 * kp2 + lk) * S::SA + 16 * ms + lr];
+                        }
                         const double sz = pb.zeta + pk.zeta;
                         const double rho_c = pb.zeta * pk.zeta / sz;
                         double rho = rho_c, w = vrsqrt<double>(sz);
@@ -489,13 +506,15 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
                                 prow_ok[i] = p < S::nhab;
                                 prow[i] = prow_ok[i] ? g_herm.e[p] : 0;
                             }
-#pragma unroll 1
+#pragma unroll(S::PRE_X ? S::KX : 1)
                             for (int kq = 0; kq < S::KX; kq++) {
                                 const int q = 4 * kq + lk;
                                 const bool qok = q < S::nhcd;
                                 const int tq = qok ? g_herm.e[q] : 0;
                                 const bool neg = ((tq & 0xff) + ((tq >> 8) & 0xff) + (tq >> 16)) & 1;
-                                const double bv = uc[q * S::SC];
+                                double bv;
+                                if constexpr (S::PRE_X) bv = ucv[kq];
+                                else bv = uc[q * S::SC];
 #pragma unroll
                                 for (int i = 0; i < S::XC; i++) {
                                     double av = 0.0;
@@ -513,14 +532,26 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
                             __syncthreads();
                             // out[ab][cd] += sum_p U_AB[p][ab] X[p][cd] over this chunk's rows p
                             const int k0 = 4 * pc, k1 = std::min(S::KO, 4 * (pc + S::XC));
-#pragma unroll 1
-                            for (int kp2 = k0; kp2 < k1; kp2++) {
-                                const int pr = 4 * kp2 + lk;
-                                const double bv = xt[(pr - 16 * pc) * 16 + lr];
+                            if constexpr (S::PRE_O) {
+                                // all K steps of the product are in registers (one chunk covers them)
 #pragma unroll
-                                for (int ms = 0; ms < S::MT; ms++) {
-                                    const double av = ua[pr * S::SA + 16 * ms + lr];
-                                    acc[ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[ms], 0, 0, 0);
+                                for (int kp2 = 0; kp2 < S::KO; kp2++) {
+                                    const int pr = 4 * kp2 + lk;
+                                    const double bv = xt[(pr - 16 * pc) * 16 + lr];
+#pragma unroll
+                                    for (int ms = 0; ms < S::MT; ms++)
+                                        acc[ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(uav[kp2][ms], bv, acc[ms], 0, 0, 0);
+                                }
+                            } else {
+#pragma unroll 1
+                                for (int kp2 = k0; kp2 < k1; kp2++) {
+                                    const int pr = 4 * kp2 + lk;
+                                    const double bv = xt[(pr - 16 * pc) * 16 + lr];
+#pragma unroll
+                                    for (int ms = 0; ms < S::MT; ms++) {
+                                        const double av = ua[pr * S::SA + 16 * ms + lr];
+                                        acc[ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[ms], 0, 0, 0);
+                                    }
                                 }
                             }
                             __syncthreads();
