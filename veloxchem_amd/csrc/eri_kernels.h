@@ -169,6 +169,17 @@ __device__ __forceinline__ double read_lane(double v, int l)
 // Sum over the 64 lanes of a wavefront (all lanes must be active); the
 // result is wave-uniform.  Butterfly within each row of 16 lanes by DPP
 // moves, then the four row sums are read out with readlane.
+// Sum over each row of 16 lanes (left in every lane of the row).
+template <typename R>
+__device__ __forceinline__ R row_sum(R v)
+{
+    v += dpp_move<0xB1>(v);
+    v += dpp_move<0x4E>(v);
+    v += dpp_move<0x141>(v);
+    v += dpp_move<0x140>(v);
+    return v;
+}
+
 template <typename R>
 __device__ __forceinline__ R wave_sum(R v)
 {
@@ -626,9 +637,11 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                         for (int k = 0; k < nc; k++)
 #pragma unroll
                             for (int l = 0; l < nd; l++) v = fma(I[((i * nb + jb) * nc + k) * nd + l], dcd[k][l], v);
-                        v = wave_sum(v);
-                        // one wave owns bra ib within the item: a plain store, no atomic
-                        if (lane == 0) t_jab[(bp.loc_a + i) * nB + bp.loc_b + jb] = static_cast<double>(v);
+                        // row sums by DPP; the four rows meet in the tile through LDS atomics
+                        // (the LDS pipe has room where the VALU does not)
+                        v = row_sum(v);
+                        if ((lane & 15) == 0 && v != R(0))
+                            unsafeAtomicAdd(&t_jab[(bp.loc_a + i) * nB + bp.loc_b + jb], static_cast<double>(v));
                     }
                 if (jcd_in_regs || active) {
                     R vcd[nc][nd];
