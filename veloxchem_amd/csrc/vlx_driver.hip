@@ -181,6 +181,8 @@ struct vlx_context {
     // primitives of one ket pair per lane of the tiled kernels (VLX_KET_PRIMS, 1..16): a ket
     // pair of n primitives takes ceil(n / ket_prims) lanes with near-equal shares
     int ket_prims = 3;
+    // stream balance by the autotuner's measured class times (VLX_BALANCE=1) or by estimated FLOPs (0)
+    int balance_measured = 0;
     hipStream_t stream = nullptr;
     hipStream_t side[NSIDE] = {};
     hipEvent_t ev_start = nullptr;
@@ -230,6 +232,7 @@ struct vlx_screener {
     std::vector<PairRef> pair_ref;  // nshells x nshells -> position in all[]
     std::map<int, std::vector<std::unique_ptr<ClassSchedule>>> schedules;  // by (ithreshold, FP32 bin-sum threshold)
     std::map<int, int> variant;  // class entry -> VARIANT_* chosen by vlx_fock_autotune
+    std::map<int, double> tuned_ms;  // class entry -> its measured time (ms) under that variant
 };
 
 namespace {
@@ -682,11 +685,17 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
         }
         list.push_back(std::move(sch));
     }
-    // spread classes over the side streams, most expensive first (estimated FLOPs)
+    // spread classes over the side streams, most expensive first: by the autotuner's measured
+    // times when every class has one (the kernel families differ ~5x in FLOP rate), else by
+    // estimated FLOPs
+    bool measured = s->ctx->balance_measured && !s->tuned_ms.empty();
+    for (auto& sp : list) measured = measured && s->tuned_ms.count(sp->entry);
     std::vector<std::pair<double, ClassSchedule*>> cost;
     for (auto& sp : list) {
         const auto& ce = g_class_table[sp->entry];
-        cost.push_back({(double)sp->prim_quartets * (ce.vrr_flops + 60) + (double)sp->quartets * ce.hrr_flops, sp.get()});
+        const double c = measured ? s->tuned_ms[sp->entry]
+                                  : (double)sp->prim_quartets * (ce.vrr_flops + 60) + (double)sp->quartets * ce.hrr_flops;
+        cost.push_back({c, sp.get()});
     }
     std::sort(cost.begin(), cost.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
     double load[NSIDE] = {0.0};
@@ -738,6 +747,7 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_GROUP_ATOMS")) ctx->group_atoms = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("VLX_TILED_WAVES")) ctx->tiled_waves = std::atoi(e);
     if (const char* e = std::getenv("VLX_KET_PRIMS")) ctx->ket_prims = std::max(1, std::min(16, std::atoi(e)));
+    if (const char* e = std::getenv("VLX_BALANCE")) ctx->balance_measured = std::atoi(e) != 0;
     if (const char* e = std::getenv("VLX_VARIANT")) ctx->force_variant = std::max(-1, std::min(NVARIANTS - 1, std::atoi(e)));
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -1228,6 +1238,7 @@ int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_densi
     scr->schedules.clear();
     if (rc) return rc;
     scr->variant = best_var;
+    scr->tuned_ms = best_ms;
     for (const auto& kv : best_var) ctx->tuned_variant[kv.first] = kv.second;
     if (nchosen) *nchosen = static_cast<int>(best_var.size());
     int r = 0;
