@@ -138,6 +138,12 @@ int vlx_context_synchronize(vlx_context* ctx);
  * each in isolation. */
 int vlx_context_set_streams(vlx_context* ctx, int nstreams);
 
+/* Kernel variant every class of later builds runs (no reference counterpart;
+ * diagnostics and tests): the autotuner's numbering, 0..5, where a class that
+ * cannot run it falls back to its untiled kernel; -1 restores the per-class
+ * choice (autotuned, else the default).  Results do not depend on it. */
+int vlx_context_set_variant(vlx_context* ctx, int variant);
+
 /* Raw HIP stream of the context (hipStream_t), for event timing by callers. */
 void* vlx_context_stream(vlx_context* ctx);
 
@@ -165,7 +171,8 @@ int vlx_fock_schedule_stats(vlx_screener* scr, int ithreshold, int max_records, 
  * 2jk build at ithreshold on `d_density` under each kernel variant
  * (0 = untiled, 1 = LDS-tiled with 4-wave blocks, 2 = LDS-tiled with the
  * class's wide block, 3 = matrix-core McMurchie-Davidson, for classes with
- * la+lb+lc+ld >= 4), `repeats` builds each on a single stream, and keeps
+ * la+lb+lc+ld >= 4, 4 and 5 = variants 1 and 2 with one ket primitive per lane
+ * instead of runs of up to three), `repeats` builds each on a single stream, and keeps
  * the fastest variant per class for later builds with this screener (and as
  * the default of screeners created later on this context).
  * Optional outputs: *nchosen records of classes (4 ints each) and variants

@@ -268,6 +268,28 @@ def test_erf_j_vs_oracle():
     assert np.max(np.abs(np.asarray(g) - ref)) < TOL
 
 
+def test_every_variant_matches_oracle():
+    """Every kernel variant forced for all classes (untiled, tiled 4-wave and wide blocks over
+    primitive runs and over single primitives, matrix-core MD) gives the oracle's 2jk at 1e-10."""
+    from veloxchem_amd import _native
+    mol = water_cluster(4, 6.0, 7)
+    bas = MolecularBasis.read(mol, 'def2-svp')
+    scr = _screened(mol, bas)
+    D = pseudo_density(bas.get_dimensions_of_basis(), 20, seed=11)
+    ref, _ = orc.fock(orc.basis_from(bas, mol), D, '2jk', 1.0, 12, 0)
+    drv = FockDriver()
+    ctx = _native.context()
+    try:
+        for v, name in enumerate(FockDriver.VARIANTS):
+            _native.check(_native.lib().vlx_context_set_variant(ctx, v))
+            g = np.asarray(drv.compute(scr, D, '2jk', 1.0, 0.0, 12))
+            err = np.max(np.abs(g - ref))
+            print(f'variant {name}: max|dG| {err:.2e}', flush=True)
+            assert err < TOL, name
+    finally:
+        _native.check(_native.lib().vlx_context_set_variant(ctx, -1))
+
+
 def test_autotune_choices_keep_parity():
     """The autotuner times every kernel variant per class; the build it leaves behind still matches the oracle."""
     mol = water_cluster(4, 6.0, 7)

@@ -51,6 +51,7 @@ def lib():
         L.vlx_context_stream.argtypes = [ctypes.c_void_p]
         L.vlx_context_synchronize.argtypes = [ctypes.c_void_p]
         L.vlx_context_set_streams.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.vlx_context_set_variant.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.vlx_set_block_size_factor.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.vlx_screener_create.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_double_p,
                                           _c_double_p, _c_double_p, _c_int_p, _c_int_p, ctypes.c_int,

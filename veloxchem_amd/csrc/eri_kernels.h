@@ -18,10 +18,6 @@
 
 #include "vlx_types.h"
 
-constexpr int VLX_WAVE = 64;
-constexpr int VLX_BLOCK = 256;
-constexpr int VLX_WAVES_PER_BLOCK = VLX_BLOCK / VLX_WAVE;
-
 // Arithmetic helpers of the generated recursions, per real type.  The FP64
 // forms are plain IEEE operations; the FP32 forms use the hardware
 // reciprocal and exponential (1 ulp), which the mixed-precision path accepts

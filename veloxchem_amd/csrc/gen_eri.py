@@ -316,8 +316,8 @@ def mmd_eligible(la, lb, lc, ld, var):
 
 
 def tiled_eligible(la, lb, lc, ld):
-    """Classes that also get the LDS-tiled Fock kernel.  L <= 4 use it by
-    default; for L = 5 it is built and chosen at run time (VLX_TILED_LMAX)."""
+    """Classes that also get the LDS-tiled Fock kernel (L <= 5; VLX_TILED_LMAX
+    lowers the limit at run time)."""
     return la + lb + lc + ld <= 5
 
 

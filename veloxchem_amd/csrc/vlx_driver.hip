@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <queue>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -121,9 +122,14 @@ struct PairSet {
         int cnt_le[NBINS];  // pairs of the segment with bin <= b
     };
     std::vector<Segment> segs;  // tiled order only: runs of equal (group A, group B)
-    std::vector<int2> kentries;           // tiled ket order: (pair, first primitive | count << 16) entries
-    std::vector<int> seg_e0, seg_e1;      // entry range of each segment
-    DevBuf<int2> d_kentries;
+    // tiled ket order: lane entries (pair, first primitive | count << 16) and the entry range
+    // of each segment, for runs of up to ket_prims primitives and for single primitives
+    struct Entries {
+        std::vector<int2> kentries;
+        std::vector<int> seg_e0, seg_e1;
+        DevBuf<int2> d_kentries;
+    };
+    Entries runs, singles;
     DevBuf<PairInfo> d_pairs;
     DevBuf<PrimPair> d_prims;
     DevBuf<double> d_u;   // matrix-core MD kernel: U of every primitive pair (built on first use)
@@ -137,6 +143,7 @@ struct ClassSchedule {
     int stream = 0;  // side stream the class runs on
     int variant = 0;      // VARIANT_* the schedule runs
     bool tiled = false;
+    bool singles = false; // tiled over single-primitive ket entries
     bool mmd = false;     // matrix-core MD kernel (MD work items)
     int tiled_waves = 0;  // 4: 4-wave blocks; 0: the class's wide block
     int tile_lds = 0;  // bytes of dynamic LDS (tiled)
@@ -157,9 +164,22 @@ constexpr int NSIDE = 4;  // side streams that run class kernels concurrently
 constexpr int NO_FP32 = 99;  // bin-sum threshold that sends no quartet to FP32
 
 // Kernel variant of a class: untiled (generated ket-/bra-stationary kernel),
-// tiled with 4-wave blocks, tiled with the class's wide block, matrix-core MD.
-// Classes without generated recursions run the matrix-core MD kernel only.
-enum { VARIANT_UNTILED = 0, VARIANT_TILED4 = 1, VARIANT_TILED_WIDE = 2, VARIANT_MMD = 3, NVARIANTS = 4 };
+// tiled with 4-wave blocks, tiled with the class's wide block, matrix-core MD,
+// and the two tiled forms over single-primitive ket entries.  Classes without
+// generated recursions run the matrix-core MD kernel only.
+enum {
+    VARIANT_UNTILED = 0,
+    VARIANT_TILED4 = 1,
+    VARIANT_TILED_WIDE = 2,
+    VARIANT_MMD = 3,
+    VARIANT_TILED4_P1 = 4,
+    VARIANT_TILED_WIDE_P1 = 5,
+    NVARIANTS = 6
+};
+bool variant_tiled(int v)
+{
+    return v == VARIANT_TILED4 || v == VARIANT_TILED_WIDE || v == VARIANT_TILED4_P1 || v == VARIANT_TILED_WIDE_P1;
+}
 
 struct vlx_context {
     int device = 0;
@@ -167,10 +187,7 @@ struct vlx_context {
     int max_chunk = BRA_CHUNK;      // bras per work item upper bound (VLX_BRA_CHUNK)
     int min_items = 16384;          // work items a class launch aims for (VLX_MIN_ITEMS)
     int use_tiled = 1;              // LDS-tiled kernels where built (VLX_TILED)
-    // largest la+lb+lc+ld that may use the tiled kernel (VLX_TILED_LMAX).  The L = 5 tiled
-    // kernels are built but off by default: with four concurrent streams they left non-finite
-    // J entries in the 32-water def2-SVP build (not with one stream); cause not found.
-    int tiled_lmax = 4;
+    int tiled_lmax = 5;             // largest la+lb+lc+ld that may use the tiled kernel (VLX_TILED_LMAX)
     int force_variant = -1;         // autotuning: every class runs this variant (VARIANT_*)
     std::map<int, int> tuned_variant;  // class entry -> VARIANT_* of the last autotune (default for new screeners)
     // atoms per group of the tiled kernels (VLX_GROUP_ATOMS); 0: the largest of 12, 8, 6, 4, 3,
@@ -183,6 +200,9 @@ struct vlx_context {
     int ket_prims = 3;
     // stream balance by the autotuner's measured class times (VLX_BALANCE=1) or by estimated FLOPs (0)
     int balance_measured = 0;
+    // untiled items reordered so each wave's contiguous range has equal estimated cost
+    // (VLX_BALANCE_ITEMS=0: the plain build order)
+    int balance_items = 1;
     hipStream_t stream = nullptr;
     hipStream_t side[NSIDE] = {};
     hipEvent_t ev_start = nullptr;
@@ -287,6 +307,41 @@ void add_pair(const vlx_screener& s, PairSet& ps, int sa, int sb)
     build_prims(s, sa, sb, ps.prims);
 }
 
+// Wavefronts of an untiled class launch (fock_kernel, fock_kernel_bs): at most 8192, in blocks
+// of VLX_WAVES_PER_BLOCK; wave g takes the contiguous items [n g / nw, n (g + 1) / nw).
+long long untiled_waves(long long nitems)
+{
+    const long long waves = std::min<long long>(nitems, 8192);
+    return (waves + VLX_WAVES_PER_BLOCK - 1) / VLX_WAVES_PER_BLOCK * VLX_WAVES_PER_BLOCK;
+}
+
+// Reorders work items so that the kernel's equal-count contiguous split over nw waves is
+// balanced by estimated cost: longest item first onto the least-loaded wave that still has a
+// free slot (greedy LPT with slot counts fixed by the split).  The set of items is unchanged.
+void balance_items(std::vector<WorkItem>& items, const std::vector<double>& cost, long long nw)
+{
+    const long long n = static_cast<long long>(items.size());
+    if (n <= nw) return;
+    std::vector<int> order(n);
+    for (long long i = 0; i < n; i++) order[i] = static_cast<int>(i);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    using Slot = std::pair<double, long long>;  // (load, wave)
+    std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
+    std::vector<long long> fill(nw, 0), cap(nw);
+    for (long long g = 0; g < nw; g++) {
+        cap[g] = n * (g + 1) / nw - n * g / nw;
+        if (cap[g] > 0) heap.push({0.0, g});
+    }
+    std::vector<WorkItem> out(n);
+    for (int idx : order) {
+        const auto [load, g] = heap.top();
+        heap.pop();
+        out[n * g / nw + fill[g]++] = items[idx];
+        if (fill[g] < cap[g]) heap.push({load + cost[idx], g});
+    }
+    items.swap(out);
+}
+
 int pair_bin(double maxval)
 {
     // src/orbdata/BlockedGtoPairBlock.cpp:32-42
@@ -323,7 +378,7 @@ void reorder_set(const vlx_screener& s, const PairSet& src, const std::vector<in
 
 // Tiled orders of `src`, both by (group A, group B, bin, nprim): the bras a
 // ket chunk's bins allow are a prefix of each segment.  Kets are expanded to
-// lane entries of at most ket_prims consecutive primitives of one pair (the
+// lane entries of at most run_cap consecutive primitives of one pair (the
 // contraction over the rest of the pair's primitives happens in the LDS
 // tiles).  Segments are the runs of equal (group A, group B).
 void build_tiled(const vlx_screener& s, const PairSet& src, PairSet& ket, PairSet& bra)
@@ -352,19 +407,6 @@ void build_tiled(const vlx_screener& s, const PairSet& src, PairSet& ket, PairSe
     std::stable_sort(border.begin(), border.end(), by(false));
     reorder_set(s, src, border, bra);
     for (int k = 0; k < bra.size(); k++) bra.pairs[k].rank = rank_of[border[k]];
-    ket.kentries.clear();
-    const int pmax = s.ctx->ket_prims;
-    std::vector<long long> eoff(ket.size() + 1, 0);
-    for (int k = 0; k < ket.size(); k++) {
-        const int np = ket.pairs[k].nprim;
-        const int m = (np + pmax - 1) / pmax;
-        for (int t = 0, q = 0; t < m; t++) {
-            const int cnt = np / m + (t < np % m ? 1 : 0);
-            ket.kentries.push_back(make_int2(k, q | (cnt << 16)));
-            q += cnt;
-        }
-        eoff[k + 1] = eoff[k] + m;
-    }
     for (PairSet* ps : {&ket, &bra}) {
         ps->segs.clear();
         for (int k = 0; k < ps->size(); k++) {
@@ -381,21 +423,41 @@ void build_tiled(const vlx_screener& s, const PairSet& src, PairSet& ket, PairSe
             for (int b = std::max(pk.bin, 0); b < NBINS; b++) sg.cnt_le[b]++;
         }
     }
-    // entry ranges of the ket segments; inside a segment the entries go by (bin, run length
-    // descending), so that a 64-entry chunk holds runs of equal length (its lanes loop to the
-    // chunk's longest run)
-    ket.seg_e0.clear();
-    ket.seg_e1.clear();
-    for (const auto& sg : ket.segs) {
-        const int e0 = static_cast<int>(eoff[sg.start]), e1 = static_cast<int>(eoff[sg.end]);
-        ket.seg_e0.push_back(e0);
-        ket.seg_e1.push_back(e1);
-        std::stable_sort(ket.kentries.begin() + e0, ket.kentries.begin() + e1, [&](const int2& x, const int2& y) {
-            const int bx = ket.pairs[x.x].bin, by = ket.pairs[y.x].bin;
-            if (bx != by) return bx < by;
-            return (x.y >> 16) > (y.y >> 16);
-        });
-    }
+    // lane entries for a run cap; inside a segment the entries go by (bin, run length
+    // descending), so that a 64-entry chunk holds runs of equal length where it can (its
+    // lanes loop to the chunk's longest run)
+    auto make_entries = [&](int pmax, PairSet::Entries& out) {
+        out.kentries.clear();
+        out.seg_e0.clear();
+        out.seg_e1.clear();
+        std::vector<long long> eoff(ket.size() + 1, 0);
+        for (int k = 0; k < ket.size(); k++) {
+            const int np = ket.pairs[k].nprim;
+            const int m = (np + pmax - 1) / pmax;
+            for (int t = 0, q = 0; t < m; t++) {
+                const int cnt = np / m + (t < np % m ? 1 : 0);
+                out.kentries.push_back(make_int2(k, q | (cnt << 16)));
+                q += cnt;
+            }
+            eoff[k + 1] = eoff[k] + m;
+        }
+        for (const auto& sg : ket.segs) {
+            const int e0 = static_cast<int>(eoff[sg.start]), e1 = static_cast<int>(eoff[sg.end]);
+            out.seg_e0.push_back(e0);
+            out.seg_e1.push_back(e1);
+            std::stable_sort(out.kentries.begin() + e0, out.kentries.begin() + e1,
+                             [&](const int2& x, const int2& y) {
+                                 const int bx = ket.pairs[x.x].bin, by = ket.pairs[y.x].bin;
+                                 if (bx != by) return bx < by;
+                                 return (x.y >> 16) > (y.y >> 16);
+                             });
+        }
+    };
+    // runs of up to ket_prims primitives (they share a pair's contracted work: HRR,
+    // digestion), and single primitives (chunks of pairs with mixed primitive counts, such
+    // as the 1-, 3- and 5-primitive d-s pairs of def2 bases, then fill every lane)
+    make_entries(s.ctx->ket_prims, ket.runs);
+    make_entries(1, ket.singles);
 }
 
 // U arrays of a pair set for the matrix-core MD Fock kernel (once per set).
@@ -415,7 +477,8 @@ int upload_set(PairSet& ps)
 {
     HIP_TRY(ps.d_pairs.upload(ps.pairs));
     HIP_TRY(ps.d_prims.upload(ps.prims));
-    if (!ps.kentries.empty()) HIP_TRY(ps.d_kentries.upload(ps.kentries));
+    for (PairSet::Entries* en : {&ps.runs, &ps.singles})
+        if (!en->kentries.empty()) HIP_TRY(en->d_kentries.upload(en->kentries));
     return VLX_OK;
 }
 
@@ -498,7 +561,7 @@ __global__ void split_kernel(const double* __restrict__ d, double* __restrict__ 
 // Schwarz bound < 10^(2 - k32)); the rest stay FP64.
 int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_ptr<ClassSchedule>>** out)
 {
-    const int key = (ithr + 1) * 128 + k32;
+    const int key = ((ithr + 1) * 128 + k32) * (NVARIANTS + 1) + (s->ctx->force_variant + 1);
     auto it = s->schedules.find(key);
     if (it != s->schedules.end()) {
         *out = &it->second;
@@ -548,26 +611,28 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
             if (vit != s->variant.end()) var = vit->second;
             else if (cit != s->ctx->tuned_variant.end()) var = cit->second;
         }
-        if ((var == VARIANT_TILED4 || var == VARIANT_TILED_WIDE) && !can_tile) var = VARIANT_UNTILED;
+        if (variant_tiled(var) && !can_tile) var = VARIANT_UNTILED;
         if (var == VARIANT_MMD && !ce.mmd) var = VARIANT_UNTILED;
         if (ce.variant == VLX_WAVE_MD) var = VARIANT_MMD;  // no generated kernel
         sch->variant = var;
         sch->mmd = var == VARIANT_MMD;
-        if (var == VARIANT_TILED4 || var == VARIANT_TILED_WIDE) {
+        if (variant_tiled(var)) {
             // tiled items: (bra segment prefix, chunk of <= 64 kets of a ket segment)
             const PairSet& tb = unscreened ? s->all_tiled_bra[ce.x] : s->kept_tiled_bra[ce.x];
             const PairSet& tk = unscreened ? s->all_tiled[ce.y] : s->kept_tiled[ce.y];
             sch->tiled = true;
-            sch->tiled_waves = (var == VARIANT_TILED4) ? 4 : 0;
+            sch->singles = var == VARIANT_TILED4_P1 || var == VARIANT_TILED_WIDE_P1;
+            const PairSet::Entries& en = sch->singles ? tk.singles : tk.runs;
+            sch->tiled_waves = (var == VARIANT_TILED4 || var == VARIANT_TILED4_P1) ? 4 : 0;
             sch->tile_lds = tile_lds;
             for (size_t sk = 0; sk < tk.segs.size(); sk++) {
                 const auto& ks = tk.segs[sk];
-                for (int j0 = tk.seg_e0[sk]; j0 < tk.seg_e1[sk]; j0 += 64) {
-                    const int j1 = std::min(tk.seg_e1[sk], j0 + 64);
-                    int bin_lo = NBINS, kmin = tk.kentries[j0].x;
+                for (int j0 = en.seg_e0[sk]; j0 < en.seg_e1[sk]; j0 += 64) {
+                    const int j1 = std::min(en.seg_e1[sk], j0 + 64);
+                    int bin_lo = NBINS, kmin = en.kentries[j0].x;
                     for (int j = j0; j < j1; j++) {
-                        bin_lo = std::min(bin_lo, std::max(tk.pairs[tk.kentries[j].x].bin, 0));
-                        kmin = std::min(kmin, tk.kentries[j].x);
+                        bin_lo = std::min(bin_lo, std::max(tk.pairs[en.kentries[j].x].bin, 0));
+                        kmin = std::min(kmin, en.kentries[j].x);
                     }
                     const int lim = unscreened ? NBINS - 1 : ithr - bin_lo;
                     if (lim < 0) continue;
@@ -640,9 +705,10 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
             const PairSet& tb = unscreened ? s->all_tiled_bra[ce.x] : s->kept_tiled_bra[ce.x];
             const PairSet& tk = unscreened ? s->all_tiled[ce.y] : s->kept_tiled[ce.y];
             // a chunk's lanes loop over the largest primitive count among its entries
+            const PairSet::Entries& en = sch->singles ? tk.singles : tk.runs;
             auto kmax = [&](const TileItem& ti) {
                 int m = 1;
-                for (int j = ti.j0; j < ti.j1; j++) m = std::max(m, tk.kentries[j].y >> 16);
+                for (int j = ti.j0; j < ti.j1; j++) m = std::max(m, en.kentries[j].y >> 16);
                 return static_cast<long long>(m);
             };
             for (const auto& ti : sch->titems) {
@@ -654,19 +720,32 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
                 sch->prim_slots32 += (tb.nprim_prefix[ti.i1] - tb.nprim_prefix[ti.i0]) * 64 * kmax(ti);
             }
             sch->prim_slots += sch->prim_slots32;
-        } else if (!sch->mmd && ce.variant == VLX_KET_STATIONARY) {
+        } else if (!sch->mmd) {
+            // estimated cost of an item in FLOPs of one lane: primitive iterations times the
+            // per-primitive work, plus the contracted work per quartet; the items are then
+            // spread so that every wave's contiguous range costs about the same
+            const int nsph = (2 * ce.la + 1) * (2 * ce.lb + 1) * (2 * ce.lc + 1) * (2 * ce.ld + 1);
+            const double pcost = ce.vrr_flops + 60.0, qcost = ce.hrr_flops + 12.0 * nsph;
+            std::vector<double> cost;
+            cost.reserve(sch->items.size());
+            const bool ks = ce.variant == VLX_KET_STATIONARY;
             for (const auto& wi : sch->items) {
-                int kmax = 0;
-                for (int j = wi.j0; j < std::min(wi.j0 + 64, ket.size()); j++) kmax = std::max(kmax, ket.pairs[j].nprim);
-                sch->lane_slots += 64LL * (wi.i1 - wi.i0);
-                sch->prim_slots += (long long)kmax * (bra.nprim_prefix[wi.i1] - bra.nprim_prefix[wi.i0]) * 64;
-            }
-        } else {
-            for (const auto& wi : sch->items) {
-                const int j1 = sch->mmd ? wi.i1 : std::min(wi.j0 + 64, wi.i1);
+                const int j1 = ks ? std::min(wi.j0 + 64, ket.size()) : std::min(wi.j0 + 64, wi.i1);
                 int kmax = 0;
                 for (int j = wi.j0; j < j1; j++) kmax = std::max(kmax, ket.pairs[j].nprim);
-                const int lanes = sch->mmd ? (j1 - wi.j0) : 64;
+                const long long bprims = ks ? bra.nprim_prefix[wi.i1] - bra.nprim_prefix[wi.i0] : bra.pairs[wi.i0].nprim;
+                const long long nbra = ks ? wi.i1 - wi.i0 : 1;
+                sch->lane_slots += 64LL * nbra;
+                sch->prim_slots += (long long)kmax * bprims * 64;
+                cost.push_back(kmax * bprims * pcost + nbra * qcost);
+            }
+            if (s->ctx->balance_items)
+                balance_items(sch->items, cost, untiled_waves(static_cast<long long>(sch->items.size())));
+        } else {
+            for (const auto& wi : sch->items) {  // matrix-core MD: one ket per wavefront pass
+                int kmax = 0;
+                for (int j = wi.j0; j < wi.i1; j++) kmax = std::max(kmax, ket.pairs[j].nprim);
+                const int lanes = wi.i1 - wi.j0;
                 sch->lane_slots += lanes;
                 sch->prim_slots += (long long)kmax * bra.pairs[wi.i0].nprim * lanes;
             }
@@ -748,6 +827,7 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_TILED_WAVES")) ctx->tiled_waves = std::atoi(e);
     if (const char* e = std::getenv("VLX_KET_PRIMS")) ctx->ket_prims = std::max(1, std::min(16, std::atoi(e)));
     if (const char* e = std::getenv("VLX_BALANCE")) ctx->balance_measured = std::atoi(e) != 0;
+    if (const char* e = std::getenv("VLX_BALANCE_ITEMS")) ctx->balance_items = std::atoi(e) != 0;
     if (const char* e = std::getenv("VLX_VARIANT")) ctx->force_variant = std::max(-1, std::min(NVARIANTS - 1, std::atoi(e)));
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -1122,6 +1202,14 @@ int vlx_context_set_streams(vlx_context* ctx, int nstreams)
     return VLX_OK;
 }
 
+int vlx_context_set_variant(vlx_context* ctx, int variant)
+{
+    if (!ctx) return fail(VLX_ERR_ARG, "NULL context");
+    if (variant < -1 || variant >= NVARIANTS) return fail(VLX_ERR_ARG, "variant %d outside -1..%d", variant, NVARIANTS - 1);
+    ctx->force_variant = variant;
+    return VLX_OK;
+}
+
 int vlx_context_set_timing(vlx_context* ctx, int enable)
 {
     if (!ctx) return fail(VLX_ERR_ARG, "context is NULL");
@@ -1374,7 +1462,7 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
         a.D = d_density;
         a.F = ctx->d_faccum.ptr;
         a.titems = sch.d_titems.ptr;
-        a.kentries = ket.d_kentries.ptr;
+        a.kentries = sch.singles ? ket.singles.d_kentries.ptr : ket.runs.d_kentries.ptr;
         a.grp_off = scr->d_grp_off.ptr;
         a.grp_n = scr->d_grp_n.ptr;
         a.grp_ao = scr->d_grp_ao.ptr;
@@ -1388,9 +1476,8 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
         a.sc.fj = fj;
         a.sc.fk = fk;
         a.sc.omega2 = omega2;
-        const long long waves = std::min<long long>(a.sc.nitems, 8192);
         const int grid = sch.tiled ? static_cast<int>(std::min<long long>(a.sc.nitems, 4096))
-                                   : static_cast<int>((waves + 3) / 4);
+                                   : static_cast<int>(untiled_waves(a.sc.nitems) / VLX_WAVES_PER_BLOCK);
         vlx_context::Pending pend{sch.entry, nullptr, nullptr};
         if (ctx->timing) {
             for (hipEvent_t* ev : {&pend.start, &pend.stop}) {

@@ -11,6 +11,11 @@
 
 #include "vlx_boys.h"
 
+// Wavefront width and the block shape of the untiled Fock kernels (4 waves).
+constexpr int VLX_WAVE = 64;
+constexpr int VLX_BLOCK = 256;
+constexpr int VLX_WAVES_PER_BLOCK = VLX_BLOCK / VLX_WAVE;
+
 // One primitive pair (a on centre A, b on centre B), A = first shell of the pair.
 struct alignas(64) PrimPair {
     double zeta;          // a + b
