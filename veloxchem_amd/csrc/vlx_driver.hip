@@ -203,6 +203,10 @@ struct vlx_context {
     // untiled items reordered so each wave's contiguous range has equal estimated cost
     // (VLX_BALANCE_ITEMS=0: the plain build order)
     int balance_items = 1;
+    // order of equal-(bin, nprim) kets in the untiled lists (VLX_KET_ORDER): 2, along diagonals
+    // of equal C - D offset, gives the lanes of a wave distinct and mostly consecutive F' targets
+    // for their exchange atomics (shell order, 0, puts lanes that share C on one address)
+    int ket_order = 2;
     hipStream_t stream = nullptr;
     hipStream_t side[NSIDE] = {};
     hipEvent_t ev_start = nullptr;
@@ -828,6 +832,7 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_KET_PRIMS")) ctx->ket_prims = std::max(1, std::min(16, std::atoi(e)));
     if (const char* e = std::getenv("VLX_BALANCE")) ctx->balance_measured = std::atoi(e) != 0;
     if (const char* e = std::getenv("VLX_BALANCE_ITEMS")) ctx->balance_items = std::atoi(e) != 0;
+    if (const char* e = std::getenv("VLX_KET_ORDER")) ctx->ket_order = std::max(0, std::min(2, std::atoi(e)));
     if (const char* e = std::getenv("VLX_VARIANT")) ctx->force_variant = std::max(-1, std::min(NVARIANTS - 1, std::atoi(e)));
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -1116,13 +1121,24 @@ int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* 
             ps.bins[i] = pair_bin(mx[i]);
             ps.pairs[i].bin = ps.bins[i];
         }
-        // kept set (ket order): bin >= 0, sorted by (bin, nprim)
+        // kept set (ket order): bin >= 0, sorted by (bin, nprim); inside equal (bin, nprim)
+        // the pairs go in shell order (ket_order 0), scattered (1) or along diagonals of equal
+        // (C - D) AO offset (2)
         std::vector<int> order;
         for (int i = 0; i < ps.size(); i++)
             if (ps.bins[i] >= 0) order.push_back(i);
+        const int ko = ctx->ket_order;
+        auto mix = [](int i) { return static_cast<unsigned>(i) * 2654435761u; };
         std::stable_sort(order.begin(), order.end(), [&](int i, int j) {
+            const PairInfo &x = ps.pairs[i], &y = ps.pairs[j];
             if (ps.bins[i] != ps.bins[j]) return ps.bins[i] < ps.bins[j];
-            return ps.pairs[i].nprim < ps.pairs[j].nprim;
+            if (x.nprim != y.nprim) return x.nprim < y.nprim;
+            if (ko == 1) return mix(i) < mix(j);
+            if (ko == 2) {
+                if (x.ao_a - x.ao_b != y.ao_a - y.ao_b) return x.ao_a - x.ao_b < y.ao_a - y.ao_b;
+                return x.ao_a < y.ao_a;
+            }
+            return false;
         });
         PairSet& ks = s->kept[x];
         reorder_set(*s, ps, order, ks);
