@@ -177,6 +177,74 @@ def tzvp_builds(args, taus):
             'shell_quartets': int(sum(w[1] for w in work)), 'mixed_sweep': sweep}
 
 
+def batched_clusters(args):
+    """Batched uneven builds: 256 water clusters of 1-12 molecules (sizes drawn with seed 4,
+    cluster i with seed 4000 + i, box scaled to the 32-water density), def2-SVP, SAD densities,
+    resident in HBM.  Aggregate builds/s of one batched call (each molecule whole on one stream,
+    molecules concurrent) against the same molecules built one after another."""
+    import ctypes
+    from veloxchem_amd import MolecularBasis, T4CScreener, _native
+    from veloxchem_amd.sadguess import sad_density
+    from veloxchem_amd.workloads import water_cluster
+    lib = _native.lib()
+    ctx = _native.context(0)
+    sizes = np.random.Generator(np.random.PCG64(4)).integers(1, 13, args.batch_clusters)
+    scrs, d_dm, d_g, naos = [], [], [], []
+    t0 = time.perf_counter()
+    for i, nw in enumerate(sizes):
+        mol = water_cluster(int(nw), 10.0 * (nw / 32.0) ** (1.0 / 3.0), 4000 + i)
+        bas = MolecularBasis.read(mol, 'def2-svp')
+        scr = T4CScreener()
+        scr.partition(bas, mol, 'eri')
+        dm = sad_density(mol, MolecularBasis.read(mol, 'ao-start-guess'), bas)
+        n = bas.get_dimensions_of_basis()
+        a, b = _native.DeviceArray((n, n)), _native.DeviceArray((n, n))
+        a.upload(dm)
+        scrs.append(scr)
+        d_dm.append(a)
+        d_g.append(b)
+        naos.append(n)
+    t_setup = time.perf_counter() - t0
+    nmol = len(scrs)
+    arr = ctypes.c_void_p * nmol
+    h_scr = arr(*[s.handle.value for s in scrs])
+    h_dm = arr(*[a.ptr.value for a in d_dm])
+    h_g = arr(*[b.ptr.value for b in d_g])
+
+    def one_by_one():
+        for s_, a, b in zip(scrs, d_dm, d_g):
+            _native.check(lib.vlx_fock_compute_device(ctx, s_.handle, a.ptr, b'2jk', 1.0, 0.0, args.ithreshold, b.ptr))
+
+    def batched():
+        _native.check(lib.vlx_fock_compute_batch_device(ctx, nmol, h_scr, h_dm, b'2jk', 1.0, 0.0, args.ithreshold, h_g))
+
+    def timed(fn, reps):
+        fn()
+        _native.check(lib.vlx_context_synchronize(ctx))
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        _native.check(lib.vlx_context_synchronize(ctx))
+        return (time.perf_counter() - t) / reps
+
+    t_seq = timed(one_by_one, args.batch_reps)
+    g_seq = [b.download() for b in d_g]
+    t_bat = timed(batched, args.batch_reps)
+    g_bat = [b.download() for b in d_g]
+    diff = max(float(np.max(np.abs(x - y))) for x, y in zip(g_seq, g_bat))
+    quartets = sum(sum(w[1] for w in s_.work(args.ithreshold)) for s_ in scrs)
+    for a, b in zip(d_dm, d_g):
+        a.free()
+        b.free()
+    return {'workload': f'{nmol} water clusters of 1-12 molecules (sizes seed 4, cluster seeds 4000+i, '
+                        f'box 10 A x (n/32)^(1/3)), def2-SVP, {min(naos)}-{max(naos)} AOs, '
+                        f'{sum(naos)} AOs in all, 2jk on SAD densities, ithreshold {args.ithreshold}',
+            'batched_builds_per_s': nmol / t_bat, 'batched_ms_per_batch': t_bat * 1e3,
+            'one_by_one_builds_per_s': nmol / t_seq, 'one_by_one_ms_per_batch': t_seq * 1e3,
+            'speedup': t_seq / t_bat, 'max_abs_dG_batched_vs_one_by_one': diff,
+            'shell_quartets': int(quartets), 'reps': args.batch_reps, 'setup_s': round(t_setup, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -201,6 +269,10 @@ def main():
     ap.add_argument('--no-tzvp', action='store_true',
                     help='skip the 64-water def2-TZVP FP64 build and mixed-precision sweep')
     ap.add_argument('--tzvp-steps', type=int, default=2)
+    ap.add_argument('--no-batch', action='store_true',
+                    help='skip the batched workload of 256 small water clusters')
+    ap.add_argument('--batch-clusters', type=int, default=256)
+    ap.add_argument('--batch-reps', type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU; the multi-GPU sharding is design only '
@@ -314,6 +386,8 @@ def main():
                 result[key] = mixed
     if not args.no_tzvp:
         result['tzvp64'] = tzvp_builds(args, taus)
+    if not args.no_batch:
+        result['batched_clusters'] = batched_clusters(args)
     if not args.no_cpu_baseline:
         result['cpu_baseline'] = cpu_baseline(mol, bas, dm, args.ithreshold, args.cpu_seconds, args.basis, g)
     print(json.dumps(result))

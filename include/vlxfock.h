@@ -107,6 +107,21 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
                             double exchange_factor, double omega, int ithreshold, double* d_fock_out);
 
 /*
+ * Many molecules in one call (no reference counterpart: the reference builds
+ * one molecule per CFockDriver::compute call, src/t4c_func/FockDriver.cpp:120-171).
+ * Molecule m has its own screener scrs[m] (same context), symmetric density
+ * d_densities[m] and output d_focks[m] (device buffers, nao_m x nao_m); label,
+ * exchange_factor, omega and ithreshold as in vlx_fock_compute, FP64 whatever
+ * the context's precision mode.  Each molecule runs whole on one side stream
+ * with its own F' accumulator (held by its screener), longest first onto the
+ * least-loaded stream, so that small molecules run concurrently.  Runs on the
+ * context's streams and returns without synchronising.
+ */
+int vlx_fock_compute_batch_device(vlx_context* ctx, int nmol, vlx_screener* const* scrs,
+                                  const double* const* d_densities, const char* label, double exchange_factor,
+                                  double omega, int ithreshold, double* const* d_focks);
+
+/*
  * CFockDriver::compute for a general (non-symmetric) density, the reference's
  * mat_t.general path (src/t4c_func/FockDriver.cpp:120-171 with the general
  * distributors of src/t4c_func/T4CLocalDistributor.cpp; its tests compare K

@@ -290,6 +290,31 @@ def test_every_variant_matches_oracle():
         _native.check(_native.lib().vlx_context_set_variant(ctx, -1))
 
 
+def test_batch_matches_single_builds():
+    """vlx_fock_compute_batch_device over clusters of 1-6 waters of uneven size (each molecule
+    whole on one stream, concurrently) gives each molecule's single-build G, and the oracle's."""
+    drv = FockDriver()
+    scrs, dens, mols = [], [], []
+    for i, nw in enumerate((3, 1, 6, 2, 4, 1)):
+        mol = water_cluster(nw, 6.0, 40 + i)
+        bas = MolecularBasis.read(mol, 'def2-svp')
+        scrs.append(_screened(mol, bas))
+        dens.append(pseudo_density(bas.get_dimensions_of_basis(), 5 * nw, seed=60 + i))
+        mols.append((mol, bas))
+    for label, xf in (('2jk', 1.0), ('j', 0.0), ('kx', 0.3)):
+        batch = drv.compute_batch(scrs, dens, label, xf, 0.0, 12)
+        for scr, D, g in zip(scrs, dens, batch):
+            single = np.asarray(drv.compute(scr, D, label, xf, 0.0, 12))
+            assert np.max(np.abs(g - single)) < 1e-12, label
+    mol, bas = mols[2]
+    ref, _ = orc.fock(orc.basis_from(bas, mol), dens[2], '2jk', 1.0, 12, 0)
+    g = drv.compute_batch(scrs, dens, '2jk', 1.0, 0.0, 12)[2]
+    assert np.max(np.abs(g - ref)) < TOL
+    assert drv.compute_batch([], [], '2jk', 1.0, 0.0, 12) == []
+    with pytest.raises(ValueError):
+        drv.compute_batch(scrs[:1], [dens[0] + np.triu(np.ones_like(dens[0]), 1)], '2jk', 1.0, 0.0, 12)
+
+
 def test_autotune_choices_keep_parity():
     """The autotuner times every kernel variant per class; the build it leaves behind still matches the oracle."""
     mol = water_cluster(4, 6.0, 7)

@@ -64,6 +64,9 @@ def lib():
         L.vlx_fock_compute_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
                                               ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
         L.vlx_fock_compute_general.argtypes = L.vlx_fock_compute.argtypes
+        L.vlx_fock_compute_batch_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                                                    ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         L.vlx_fock_compute_general_device.argtypes = L.vlx_fock_compute_device.argtypes
         L.vlx_fock_work.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_ll_p, _c_ll_p,
                                     _c_double_p]

@@ -257,6 +257,7 @@ struct vlx_screener {
     std::map<int, std::vector<std::unique_ptr<ClassSchedule>>> schedules;  // by (ithreshold, FP32 bin-sum threshold)
     std::map<int, int> variant;  // class entry -> VARIANT_* chosen by vlx_fock_autotune
     std::map<int, double> tuned_ms;  // class entry -> its measured time (ms) under that variant
+    DevBuf<double> d_faccum;  // F' of this screener's molecule in batched builds
 };
 
 namespace {
@@ -1421,6 +1422,86 @@ namespace {
 // One build on a device density: F' by the class kernels, then
 // G = F' + F'^T into d_out (anti_coef == 0), or d_out += anti_coef (F' - F'^T)
 // (the exchange of the antisymmetric part of a general density).
+// Enqueues every class kernel of one build that accumulates F' (nao x nao) from D: on `stream`,
+// or, when it is null, on the side stream each class schedule was given.  No synchronisation.
+int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::unique_ptr<ClassSchedule>>& list,
+                    const double* d_density, double* F, double fj, double fk, double omega2, int ithreshold,
+                    hipStream_t stream)
+{
+    const bool unscreened = ithreshold < 0;
+    for (const auto& schp : list) {
+        if (fj == 0.0 && fk == 0.0) break;  // nothing to digest: G = 0
+        const ClassSchedule& sch = *schp;
+        const auto& ce = g_class_table[sch.entry];
+        const PairSet& bra = sch.tiled ? (unscreened ? scr->all_tiled_bra[ce.x] : scr->kept_tiled_bra[ce.x])
+                                       : (unscreened ? scr->all_bra[ce.x] : scr->kept_bra[ce.x]);
+        const PairSet& ket = sch.tiled ? (unscreened ? scr->all_tiled[ce.y] : scr->kept_tiled[ce.y])
+                                       : (unscreened ? scr->all[ce.y] : scr->kept[ce.y]);
+        FockArgs a{};
+        a.bra = bra.d_pairs.ptr;
+        a.bra_prims = bra.d_prims.ptr;
+        a.ket = ket.d_pairs.ptr;
+        a.ket_prims = ket.d_prims.ptr;
+        a.bra_u = bra.d_u.ptr;
+        a.ket_u = ket.d_u.ptr;
+        a.items = sch.d_items.ptr;
+        a.boys_tab = ctx->d_boys.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;
+        a.boys_tab_f = ctx->d_boys_f.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;
+        a.D = d_density;
+        a.F = F;
+        a.titems = sch.d_titems.ptr;
+        a.kentries = sch.singles ? ket.singles.d_kentries.ptr : ket.runs.d_kentries.ptr;
+        a.grp_off = scr->d_grp_off.ptr;
+        a.grp_n = scr->d_grp_n.ptr;
+        a.grp_ao = scr->d_grp_ao.ptr;
+        a.tile_lds_bytes = sch.tile_lds;
+        a.tiled_waves = sch.tiled_waves;
+        a.sc.nitems = static_cast<long long>(sch.tiled ? sch.titems.size() : sch.items.size());
+        a.sc.nket = ket.size();
+        a.sc.ithreshold = ithreshold;
+        a.sc.same_class = (ce.x == ce.y) ? 1 : 0;
+        a.sc.nao = scr->nao;
+        a.sc.fj = fj;
+        a.sc.fk = fk;
+        a.sc.omega2 = omega2;
+        const int grid = sch.tiled ? static_cast<int>(std::min<long long>(a.sc.nitems, 4096))
+                                   : static_cast<int>(untiled_waves(a.sc.nitems) / VLX_WAVES_PER_BLOCK);
+        hipStream_t st = stream ? stream : ctx->side[sch.stream % ctx->nstreams];
+        vlx_context::Pending pend{sch.entry, nullptr, nullptr};
+        if (ctx->timing) {
+            for (hipEvent_t* ev : {&pend.start, &pend.stop}) {
+                if (ctx->event_pool.empty()) {
+                    HIP_TRY(hipEventCreate(ev));
+                } else {
+                    *ev = ctx->event_pool.back();
+                    ctx->event_pool.pop_back();
+                }
+            }
+            HIP_TRY(hipEventRecord(pend.start, st));
+        }
+        if (sch.tiled) {
+            if (!sch.titems.empty()) ce.tiled(a, grid, st);
+            if (!sch.titems32.empty()) {
+                FockArgs a32 = a;
+                a32.titems = sch.d_titems32.ptr;
+                a32.sc.nitems = static_cast<long long>(sch.titems32.size());
+                const int grid32 = static_cast<int>(std::min<long long>(a32.sc.nitems, 4096));
+                ce.tiled32(a32, grid32, st);
+            }
+        } else if (sch.mmd) {
+            ce.mmd(a, grid, st);
+        } else {
+            ce.fock(a, grid, st);
+        }
+        HIP_TRY(hipGetLastError());
+        if (ctx->timing) {
+            HIP_TRY(hipEventRecord(pend.stop, st));
+            ctx->pending.push_back(pend);
+        }
+    }
+    return VLX_OK;
+}
+
 int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, double fj, double fk, double omega2,
                int ithreshold, double* d_fock_out, double anti_coef)
 {
@@ -1454,78 +1535,10 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
     std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
     rc = get_schedules(scr, ithreshold, k32, &list);
     if (rc) return rc;
-    const bool unscreened = ithreshold < 0;
     HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
     for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamWaitEvent(ctx->side[k], ctx->ev_start, 0));
-    for (const auto& schp : *list) {
-        if (fj == 0.0 && fk == 0.0) break;  // nothing to digest: G = 0
-        const ClassSchedule& sch = *schp;
-        const auto& ce = g_class_table[sch.entry];
-        const PairSet& bra = sch.tiled ? (unscreened ? scr->all_tiled_bra[ce.x] : scr->kept_tiled_bra[ce.x])
-                                       : (unscreened ? scr->all_bra[ce.x] : scr->kept_bra[ce.x]);
-        const PairSet& ket = sch.tiled ? (unscreened ? scr->all_tiled[ce.y] : scr->kept_tiled[ce.y])
-                                       : (unscreened ? scr->all[ce.y] : scr->kept[ce.y]);
-        FockArgs a{};
-        a.bra = bra.d_pairs.ptr;
-        a.bra_prims = bra.d_prims.ptr;
-        a.ket = ket.d_pairs.ptr;
-        a.ket_prims = ket.d_prims.ptr;
-        a.bra_u = bra.d_u.ptr;
-        a.ket_u = ket.d_u.ptr;
-        a.items = sch.d_items.ptr;
-        a.boys_tab = ctx->d_boys.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;
-        a.boys_tab_f = ctx->d_boys_f.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;
-        a.D = d_density;
-        a.F = ctx->d_faccum.ptr;
-        a.titems = sch.d_titems.ptr;
-        a.kentries = sch.singles ? ket.singles.d_kentries.ptr : ket.runs.d_kentries.ptr;
-        a.grp_off = scr->d_grp_off.ptr;
-        a.grp_n = scr->d_grp_n.ptr;
-        a.grp_ao = scr->d_grp_ao.ptr;
-        a.tile_lds_bytes = sch.tile_lds;
-        a.tiled_waves = sch.tiled_waves;
-        a.sc.nitems = static_cast<long long>(sch.tiled ? sch.titems.size() : sch.items.size());
-        a.sc.nket = ket.size();
-        a.sc.ithreshold = ithreshold;
-        a.sc.same_class = (ce.x == ce.y) ? 1 : 0;
-        a.sc.nao = n;
-        a.sc.fj = fj;
-        a.sc.fk = fk;
-        a.sc.omega2 = omega2;
-        const int grid = sch.tiled ? static_cast<int>(std::min<long long>(a.sc.nitems, 4096))
-                                   : static_cast<int>(untiled_waves(a.sc.nitems) / VLX_WAVES_PER_BLOCK);
-        vlx_context::Pending pend{sch.entry, nullptr, nullptr};
-        if (ctx->timing) {
-            for (hipEvent_t* ev : {&pend.start, &pend.stop}) {
-                if (ctx->event_pool.empty()) {
-                    HIP_TRY(hipEventCreate(ev));
-                } else {
-                    *ev = ctx->event_pool.back();
-                    ctx->event_pool.pop_back();
-                }
-            }
-            HIP_TRY(hipEventRecord(pend.start, ctx->side[sch.stream % ctx->nstreams]));
-        }
-        if (sch.tiled) {
-            if (!sch.titems.empty()) ce.tiled(a, grid, ctx->side[sch.stream % ctx->nstreams]);
-            if (!sch.titems32.empty()) {
-                FockArgs a32 = a;
-                a32.titems = sch.d_titems32.ptr;
-                a32.sc.nitems = static_cast<long long>(sch.titems32.size());
-                const int grid32 = static_cast<int>(std::min<long long>(a32.sc.nitems, 4096));
-                ce.tiled32(a32, grid32, ctx->side[sch.stream % ctx->nstreams]);
-            }
-        } else if (sch.mmd) {
-            ce.mmd(a, grid, ctx->side[sch.stream % ctx->nstreams]);
-        } else {
-            ce.fock(a, grid, ctx->side[sch.stream % ctx->nstreams]);
-        }
-        HIP_TRY(hipGetLastError());
-        if (ctx->timing) {
-            HIP_TRY(hipEventRecord(pend.stop, ctx->side[sch.stream % ctx->nstreams]));
-            ctx->pending.push_back(pend);
-        }
-    }
+    rc = enqueue_classes(ctx, scr, *list, d_density, ctx->d_faccum.ptr, fj, fk, omega2, ithreshold, nullptr);
+    if (rc) return rc;
     for (int k = 0; k < NSIDE; k++) {
         HIP_TRY(hipEventRecord(ctx->ev_done[k], ctx->side[k]));
         HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_done[k], 0));
@@ -1554,6 +1567,65 @@ int parse_build(vlx_context* ctx, vlx_screener* scr, const void* d_density, cons
 }
 
 }  // namespace
+
+int vlx_fock_compute_batch_device(vlx_context* ctx, int nmol, vlx_screener* const* scrs, const double* const* d_densities,
+                                  const char* label, double exchange_factor, double omega, int ithreshold,
+                                  double* const* d_focks)
+{
+    if (!ctx || nmol < 0 || (nmol > 0 && (!scrs || !d_densities || !d_focks)))
+        return fail(VLX_ERR_ARG, "NULL argument to vlx_fock_compute_batch_device");
+    if (nmol == 0) return VLX_OK;
+    double fj = 0.0, fk = 0.0, omega2 = 0.0;
+    for (int m = 0; m < nmol; m++) {
+        const int rc = parse_build(ctx, scrs[m], d_densities[m], d_focks[m], label, exchange_factor, omega, ithreshold,
+                                   &fj, &fk, &omega2);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    // host side first (schedules, accumulators), then the launches: molecules go whole to one
+    // side stream each, longest first onto the least-loaded stream
+    std::vector<std::vector<std::unique_ptr<ClassSchedule>>*> lists(nmol, nullptr);
+    std::vector<std::pair<double, int>> cost(nmol);
+    for (int m = 0; m < nmol; m++) {
+        vlx_screener* scr = scrs[m];
+        const int rc = get_schedules(scr, ithreshold, NO_FP32, &lists[m]);
+        if (rc) return rc;
+        HIP_TRY(scr->d_faccum.alloc((size_t)scr->nao * scr->nao));
+        double c = 0.0;
+        for (const auto& sp : *lists[m]) {
+            const auto& ce = g_class_table[sp->entry];
+            c += (double)sp->prim_quartets * (ce.vrr_flops + 60) + (double)sp->quartets * ce.hrr_flops;
+        }
+        cost[m] = {c + 1.0e4 * lists[m]->size(), m};  // plus a per-launch term
+    }
+    std::sort(cost.begin(), cost.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    const int nst = ctx->nstreams;
+    std::vector<double> load(nst, 0.0);
+    HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+    for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamWaitEvent(ctx->side[k], ctx->ev_start, 0));
+    for (const auto& cm : cost) {
+        const int m = cm.second;
+        const int k = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        load[k] += cm.first;
+        hipStream_t st = ctx->side[k];
+        vlx_screener* scr = scrs[m];
+        const int n = scr->nao;
+        const size_t nn = (size_t)n * n;
+        zero_kernel<<<std::min<size_t>((nn + 255) / 256, 4096), 256, 0, st>>>(scr->d_faccum.ptr, nn);
+        HIP_TRY(hipGetLastError());
+        const int rc = enqueue_classes(ctx, scr, *lists[m], d_densities[m], scr->d_faccum.ptr, fj, fk, omega2,
+                                       ithreshold, st);
+        if (rc) return rc;
+        dim3 sg((n + 255) / 256, n);
+        symmetrize_kernel<<<sg, 256, 0, st>>>(scr->d_faccum.ptr, d_focks[m], n);
+        HIP_TRY(hipGetLastError());
+    }
+    for (int k = 0; k < NSIDE; k++) {
+        HIP_TRY(hipEventRecord(ctx->ev_done[k], ctx->side[k]));
+        HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_done[k], 0));
+    }
+    return VLX_OK;
+}
 
 int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,
                             double exchange_factor, double omega, int ithreshold, double* d_fock_out)

@@ -149,6 +149,47 @@ class FockDriver:
                 eri[np.ix_(*idx)] = perm_blk
         return eri
 
+    def compute_batch(self, screeners, densities, label, exchange_factor=0.0, omega=0.0, ithreshold=15):
+        """G of many molecules in one call (vlx_fock_compute_batch_device).
+
+        ``screeners[m]`` and ``densities[m]`` (symmetric, nao_m x nao_m, Matrix or array) describe
+        molecule m; returns the list of G as numpy arrays.  Every molecule runs whole on one of the
+        context's streams, so that small molecules run concurrently; FP64.  No reference
+        counterpart: the reference builds one molecule per ``compute`` call.
+        """
+        if len(screeners) != len(densities):
+            raise ValueError('compute_batch: one density per screener')
+        if ithreshold > 30:
+            raise ValueError(f'FockDriver: ithreshold {ithreshold} outside 0..30')
+        bufs = []
+        try:
+            d_ptrs, g_ptrs = [], []
+            for scr, dens in zip(screeners, densities):
+                dm, _ = _dense(dens)
+                n = scr.number_of_ao()
+                if dm.shape != (n, n):
+                    raise ValueError(f'FockDriver: density has shape {dm.shape}, basis has {n} AOs')
+                # the same test as the symmetric host entry point (vlx_fock_compute)
+                if np.any(np.abs(dm - dm.T) > 1.0e-12 * np.maximum(1.0, np.maximum(np.abs(dm), np.abs(dm.T)))):
+                    raise ValueError('compute_batch: density is not symmetric')
+                d = _native.DeviceArray((n, n), self._device)
+                bufs.append(d)
+                d.upload(dm)
+                g = _native.DeviceArray((n, n), self._device)
+                bufs.append(g)
+                d_ptrs.append(d.ptr.value)
+                g_ptrs.append(g.ptr.value)
+            nmol = len(screeners)
+            arr = ctypes.c_void_p * max(nmol, 1)
+            _native.check(_native.lib().vlx_fock_compute_batch_device(
+                _native.context(self._device), nmol, arr(*[s.handle.value for s in screeners]), arr(*d_ptrs),
+                label.encode(), ctypes.c_double(exchange_factor), ctypes.c_double(omega), int(ithreshold),
+                arr(*g_ptrs)))
+            return [bufs[2 * m + 1].download() for m in range(nmol)]
+        finally:
+            for b in bufs:
+                b.free()
+
     # -- helpers ----------------------------------------------------------------
 
     def eri_quartets(self, screener, quartets):
