@@ -140,7 +140,7 @@ struct PairSet {
 struct ClassSchedule {
     int entry = -1;  // index into g_class_table
     long long lane_slots = 0, prim_slots = 0;  // schedule efficiency diagnostics
-    int stream = 0;  // side stream the class runs on
+    int stream = 0;  // side stream the class runs on; -1: alone on the main stream before the others
     int variant = 0;      // VARIANT_* the schedule runs
     bool tiled = false;
     bool singles = false; // tiled over single-primitive ket entries
@@ -203,6 +203,8 @@ struct vlx_context {
     // untiled items reordered so each wave's contiguous range has equal estimated cost
     // (VLX_BALANCE_ITEMS=0: the plain build order)
     int balance_items = 1;
+    // classes whose estimated share of the build reaches solo_frac run alone (VLX_SOLO_FRAC; 0: none)
+    double solo_frac = 0.0;
     // order of equal-(bin, nprim) kets in the untiled lists (VLX_KET_ORDER): 2, along diagonals
     // of equal C - D offset, gives the lanes of a wave distinct and mostly consecutive F' targets
     // for their exchange atomics (shell order, 0, puts lanes that share C on one address)
@@ -782,9 +784,17 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
         cost.push_back({c, sp.get()});
     }
     std::sort(cost.begin(), cost.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    // the classes above solo_frac of the build fill the chip by themselves: they run one after
+    // another alone, and the rest share the side streams (overlap fills the tails of small launches)
+    double total = 0.0;
+    for (auto& c : cost) total += c.first;
     double load[NSIDE] = {0.0};
     const int nst = s->ctx->nstreams;
     for (auto& c : cost) {
+        if (nst > 1 && s->ctx->solo_frac > 0.0 && c.first >= s->ctx->solo_frac * total) {
+            c.second->stream = -1;
+            continue;
+        }
         const int k = static_cast<int>(std::min_element(load, load + nst) - load);
         c.second->stream = k;
         load[k] += c.first;
@@ -833,6 +843,7 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_KET_PRIMS")) ctx->ket_prims = std::max(1, std::min(16, std::atoi(e)));
     if (const char* e = std::getenv("VLX_BALANCE")) ctx->balance_measured = std::atoi(e) != 0;
     if (const char* e = std::getenv("VLX_BALANCE_ITEMS")) ctx->balance_items = std::atoi(e) != 0;
+    if (const char* e = std::getenv("VLX_SOLO_FRAC")) ctx->solo_frac = std::max(0.0, std::atof(e));
     if (const char* e = std::getenv("VLX_KET_ORDER")) ctx->ket_order = std::max(0, std::min(2, std::atoi(e)));
     if (const char* e = std::getenv("VLX_VARIANT")) ctx->force_variant = std::max(-1, std::min(NVARIANTS - 1, std::atoi(e)));
     HIP_TRY(hipSetDevice(device));
@@ -1422,16 +1433,18 @@ namespace {
 // One build on a device density: F' by the class kernels, then
 // G = F' + F'^T into d_out (anti_coef == 0), or d_out += anti_coef (F' - F'^T)
 // (the exchange of the antisymmetric part of a general density).
-// Enqueues every class kernel of one build that accumulates F' (nao x nao) from D: on `stream`,
-// or, when it is null, on the side stream each class schedule was given.  No synchronisation.
+// Enqueues the class kernels of one build that accumulate F' (nao x nao) from D: on `stream`, or,
+// when it is null, on the side stream each class schedule was given.  solo: 1 only the classes
+// that run alone (stream -1), 0 only the others, -1 all.  No synchronisation.
 int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::unique_ptr<ClassSchedule>>& list,
                     const double* d_density, double* F, double fj, double fk, double omega2, int ithreshold,
-                    hipStream_t stream)
+                    hipStream_t stream, int solo = -1)
 {
     const bool unscreened = ithreshold < 0;
     for (const auto& schp : list) {
         if (fj == 0.0 && fk == 0.0) break;  // nothing to digest: G = 0
         const ClassSchedule& sch = *schp;
+        if (solo >= 0 && (sch.stream < 0) != (solo == 1)) continue;
         const auto& ce = g_class_table[sch.entry];
         const PairSet& bra = sch.tiled ? (unscreened ? scr->all_tiled_bra[ce.x] : scr->kept_tiled_bra[ce.x])
                                        : (unscreened ? scr->all_bra[ce.x] : scr->kept_bra[ce.x]);
@@ -1466,7 +1479,7 @@ int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::
         a.sc.omega2 = omega2;
         const int grid = sch.tiled ? static_cast<int>(std::min<long long>(a.sc.nitems, 4096))
                                    : static_cast<int>(untiled_waves(a.sc.nitems) / VLX_WAVES_PER_BLOCK);
-        hipStream_t st = stream ? stream : ctx->side[sch.stream % ctx->nstreams];
+        hipStream_t st = stream ? stream : (sch.stream < 0 ? ctx->stream : ctx->side[sch.stream % ctx->nstreams]);
         vlx_context::Pending pend{sch.entry, nullptr, nullptr};
         if (ctx->timing) {
             for (hipEvent_t* ev : {&pend.start, &pend.stop}) {
@@ -1535,9 +1548,11 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
     std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
     rc = get_schedules(scr, ithreshold, k32, &list);
     if (rc) return rc;
+    rc = enqueue_classes(ctx, scr, *list, d_density, ctx->d_faccum.ptr, fj, fk, omega2, ithreshold, nullptr, 1);
+    if (rc) return rc;
     HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
     for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamWaitEvent(ctx->side[k], ctx->ev_start, 0));
-    rc = enqueue_classes(ctx, scr, *list, d_density, ctx->d_faccum.ptr, fj, fk, omega2, ithreshold, nullptr);
+    rc = enqueue_classes(ctx, scr, *list, d_density, ctx->d_faccum.ptr, fj, fk, omega2, ithreshold, nullptr, 0);
     if (rc) return rc;
     for (int k = 0; k < NSIDE; k++) {
         HIP_TRY(hipEventRecord(ctx->ev_done[k], ctx->side[k]));
