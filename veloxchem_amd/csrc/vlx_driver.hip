@@ -107,6 +107,12 @@ struct DevBuf {
     }
 };
 
+// Non-owning device pointer into an arena (a DevBuf<char> that holds several arrays).
+template <typename T>
+struct DevView {
+    T* ptr = nullptr;
+};
+
 }  // namespace
 
 struct PairSet {
@@ -127,11 +133,13 @@ struct PairSet {
     struct Entries {
         std::vector<int2> kentries;
         std::vector<int> seg_e0, seg_e1;
-        DevBuf<int2> d_kentries;
+        DevView<int2> d_kentries;
     };
     Entries runs, singles;
-    DevBuf<PairInfo> d_pairs;
-    DevBuf<PrimPair> d_prims;
+    // device copies of pairs, prims and both entry lists, in one allocation
+    DevBuf<char> d_arena;
+    DevView<PairInfo> d_pairs;
+    DevView<PrimPair> d_prims;
     DevBuf<double> d_u;   // matrix-core MD kernel: U of every primitive pair (built on first use)
     bool u_ready = false;
     int size() const { return static_cast<int>(pairs.size()); }
@@ -374,10 +382,24 @@ void reorder_set(const vlx_screener& s, const PairSet& src, const std::vector<in
     dst.shell_a.clear();
     dst.shell_b.clear();
     dst.bins.clear();
+    (void)s;
+    // the pair records and their primitive pairs are copied (they do not depend on the order),
+    // with the primitive range moved to the new position
+    dst.pairs.reserve(order.size());
+    dst.shell_a.reserve(order.size());
+    dst.shell_b.reserve(order.size());
+    dst.bins.reserve(order.size());
+    size_t np = 0;
+    for (int k : order) np += src.pairs[k].nprim;
+    dst.prims.reserve(np);
     for (int k : order) {
-        add_pair(s, dst, src.shell_a[k], src.shell_b[k]);
-        dst.pairs.back().bin = src.pairs[k].bin;
-        dst.pairs.back().rank = src.pairs[k].rank;
+        PairInfo pi = src.pairs[k];
+        const auto p0 = src.prims.begin() + pi.prim_start;
+        pi.prim_start = static_cast<int>(dst.prims.size());
+        dst.prims.insert(dst.prims.end(), p0, p0 + pi.nprim);
+        dst.pairs.push_back(pi);
+        dst.shell_a.push_back(src.shell_a[k]);
+        dst.shell_b.push_back(src.shell_b[k]);
         dst.bins.push_back(src.pairs[k].bin);
     }
     finalize_set(dst);
@@ -482,10 +504,25 @@ int ensure_u(vlx_context* ctx, PairSet& ps)
 
 int upload_set(PairSet& ps)
 {
-    HIP_TRY(ps.d_pairs.upload(ps.pairs));
-    HIP_TRY(ps.d_prims.upload(ps.prims));
-    for (PairSet::Entries* en : {&ps.runs, &ps.singles})
-        if (!en->kentries.empty()) HIP_TRY(en->d_kentries.upload(en->kentries));
+    // one allocation per set (a screener holds dozens of sets; hipMalloc per array dominated
+    // its set-up), each array at a 256-byte boundary
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t b_pairs = up(ps.pairs.size() * sizeof(PairInfo)), b_prims = up(ps.prims.size() * sizeof(PrimPair));
+    const size_t b_runs = up(ps.runs.kentries.size() * sizeof(int2));
+    const size_t b_singles = up(ps.singles.kentries.size() * sizeof(int2));
+    HIP_TRY(ps.d_arena.alloc(b_pairs + b_prims + b_runs + b_singles));
+    char* base = ps.d_arena.ptr;
+    ps.d_pairs.ptr = reinterpret_cast<PairInfo*>(base);
+    ps.d_prims.ptr = reinterpret_cast<PrimPair*>(base + b_pairs);
+    ps.runs.d_kentries.ptr = b_runs ? reinterpret_cast<int2*>(base + b_pairs + b_prims) : nullptr;
+    ps.singles.d_kentries.ptr = b_singles ? reinterpret_cast<int2*>(base + b_pairs + b_prims + b_runs) : nullptr;
+    auto copy = [](void* dst, const void* src, size_t bytes) {
+        return bytes ? hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    HIP_TRY(copy(ps.d_pairs.ptr, ps.pairs.data(), ps.pairs.size() * sizeof(PairInfo)));
+    HIP_TRY(copy(ps.d_prims.ptr, ps.prims.data(), ps.prims.size() * sizeof(PrimPair)));
+    HIP_TRY(copy(ps.runs.d_kentries.ptr, ps.runs.kentries.data(), ps.runs.kentries.size() * sizeof(int2)));
+    HIP_TRY(copy(ps.singles.d_kentries.ptr, ps.singles.kentries.data(), ps.singles.kentries.size() * sizeof(int2)));
     return VLX_OK;
 }
 

@@ -31,6 +31,7 @@ import ctypes
 import math
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -224,9 +225,14 @@ class ScfRestrictedDriver:
             t = time.perf_counter()
             den = sad_density(molecule, min_basis, val_basis)
             self.timings['guess'] += time.perf_counter() - t
-            first = self._comp_diis(molecule, val_basis, den, 1.0e-3, 5)
-            den = self._project_density(molecule, ao_basis, val_basis, first['C_alpha'])
-            return self._comp_diis(molecule, ao_basis, den, self.conv_thresh, self.max_iter)
+            # the one-electron integrals of both stages (host threads, the GIL released) are
+            # computed while the screeners are built and the valence stage runs on the GPU
+            with ThreadPoolExecutor(max_workers=2) as pool:
+                one_val = pool.submit(compute_one_electron_integrals, molecule, val_basis)
+                one_full = pool.submit(compute_one_electron_integrals, molecule, ao_basis)
+                first = self._comp_diis(molecule, val_basis, den, 1.0e-3, 5, one_val)
+                den = self._project_density(molecule, ao_basis, val_basis, first['C_alpha'])
+                return self._comp_diis(molecule, ao_basis, den, self.conv_thresh, self.max_iter, one_full)
         raise ValueError(f'ScfRestrictedDriver: unknown acc_type {self.acc_type!r}')
 
     def _project_density(self, molecule, ao_basis, val_basis, c_val):
@@ -247,10 +253,18 @@ class ScfRestrictedDriver:
         nocc = molecule.number_of_alpha_electrons()
         return c[:, :nocc] @ c[:, :nocc].T
 
-    def _comp_diis(self, molecule, basis, den_np, conv_thresh, max_iter):
+    def _comp_diis(self, molecule, basis, den_np, conv_thresh, max_iter, one_e=None):
         dev = self.device
         t = time.perf_counter()
-        S_np, T_np, V_np = compute_one_electron_integrals(molecule, basis)
+        if self.screener_factory is None:
+            screener = T4CScreener()
+            screener.partition(basis, molecule, 'eri')
+        else:
+            screener = self.screener_factory(basis, molecule)
+        self.timings['screener'] += time.perf_counter() - t
+        t = time.perf_counter()
+        # one_e: a future of the integrals started earlier (only the wait is timed)
+        S_np, T_np, V_np = one_e.result() if one_e is not None else compute_one_electron_integrals(molecule, basis)
         self.timings['one_electron'] += time.perf_counter() - t
         t = time.perf_counter()
         S = torch.from_numpy(S_np).to(dev)
@@ -261,13 +275,6 @@ class ScfRestrictedDriver:
         keep = w > self.ovl_thresh
         X = (U[:, keep] * (1.0 / torch.sqrt(w[keep]))).contiguous()
         self.timings['linear_algebra'] += time.perf_counter() - t
-        t = time.perf_counter()
-        if self.screener_factory is None:
-            screener = T4CScreener()
-            screener.partition(basis, molecule, 'eri')
-        else:
-            screener = self.screener_factory(basis, molecule)
-        self.timings['screener'] += time.perf_counter() - t
         nocc = molecule.number_of_alpha_electrons()
         focks, errs = deque(), deque()
         bmat = np.zeros((0, 0))
