@@ -114,8 +114,9 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
  * exchange_factor, omega and ithreshold as in vlx_fock_compute, FP64 whatever
  * the context's precision mode.  Each molecule runs whole on one side stream
  * with its own F' accumulator (held by its screener), longest first onto the
- * least-loaded stream, so that small molecules run concurrently.  Runs on the
- * context's streams and returns without synchronising.
+ * least-loaded stream, so that small molecules run concurrently (a screener
+ * listed more than once keeps one stream).  The d_focks must not overlap.
+ * Runs on the context's streams and returns without synchronising.
  */
 int vlx_fock_compute_batch_device(vlx_context* ctx, int nmol, vlx_screener* const* scrs,
                                   const double* const* d_densities, const char* label, double exchange_factor,

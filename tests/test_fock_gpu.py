@@ -311,6 +311,9 @@ def test_batch_matches_single_builds():
     g = drv.compute_batch(scrs, dens, '2jk', 1.0, 0.0, 12)[2]
     assert np.max(np.abs(g - ref)) < TOL
     assert drv.compute_batch([], [], '2jk', 1.0, 0.0, 12) == []
+    # one screener twice with different densities: the two builds share its accumulator in turn
+    twice = drv.compute_batch([scrs[0], scrs[2], scrs[0]], [dens[0], dens[2], 2.0 * dens[0]], 'j', 0.0, 0.0, 12)
+    assert np.max(np.abs(twice[2] - 2.0 * twice[0])) < 1e-12
     with pytest.raises(ValueError):
         drv.compute_batch(scrs[:1], [dens[0] + np.triu(np.ones_like(dens[0]), 1)], '2jk', 1.0, 0.0, 12)
 

@@ -1655,9 +1655,16 @@ int vlx_fock_compute_batch_device(vlx_context* ctx, int nmol, vlx_screener* cons
     std::vector<double> load(nst, 0.0);
     HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
     for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamWaitEvent(ctx->side[k], ctx->ev_start, 0));
+    // a screener listed more than once keeps one stream: its F' accumulator is then reused in
+    // stream order, never by two streams at once
+    std::map<const vlx_screener*, int> stream_of;
     for (const auto& cm : cost) {
         const int m = cm.second;
-        const int k = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        const auto seen = stream_of.find(scrs[m]);
+        const int k = seen != stream_of.end()
+                          ? seen->second
+                          : static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        stream_of[scrs[m]] = k;
         load[k] += cm.first;
         hipStream_t st = ctx->side[k];
         vlx_screener* scr = scrs[m];
