@@ -206,8 +206,9 @@ struct vlx_context {
     // primitives of one ket pair per lane of the tiled kernels (VLX_KET_PRIMS, 1..16): a ket
     // pair of n primitives takes ceil(n / ket_prims) lanes with near-equal shares
     int ket_prims = 3;
-    // stream balance by the autotuner's measured class times (VLX_BALANCE=1) or by estimated FLOPs (0)
-    int balance_measured = 0;
+    // stream balance by the autotuner's measured class times when every class has one (VLX_BALANCE=1,
+    // default; 40.4 against 40.1 builds/s on the 32-water build) or by estimated FLOPs (0)
+    int balance_measured = 1;
     // untiled items reordered so each wave's contiguous range has equal estimated cost
     // (VLX_BALANCE_ITEMS=0: the plain build order)
     int balance_items = 1;
