@@ -209,6 +209,7 @@ struct vlx_context {
     // stream balance by the autotuner's measured class times when every class has one (VLX_BALANCE=1,
     // default; 40.4 against 40.1 builds/s on the 32-water build) or by estimated FLOPs (0)
     int balance_measured = 1;
+    bool launch_by_cost = true;  // VLX_LAUNCH_ORDER=0: class-table order on every stream
     // untiled items reordered so each wave's contiguous range has equal estimated cost
     // (VLX_BALANCE_ITEMS=0: the plain build order)
     int balance_items = 1;
@@ -837,6 +838,14 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
         c.second->stream = k;
         load[k] += c.first;
     }
+    // launch order: most expensive first on every stream, so that the streams end on short launches
+    if (s->ctx->launch_by_cost) {
+        std::vector<std::unique_ptr<ClassSchedule>> sorted;
+        for (auto& c : cost)
+            for (auto& sp : list)
+                if (sp.get() == c.second) sorted.push_back(std::move(sp));
+        list.swap(sorted);
+    }
     auto res = s->schedules.emplace(key, std::move(list));
     *out = &res.first->second;
     return VLX_OK;
@@ -881,6 +890,7 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_KET_PRIMS")) ctx->ket_prims = std::max(1, std::min(16, std::atoi(e)));
     if (const char* e = std::getenv("VLX_BALANCE")) ctx->balance_measured = std::atoi(e) != 0;
     if (const char* e = std::getenv("VLX_BALANCE_ITEMS")) ctx->balance_items = std::atoi(e) != 0;
+    if (const char* e = std::getenv("VLX_LAUNCH_ORDER")) ctx->launch_by_cost = std::atoi(e) != 0;
     if (const char* e = std::getenv("VLX_SOLO_FRAC")) ctx->solo_frac = std::max(0.0, std::atof(e));
     if (const char* e = std::getenv("VLX_KET_ORDER")) ctx->ket_order = std::max(0, std::min(2, std::atoi(e)));
     if (const char* e = std::getenv("VLX_VARIANT")) ctx->force_variant = std::max(-1, std::min(NVARIANTS - 1, std::atoi(e)));
