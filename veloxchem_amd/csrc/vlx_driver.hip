@@ -213,6 +213,7 @@ struct vlx_context {
     // largest grid of a tiled launch, blocks striding over the items (VLX_TILED_GRID); 2048 measured
     // best on the 32-water build (42.2 builds/s against 41.7 at 4096 and 40.2 at 8192)
     int tiled_grid = 2048;
+    int untiled_wave_cap = 8192;  // largest wavefront count of an untiled launch (VLX_UNTILED_WAVES)
     // untiled items reordered so each wave's contiguous range has equal estimated cost
     // (VLX_BALANCE_ITEMS=0: the plain build order)
     int balance_items = 1;
@@ -327,11 +328,11 @@ void add_pair(const vlx_screener& s, PairSet& ps, int sa, int sb)
     build_prims(s, sa, sb, ps.prims);
 }
 
-// Wavefronts of an untiled class launch (fock_kernel, fock_kernel_bs): at most 8192, in blocks
+// Wavefronts of an untiled class launch (fock_kernel, fock_kernel_bs): at most `cap`, in blocks
 // of VLX_WAVES_PER_BLOCK; wave g takes the contiguous items [n g / nw, n (g + 1) / nw).
-long long untiled_waves(long long nitems)
+long long untiled_waves(long long nitems, int cap)
 {
-    const long long waves = std::min<long long>(nitems, 8192);
+    const long long waves = std::min<long long>(nitems, cap);
     return (waves + VLX_WAVES_PER_BLOCK - 1) / VLX_WAVES_PER_BLOCK * VLX_WAVES_PER_BLOCK;
 }
 
@@ -789,7 +790,8 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
                 cost.push_back(kmax * bprims * pcost + nbra * qcost);
             }
             if (s->ctx->balance_items)
-                balance_items(sch->items, cost, untiled_waves(static_cast<long long>(sch->items.size())));
+                balance_items(sch->items, cost,
+                              untiled_waves(static_cast<long long>(sch->items.size()), s->ctx->untiled_wave_cap));
         } else {
             for (const auto& wi : sch->items) {  // matrix-core MD: one ket per wavefront pass
                 int kmax = 0;
@@ -895,6 +897,8 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_BALANCE_ITEMS")) ctx->balance_items = std::atoi(e) != 0;
     if (const char* e = std::getenv("VLX_LAUNCH_ORDER")) ctx->launch_by_cost = std::atoi(e) != 0;
     if (const char* e = std::getenv("VLX_TILED_GRID")) ctx->tiled_grid = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("VLX_UNTILED_WAVES"))
+        ctx->untiled_wave_cap = std::max(VLX_WAVES_PER_BLOCK, std::atoi(e));
     if (const char* e = std::getenv("VLX_SOLO_FRAC")) ctx->solo_frac = std::max(0.0, std::atof(e));
     if (const char* e = std::getenv("VLX_KET_ORDER")) ctx->ket_order = std::max(0, std::min(2, std::atoi(e)));
     if (const char* e = std::getenv("VLX_VARIANT")) ctx->force_variant = std::max(-1, std::min(NVARIANTS - 1, std::atoi(e)));
@@ -1530,7 +1534,7 @@ int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::
         a.sc.fk = fk;
         a.sc.omega2 = omega2;
         const int grid = sch.tiled ? static_cast<int>(std::min<long long>(a.sc.nitems, ctx->tiled_grid))
-                                   : static_cast<int>(untiled_waves(a.sc.nitems) / VLX_WAVES_PER_BLOCK);
+                                   : static_cast<int>(untiled_waves(a.sc.nitems, ctx->untiled_wave_cap) / VLX_WAVES_PER_BLOCK);
         hipStream_t st = stream ? stream : (sch.stream < 0 ? ctx->stream : ctx->side[sch.stream % ctx->nstreams]);
         vlx_context::Pending pend{sch.entry, nullptr, nullptr};
         if (ctx->timing) {
