@@ -167,6 +167,12 @@ def tzvp_builds(args, taus):
     """Mixed-precision workload: seeded 64-water cluster (12.8 A box, seed 64), def2-TZVP
     (2752 AOs, f shells): FP64 build rate and the mixed sweep with its error against FP64."""
     mol, bas, scr, dm, t_screen = build_workload(64, 12.8, 64, 'def2-tzvp')
+    t_tune = 0.0
+    if not args.no_tzvp_autotune:
+        from veloxchem_amd import FockDriver
+        t0 = time.perf_counter()
+        FockDriver()._autotune(scr, dm, args.ithreshold, 1)
+        t_tune = time.perf_counter() - t0
     ms, times, _, sweep = measure_builds(scr, dm, args.ithreshold, args.tzvp_steps, 1, taus)
     work = scr.work(args.ithreshold)
     flops = sum(w[3] for w in work)
@@ -174,6 +180,7 @@ def tzvp_builds(args, taus):
                         f'{bas.get_dimensions_of_basis()} AOs, 2jk on the SAD density, ithreshold {args.ithreshold}',
             'fp64_builds_per_s': 1e3 / ms, 'fp64_ms_per_step': ms, 'steps': args.tzvp_steps,
             'build_achieved_tflops': flops / (ms * 1e-3) / 1e12, 'screener_setup_s': round(t_screen, 3),
+            'autotune_s': round(t_tune, 1),
             'shell_quartets': int(sum(w[1] for w in work)), 'mixed_sweep': sweep}
 
 
@@ -269,6 +276,8 @@ def main():
     ap.add_argument('--no-tzvp', action='store_true',
                     help='skip the 64-water def2-TZVP FP64 build and mixed-precision sweep')
     ap.add_argument('--tzvp-steps', type=int, default=2)
+    ap.add_argument('--no-tzvp-autotune', action='store_true',
+                    help='run the def2-TZVP builds with the default kernel variants (no autotuner pass)')
     ap.add_argument('--no-batch', action='store_true',
                     help='skip the batched workload of 256 small water clusters')
     ap.add_argument('--batch-clusters', type=int, default=256)
