@@ -637,16 +637,14 @@ __global__ void __launch_bounds__(VLX_WAVE) diag_kernel_md(const DiagArgs a)
 #define VLX_INSTANTIATE_MMD(la, lb, lc, ld)                                                             \
     void launch_mmd_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                        \
     {                                                                                                   \
-        (void)grid;                                                                                     \
-        const int g = static_cast<int>(std::min<long long>(a.sc.nitems, 16384));                        \
+        const int g = grid > 0 ? grid : static_cast<int>(std::min<long long>(a.sc.nitems, 16384));       \
         if (a.sc.omega2 > 0.0) VLX_MMD_LAUNCH(la, lb, lc, ld, true);                                    \
         else VLX_MMD_LAUNCH(la, lb, lc, ld, false);                                                     \
     }
 #define VLX_INSTANTIATE_MD_CLASS(la, lb, lc, ld)                                                        \
     void launch_fock_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                       \
     {                                                                                                   \
-        (void)grid;                                                                                     \
-        const int g = static_cast<int>(std::min<long long>(a.sc.nitems, 16384));                        \
+        const int g = grid > 0 ? grid : static_cast<int>(std::min<long long>(a.sc.nitems, 16384));       \
         if (a.sc.omega2 > 0.0) VLX_MMD_LAUNCH(la, lb, lc, ld, true);                                    \
         else VLX_MMD_LAUNCH(la, lb, lc, ld, false);                                                     \
     }                                                                                                   \

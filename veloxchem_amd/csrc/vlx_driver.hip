@@ -214,6 +214,7 @@ struct vlx_context {
     // best on the 32-water build (42.2 builds/s against 41.7 at 4096 and 40.2 at 8192)
     int tiled_grid = 2048;
     int untiled_wave_cap = 8192;  // largest wavefront count of an untiled launch (VLX_UNTILED_WAVES)
+    int mmd_grid = 16384;         // largest grid (one wavefront per block) of a matrix-core MD launch (VLX_MMD_GRID)
     // untiled items reordered so each wave's contiguous range has equal estimated cost
     // (VLX_BALANCE_ITEMS=0: the plain build order)
     int balance_items = 1;
@@ -897,6 +898,7 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_BALANCE_ITEMS")) ctx->balance_items = std::atoi(e) != 0;
     if (const char* e = std::getenv("VLX_LAUNCH_ORDER")) ctx->launch_by_cost = std::atoi(e) != 0;
     if (const char* e = std::getenv("VLX_TILED_GRID")) ctx->tiled_grid = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("VLX_MMD_GRID")) ctx->mmd_grid = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("VLX_UNTILED_WAVES"))
         ctx->untiled_wave_cap = std::max(VLX_WAVES_PER_BLOCK, std::atoi(e));
     if (const char* e = std::getenv("VLX_SOLO_FRAC")) ctx->solo_frac = std::max(0.0, std::atof(e));
@@ -1534,6 +1536,7 @@ int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::
         a.sc.fk = fk;
         a.sc.omega2 = omega2;
         const int grid = sch.tiled ? static_cast<int>(std::min<long long>(a.sc.nitems, ctx->tiled_grid))
+                         : sch.mmd ? static_cast<int>(std::min<long long>(a.sc.nitems, ctx->mmd_grid))
                                    : static_cast<int>(untiled_waves(a.sc.nitems, ctx->untiled_wave_cap) / VLX_WAVES_PER_BLOCK);
         hipStream_t st = stream ? stream : (sch.stream < 0 ? ctx->stream : ctx->side[sch.stream % ctx->nstreams]);
         vlx_context::Pending pend{sch.entry, nullptr, nullptr};
