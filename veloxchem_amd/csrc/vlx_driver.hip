@@ -899,6 +899,9 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_LAUNCH_ORDER")) ctx->launch_by_cost = std::atoi(e) != 0;
     if (const char* e = std::getenv("VLX_TILED_GRID")) ctx->tiled_grid = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("VLX_MMD_GRID")) ctx->mmd_grid = std::max(1, std::atoi(e));
+    // dynamic LDS of a tiled block: the Boys table (static) and the tiles share the 160 KB of a CU
+    if (const char* e = std::getenv("VLX_MAX_TILE_LDS"))
+        ctx->max_tile_lds = std::max(16 * 1024, std::min(160 * 1024 - (int)(BOYS_TAB_SIZE * sizeof(double)), std::atoi(e)));
     if (const char* e = std::getenv("VLX_UNTILED_WAVES"))
         ctx->untiled_wave_cap = std::max(VLX_WAVES_PER_BLOCK, std::atoi(e));
     if (const char* e = std::getenv("VLX_SOLO_FRAC")) ctx->solo_frac = std::max(0.0, std::atof(e));
