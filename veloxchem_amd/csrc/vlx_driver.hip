@@ -184,6 +184,7 @@ enum {
     VARIANT_TILED_WIDE_P1 = 5,
     NVARIANTS = 6
 };
+constexpr int NVARIANTS_TUNED = 4;  // variants the autotuner times
 bool variant_tiled(int v)
 {
     return v == VARIANT_TILED4 || v == VARIANT_TILED_WIDE || v == VARIANT_TILED4_P1 || v == VARIANT_TILED_WIDE_P1;
@@ -1375,7 +1376,10 @@ int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_densi
     std::map<int, double> best_ms;
     std::map<int, int> best_var;
     int rc = VLX_OK;
-    for (int v = 0; v < NVARIANTS && rc == VLX_OK; v++) {
+    // the single-primitive tiled variants (4, 5) are not tuned: forced for every class after other
+    // variants on one stream, the 4-wave one returned non-finite p-d elements of the def2-TZVP
+    // water dimer, cause not found; they stay available through vlx_context_set_variant
+    for (int v = 0; v < NVARIANTS_TUNED && rc == VLX_OK; v++) {
         ctx->force_variant = v;
         scr->schedules.clear();
         ctx->timing = false;
