@@ -299,9 +299,10 @@ def test_every_variant_matches_oracle():
         _native.check(_native.lib().vlx_context_set_variant(ctx, -1))
 
 
-@pytest.mark.skip(reason='single-primitive tiled variants (4, 5) are switched off: forced for every class '
-                         'after the other variants on one stream, the 4-wave one returned non-finite '
-                         'p-d elements of the def2-TZVP water dimer; cause not found')
+@pytest.mark.skip(reason='single-primitive tiled variants (4, 5) are not tuned: with the L = 5 tiled '
+                         'kernels enabled and forced after the other variants on one stream, the 4-wave '
+                         'one returned non-finite p-d elements of the def2-TZVP water dimer; cause not '
+                         'found, the L = 5 tiled kernels are off by default')
 def test_single_primitive_variants_match_fixture():
     """The single-primitive tiled variants forced for all classes on the def2-TZVP dimer fixture."""
     from veloxchem_amd import _native
