@@ -187,9 +187,8 @@ int vlx_fock_schedule_stats(vlx_screener* scr, int ithreshold, int max_records, 
  * 2jk build at ithreshold on `d_density` under each kernel variant
  * (0 = untiled, 1 = LDS-tiled with 4-wave blocks, 2 = LDS-tiled with the
  * class's wide block, 3 = matrix-core McMurchie-Davidson, for classes with
- * la+lb+lc+ld >= 4; variants 4 and 5, tiles over single ket primitives, are
- * not tuned: forced after other variants they returned non-finite elements on
- * a def2-TZVP case), `repeats` builds each on a single stream, and keeps
+ * la+lb+lc+ld >= 4, 4 and 5 = variants 1 and 2 with one ket primitive per lane
+ * instead of runs of up to three), `repeats` builds each on a single stream, and keeps
  * the fastest variant per class for later builds with this screener (and as
  * the default of screeners created later on this context).
  * Optional outputs: *nchosen records of classes (4 ints each) and variants

@@ -269,9 +269,10 @@ def test_erf_j_vs_oracle():
 
 
 def test_every_variant_matches_oracle():
-    """Every tuned kernel variant forced for all classes (untiled, tiled 4-wave and wide blocks,
-    matrix-core MD) gives the oracle's 2jk at 1e-10 on a 4-water cluster, and the reference's
-    def2-TZVP dimer fixture (f shells)."""
+    """Every kernel variant forced for all classes (untiled, tiled 4-wave and wide blocks over
+    primitive runs and over single primitives, matrix-core MD) gives the oracle's 2jk at 1e-10 on a
+    4-water cluster and the reference's def2-TZVP dimer fixture (f shells), each variant run after
+    the others (the order in which masked lanes once carried non-finite values)."""
     from veloxchem_amd import _native
     mol = water_cluster(4, 6.0, 7)
     bas = MolecularBasis.read(mol, 'def2-svp')
@@ -286,7 +287,7 @@ def test_every_variant_matches_oracle():
             tmol, tbas = dimer('def2-tzvp')
             tz = (_screened(tmol, tbas), load('h2o.dimer.tzvp.density.npy'),
                   2 * load('h2o.dimer.tzvp.j.npy') - load('h2o.dimer.tzvp.k.npy'))
-        for v, name in enumerate(FockDriver.VARIANTS[:4]):
+        for v, name in enumerate(FockDriver.VARIANTS):
             _native.check(_native.lib().vlx_context_set_variant(ctx, v))
             g = np.asarray(drv.compute(scr, D, '2jk', 1.0, 0.0, 12))
             err = np.max(np.abs(g - ref))
@@ -295,26 +296,6 @@ def test_every_variant_matches_oracle():
             if tz is not None:  # f shells: the reference's def2-TZVP dimer fixture
                 gt = np.asarray(drv.compute(tz[0], tz[1], '2jk', 0.0, 0.0, 15))
                 assert reference_equal(gt, tz[2]), name
-    finally:
-        _native.check(_native.lib().vlx_context_set_variant(ctx, -1))
-
-
-@pytest.mark.skip(reason='single-primitive tiled variants (4, 5) are not tuned: with the L = 5 tiled '
-                         'kernels enabled and forced after the other variants on one stream, the 4-wave '
-                         'one returned non-finite p-d elements of the def2-TZVP water dimer; cause not '
-                         'found, the L = 5 tiled kernels are off by default')
-def test_single_primitive_variants_match_fixture():
-    """The single-primitive tiled variants forced for all classes on the def2-TZVP dimer fixture."""
-    from veloxchem_amd import _native
-    tmol, tbas = dimer('def2-tzvp')
-    scr = _screened(tmol, tbas)
-    ref = 2 * load('h2o.dimer.tzvp.j.npy') - load('h2o.dimer.tzvp.k.npy')
-    ctx = _native.context()
-    try:
-        for v in (4, 5):
-            _native.check(_native.lib().vlx_context_set_variant(ctx, v))
-            g = np.asarray(FockDriver().compute(scr, load('h2o.dimer.tzvp.density.npy'), '2jk', 0.0, 0.0, 15))
-            assert reference_equal(g, ref), FockDriver.VARIANTS[v]
     finally:
         _native.check(_native.lib().vlx_context_set_variant(ctx, -1))
 

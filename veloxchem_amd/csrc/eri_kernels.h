@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(VLX_BLOCK)
                                             ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz,
                                             tab, a.omega2, I);
 #pragma unroll
-            for (int x = 0; x < nsph; x++) I[x] *= s;
+            for (int x = 0; x < nsph; x++) I[x] = active ? I[x] * s : 0.0;
             int q[nb];
 #pragma unroll
             for (int jb = 0; jb < nb; jb++) q[jb] = bp.ao_b + jb * bp.str_b;
@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(VLX_BLOCK)
                                         ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz, tab,
                                         a.omega2, I);
 #pragma unroll
-        for (int x = 0; x < nsph; x++) I[x] *= s;
+        for (int x = 0; x < nsph; x++) I[x] = active ? I[x] * s : 0.0;
         int p[na], q[nb], r[nc], t[nd];
 #pragma unroll
         for (int i = 0; i < na; i++) p[i] = bp.ao_a + i * bp.str_a;
@@ -617,7 +617,10 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                                                           R(bp.abz), ket_prims + kp.prim_start + kq0, active ? kqn : 0,
                                                           R(kp.abx), R(kp.aby), R(kp.abz), tab, R(a.omega2), I);
 #pragma unroll
-            for (int x = 0; x < nsph; x++) I[x] *= s;
+            // masked lanes contribute exact zeros: multiplying their integrals by s = 0 would turn
+            // a non-finite value left in a masked lane's registers into NaN (seen in the L = 5
+            // tiled kernels, depending on what ran before them)
+            for (int x = 0; x < nsph; x++) I[x] = active ? I[x] * s : R(0);
             int p[na], q[nb];
 #pragma unroll
             for (int i = 0; i < na; i++) p[i] = bp.ao_a + i * bp.str_a;

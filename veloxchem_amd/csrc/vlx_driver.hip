@@ -184,7 +184,6 @@ enum {
     VARIANT_TILED_WIDE_P1 = 5,
     NVARIANTS = 6
 };
-constexpr int NVARIANTS_TUNED = 4;  // variants the autotuner times
 bool variant_tiled(int v)
 {
     return v == VARIANT_TILED4 || v == VARIANT_TILED_WIDE || v == VARIANT_TILED4_P1 || v == VARIANT_TILED_WIDE_P1;
@@ -196,11 +195,7 @@ struct vlx_context {
     int max_chunk = BRA_CHUNK;      // bras per work item upper bound (VLX_BRA_CHUNK)
     int min_items = 16384;          // work items a class launch aims for (VLX_MIN_ITEMS)
     int use_tiled = 1;              // LDS-tiled kernels where built (VLX_TILED)
-    // largest la+lb+lc+ld that may use the tiled kernel (VLX_TILED_LMAX).  The L = 5 tiled kernels
-    // are built but off by default: twice they returned non-finite elements depending on what ran
-    // before them (four streams, J entries of a def2-SVP cluster; single-primitive ket entries on one
-    // stream, p-d elements of a def2-TZVP dimer); cause not found
-    int tiled_lmax = 4;
+    int tiled_lmax = 5;             // largest la+lb+lc+ld that may use the tiled kernel (VLX_TILED_LMAX)
     int force_variant = -1;         // autotuning: every class runs this variant (VARIANT_*)
     std::map<int, int> tuned_variant;  // class entry -> VARIANT_* of the last autotune (default for new screeners)
     // atoms per group of the tiled kernels (VLX_GROUP_ATOMS); 0: the largest of 12, 8, 6, 4, 3,
@@ -1380,9 +1375,7 @@ int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_densi
     std::map<int, double> best_ms;
     std::map<int, int> best_var;
     int rc = VLX_OK;
-    // the single-primitive tiled variants (4, 5) are not tuned: they were added for the L = 5 tiled
-    // classes, which are off by default; they stay available through vlx_context_set_variant
-    for (int v = 0; v < NVARIANTS_TUNED && rc == VLX_OK; v++) {
+    for (int v = 0; v < NVARIANTS && rc == VLX_OK; v++) {
         ctx->force_variant = v;
         scr->schedules.clear();
         ctx->timing = false;
