@@ -1,17 +1,23 @@
 """Benchmark of the two-electron J+K Fock build on one MI355X.
 
-Workload (one "step" = one build): seeded 32-water cluster in a 10 Angstrom
-box (seed 32), def2-SVP (768 AOs), FP64 G = 2J - K ('2jk', exchange factor
-1, omega 0, ithreshold 12) on the SAD guess density.  D and G stay resident
-in HBM; the timed region is K builds between stream synchronisations.  After
-the timed builds, the full RHF SCF of the same cluster (L2-DIIS, gradient
-norm 1e-8) is run once and its wall time reported beside the build rate.
+Headline workload (one "step" = one build): seeded 64-water cluster in a
+12.8 Angstrom box (seed 64), def2-TZVP (2752 AOs, f shells on O), FP64
+G = 2J - K ('2jk', exchange factor 1, omega 0, ithreshold 12) on the SAD
+guess density, after a per-class kernel-variant autotuner pass.  D and G stay
+resident in HBM; the timed region is K builds between stream
+synchronisations.  value = FP64 builds/s.
 
-Printed: one JSON line with value = builds/s, the roofline of the dominant
-class kernel (FP64 vector-ALU bound; per-launch algorithmic FLOPs over the
-HIP-event kernel time measured inside the timed region), and a CPU baseline
-(the project's C oracle on the host cores over a bounded strided sample of
-the same build, extrapolated to builds/s).
+Printed: one JSON line with the roofline of the dominant class kernel (FP64
+vector-ALU bound; per-launch algorithmic FLOPs of the reference scheme over
+the HIP-event kernel time measured inside the timed region), the mixed
+precision sweep on the same density, and a CPU baseline (the reference's own
+CFockDriver::compute, built from its sources into oracle/_ref, on the host
+cores for one full build of the same workload).  Extra keys: the 32-water
+def2-SVP build and its RHF SCF, and the batched many-molecule workload.
+
+Algorithmic FLOPs per class (reference scheme): per screened contracted
+quartet nprim_bra nprim_ket (V + 53 + 4 L) + C + 12 n_sph, with V and C of
+the reference's recursions (tests/golden/ref_scheme_flops.json).
 """
 
 import argparse
@@ -27,16 +33,62 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 FP64_VALU_PEAK_TFLOPS = 78.6
-# HBM bytes per launch per kernel, from FETCH_SIZE/WRITE_SIZE passes scaled by a copy calibration
-TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'fock_build_water32_svp_traffic.json')
+# HBM bytes per launch per kernel and workload, from FETCH_SIZE/WRITE_SIZE passes scaled by a copy
+# calibration (profiles/*_traffic.json)
+TRAFFIC_FILES = {'water64_tzvp': os.path.join(ROOT, 'profiles', 'fock_build_water64_tzvp_traffic.json'),
+                 'water32_svp': os.path.join(ROOT, 'profiles', 'fock_build_water32_svp_traffic.json')}
+REF_FLOPS_FILE = os.path.join(ROOT, 'tests', 'golden', 'ref_scheme_flops.json')
 
 
-def kernel_traffic(name):
+def kernel_traffic(workload, name):
     try:
-        with open(TRAFFIC_FILE) as fh:
+        with open(TRAFFIC_FILES[workload]) as fh:
             return json.load(fh)['bytes_per_launch'].get(name)
     except (OSError, ValueError, KeyError):
         return None
+
+
+def ref_scheme_key(cls):
+    """Class (la lb|lc ld) of this build (la >= lb, lc >= ld) in the reference's orientation: pairs
+    (l_first <= l_second), the lexicographically lower pair on the bra side, as its triangular
+    loops over angular-momentum-ordered pair blocks run them."""
+    la, lb, lc, ld = cls
+    p1, p2 = (lb, la), (ld, lc)
+    lo, hi = min(p1, p2), max(p1, p2)
+    return '%d%d%d%d' % (lo + hi)
+
+
+def ref_scheme_flops(work):
+    """{class: FLOPs of one 2jk build} from the per-class quartet counts of Screener.work() and the
+    reference scheme's V (per primitive quartet) and C (per contracted quartet):
+    prim_quartets (V + 53 + 4 L) + quartets (C + 12 n_sph)."""
+    with open(REF_FLOPS_FILE) as fh:
+        table = json.load(fh)['classes']
+    out = {}
+    for cls, nq, npq, _ in work:
+        vc = table[ref_scheme_key(cls)]
+        L = sum(cls)
+        nsph = np.prod([2 * l + 1 for l in cls])
+        out[cls] = npq * (vc['V'] + 53 + 4 * L) + nq * (vc['C'] + 12 * nsph)
+    return out
+
+
+def roofline(scr, ithr, times, ms_per_step, workload):
+    """Dominant class kernel (largest summed HIP-event time in the timed region) with its
+    reference-scheme FLOPs per launch over its average launch time, and the whole build's rate."""
+    from veloxchem_amd import _native
+    flops = ref_scheme_flops(scr.work(ithr))
+    dom = max(times, key=lambda c: times[c][0])
+    dom_ms = times[dom][0] / times[dom][1]
+    achieved = flops[dom] / (dom_ms * 1e-3) / 1e12
+    kernel = _native.class_kernel(scr, ithr, dom)
+    build = sum(flops.values()) / (ms_per_step * 1e-3) / 1e12
+    return {'bound': 'valu', 'kernel': kernel, 'class': '%d%d%d%d' % dom, 'achieved': achieved,
+            'peak': FP64_VALU_PEAK_TFLOPS, 'unit': 'TFLOP/s', 'frac': achieved / FP64_VALU_PEAK_TFLOPS,
+            'traffic': kernel_traffic(workload, kernel), 'kernel_ms': dom_ms,
+            'flops_per_launch': flops[dom], 'kernel_share_of_build': times[dom][0] / sum(v[0] for v in times.values()),
+            'build_achieved': build, 'build_frac': build / FP64_VALU_PEAK_TFLOPS,
+            'build_flops': sum(flops.values())}
 
 
 def build_workload(nmol, box, seed, basis):
@@ -88,8 +140,8 @@ def measure_builds(scr, dm, ithr, steps, warmup, taus, label=b'2jk', precision='
     def one_build():
         _native.check(lib.vlx_fock_compute_device(ctx, scr.handle, d_dm.ptr, label, 1.0, 0.0, ithr, d_g.ptr))
 
-    def timed(k):
-        for _ in range(max(1, warmup)):
+    def timed(k, w=warmup):
+        for _ in range(max(1, w)):
             one_build()
         _native.check(lib.vlx_context_synchronize(ctx))
         t0 = time.perf_counter()
@@ -113,9 +165,10 @@ def measure_builds(scr, dm, ithr, steps, warmup, taus, label=b'2jk', precision='
             ms64 = timed(steps)
             g64 = d_g.download()
         sweep = []
+        sweep_steps = min(steps, 3)  # the sweep's rates: three builds per threshold after one warm-up
         for t in taus:
             _native.set_precision('mixed', t)
-            ms_m = timed(steps)
+            ms_m = timed(sweep_steps, 1)
             gm = d_g.download()
             k32 = _native.last_fp32_bin_sum()
             p64, p32 = scr.precision_split(ithr, k32)
@@ -123,6 +176,7 @@ def measure_builds(scr, dm, ithr, steps, warmup, taus, label=b'2jk', precision='
                      'fp32_prim_fraction': p32 / max(1, p64 + p32), 'max_abs_dG': float(np.max(np.abs(gm - g64))),
                      'dE2_vs_fp64': float(np.sum(dm * (gm - g64)))}
             entry['speedup_vs_fp64'] = ms64 / ms_m
+            entry['steps'] = sweep_steps
             sweep.append(entry)
     finally:
         _native.set_precision('fp64')
@@ -131,18 +185,18 @@ def measure_builds(scr, dm, ithr, steps, warmup, taus, label=b'2jk', precision='
     return ms, times, g, sweep
 
 
-def cpu_baseline(mol, bas, dm, ithr, target_s, basis_name, g_gpu):
-    """The reference's own C++ Fock path (oracle/_ref, CFockDriver::compute) on the host cores for
-    one full build, with its elementwise difference from the GPU build of the same D; where that
-    harness is not built, the oracle over every k-th bra pair, extrapolated."""
+def cpu_baseline(mol, bas, dm, ithr, target_s, basis_name, g_gpu, repeats=1):
+    """The reference's own C++ Fock path (oracle/_ref, CFockDriver::compute) on the host cores,
+    median of `repeats` full builds, with its elementwise difference from the GPU build of the same
+    D; where that harness is not built, the oracle over every k-th bra pair, extrapolated."""
     from oracle import refharness
     cores = min(os.cpu_count() or 1, 16)
     if refharness.available():
-        g_ref, info = refharness.fock(mol, basis_name, dm, '2jk', 1.0, 0.0, ithr, repeats=1, nthreads=cores)
+        g_ref, info = refharness.fock(mol, basis_name, dm, '2jk', 1.0, 0.0, ithr, repeats=repeats, nthreads=cores)
         return {'value': 1.0 / info['median_s'], 'unit': 'builds/s', 'cores': cores, 'kind': 'reference',
-                'sample': f'one full 2jk build of the same workload by the reference CFockDriver::compute '
-                          f'(OpenMP, {cores} threads): {info["median_s"]:.2f} s; its screener '
-                          f'{info["screener_s"]:.2f} s not included',
+                'sample': f'{repeats} full 2jk build(s) of the same workload by the reference '
+                          f'CFockDriver::compute (OpenMP, {cores} threads), median {info["median_s"]:.2f} s; '
+                          f'its screener ({info["screener_s"]:.2f} s) not included',
                 'max_abs_dG_gpu_vs_reference': float(np.max(np.abs(g_ref - g_gpu)))}
     from oracle import oracle as orc
     ob = orc.basis_from(bas, mol)
@@ -163,25 +217,45 @@ def cpu_baseline(mol, bas, dm, ithr, target_s, basis_name, g_gpu):
                       f'{dt:.1f} s), extrapolated by quartet count'}
 
 
-def tzvp_builds(args, taus):
-    """Mixed-precision workload: seeded 64-water cluster (12.8 A box, seed 64), def2-TZVP
-    (2752 AOs, f shells): FP64 build rate and the mixed sweep with its error against FP64."""
-    mol, bas, scr, dm, t_screen = build_workload(64, 12.8, 64, 'def2-tzvp')
-    t_tune = 0.0
-    if not args.no_tzvp_autotune:
-        from veloxchem_amd import FockDriver
+def workload_builds(nmol, box, seed, basis, ithr, steps, warmup, taus, autotune_repeats, workload_key):
+    """FP64 builds of one seeded water cluster after its own autotuner pass: rate, per-class kernel
+    times and the roofline of the dominant class kernel (timed region, four streams), the same
+    kernel alone on one stream, and the mixed sweep with its error against FP64."""
+    from veloxchem_amd import FockDriver, _native
+    mol, bas, scr, dm, t_screen = build_workload(nmol, box, seed, basis)
+    n = bas.get_dimensions_of_basis()
+    tuned, t_tune = None, 0.0
+    if autotune_repeats > 0:
         t0 = time.perf_counter()
-        FockDriver()._autotune(scr, dm, args.ithreshold, 1)
+        tuned = FockDriver()._autotune(scr, dm, ithr, autotune_repeats)
         t_tune = time.perf_counter() - t0
-    ms, times, _, sweep = measure_builds(scr, dm, args.ithreshold, args.tzvp_steps, 1, taus)
-    work = scr.work(args.ithreshold)
-    flops = sum(w[3] for w in work)
-    return {'workload': '64-water cluster (seed 64, 12.8 A box), def2-TZVP, '
-                        f'{bas.get_dimensions_of_basis()} AOs, 2jk on the SAD density, ithreshold {args.ithreshold}',
-            'fp64_builds_per_s': 1e3 / ms, 'fp64_ms_per_step': ms, 'steps': args.tzvp_steps,
-            'build_achieved_tflops': flops / (ms * 1e-3) / 1e12, 'screener_setup_s': round(t_screen, 3),
-            'autotune_s': round(t_tune, 1),
-            'shell_quartets': int(sum(w[1] for w in work)), 'mixed_sweep': sweep}
+    ms, times, g, sweep = measure_builds(scr, dm, ithr, steps, warmup, taus)
+    work = scr.work(ithr)
+    rf = roofline(scr, ithr, times, ms, workload_key)
+    # the classes timed alone: one stream serialises the class kernels (in the timed region four
+    # streams share the chip, so each launch's time includes its neighbours)
+    lib, ctx = _native.lib(), _native.context(0)
+    _native.check(lib.vlx_context_set_streams(ctx, 1))
+    try:
+        ms1, times1, _, _ = measure_builds(scr, dm, ithr, 1, 1, [])
+    finally:
+        _native.check(lib.vlx_context_set_streams(ctx, max(1, min(4, int(os.environ.get('VLX_STREAMS', 4))))))
+    rf1 = roofline(scr, ithr, times1, ms1, workload_key)
+    out = {'workload': f'{nmol}-water cluster (seed {seed}, {box} A box), {basis}, {n} AOs, 2jk FP64 build on the '
+                       f'SAD density, ithreshold {ithr}',
+           'nao': n, 'builds_per_s': 1e3 / ms, 'ms_per_step': ms, 'steps': steps, 'warmup': warmup,
+           'shell_quartets': int(sum(w[1] for w in work)), 'primitive_quartets': int(sum(w[2] for w in work)),
+           'screener_setup_s': round(t_screen, 3), 'autotune_s': round(t_tune, 2),
+           'roofline': rf,
+           'roofline_single_stream': {k: rf1[k] for k in ('kernel', 'class', 'achieved', 'frac', 'kernel_ms')}
+           | {'build_ms_one_stream': ms1},
+           'class_ms': {'%d%d%d%d' % c: round(v[0] / v[1], 4) for c, v in sorted(times.items())},
+           'checksum': float(np.sum(g * dm))}
+    if tuned:
+        out['autotune'] = {'%d%d%d%d' % c: v for c, v in sorted(tuned.items())}
+    if sweep:
+        out['mixed_sweep'] = sweep
+    return out, (mol, bas, scr, dm, g)
 
 
 def batched_clusters(args):
@@ -257,27 +331,22 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--nmol', type=int, default=32)
-    ap.add_argument('--box', type=float, default=10.0)
-    ap.add_argument('--seed', type=int, default=32)
-    ap.add_argument('--basis', default='def2-svp')
     ap.add_argument('--ithreshold', type=int, default=12)
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    ap.add_argument('--cpu-repeats', type=int, default=1,
+                    help='reference CPU builds of the headline workload (median); about 90 s each on 16 cores')
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--no-scf', action='store_true')
     ap.add_argument('--no-autotune', action='store_true', help='skip the per-class kernel-variant autotuner')
-    ap.add_argument('--precision', default='fp64', choices=['fp64', 'mixed'],
-                    help='precision of the main timed builds (value)')
-    ap.add_argument('--tau', type=float, default=1e-6, help='mixed-precision threshold of --precision mixed')
     ap.add_argument('--mixed', default='1e-6,1e-8,1e-10',
                     help='Schwarz x max|D| thresholds of the mixed-precision sweep (comma list, empty: none)')
     ap.add_argument('--scf-conv', type=float, default=1.0e-8)
-    ap.add_argument('--label', default='2jk', choices=['2jk', 'j', 'k'])
-    ap.add_argument('--no-tzvp', action='store_true',
-                    help='skip the 64-water def2-TZVP FP64 build and mixed-precision sweep')
-    ap.add_argument('--tzvp-steps', type=int, default=2)
-    ap.add_argument('--no-tzvp-autotune', action='store_true',
-                    help='run the def2-TZVP builds with the default kernel variants (no autotuner pass)')
+    ap.add_argument('--only-headline', action='store_true',
+                    help='the 64-water def2-TZVP builds only (profiling runs): no sweep, SCF, extras or CPU baseline')
+    ap.add_argument('--svp-only', action='store_true',
+                    help='the 32-water def2-SVP builds only (profiling runs)')
+    ap.add_argument('--no-svp', action='store_true', help='skip the 32-water def2-SVP builds and SCF')
+    ap.add_argument('--no-scf', action='store_true')
+    ap.add_argument('--svp-steps', type=int, default=10)
     ap.add_argument('--no-batch', action='store_true',
                     help='skip the batched workload of 256 small water clusters')
     ap.add_argument('--batch-clusters', type=int, default=256)
@@ -290,115 +359,68 @@ def main():
 
     from veloxchem_amd import _native
     _native.lib()
-    mol, bas, scr, dm, t_screen = build_workload(args.nmol, args.box, args.seed, args.basis)
-    n = bas.get_dimensions_of_basis()
-
-    tuned, t_tune = None, 0.0
-    if not args.no_autotune:
-        from veloxchem_amd import FockDriver
-        t0 = time.perf_counter()
-        tuned = FockDriver()._autotune(scr, dm, args.ithreshold, repeats=2)
-        t_tune = time.perf_counter() - t0
-
     taus = [float(x) for x in args.mixed.split(',') if x.strip()]
-    ms_per_step, times, g, sweep = measure_builds(scr, dm, args.ithreshold, args.steps, args.warmup, taus,
-                                                  args.label.encode(), args.precision,
-                                                  args.tau if args.precision == 'mixed' else 0.0)
-    work = scr.work(args.ithreshold)
-    flops_build = sum(w[3] for w in work)
-    by_class = {w[0]: w for w in work}
-    dom = max(times, key=lambda c: times[c][0])
-    dom_ms = times[dom][0] / times[dom][1]
-    dom_flops = by_class[dom][3]
-    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
-    dom_kernel = _native.class_kernel(scr, args.ithreshold, dom)
-    build_tflops = flops_build / (ms_per_step * 1e-3) / 1e12
-
-    # the same kernel timed alone: one stream serialises the class kernels (in the timed region
-    # above four streams share the chip, so each kernel's launch time includes its neighbours)
-    lib = _native.lib()
-    ctx = _native.context(0)
-    _native.check(lib.vlx_context_set_streams(ctx, 1))
-    try:
-        ms1, times1, _, _ = measure_builds(scr, dm, args.ithreshold, 2, 1, [], args.label.encode())
-    finally:
-        _native.check(lib.vlx_context_set_streams(ctx, max(1, min(4, int(os.environ.get('VLX_STREAMS', 4))))))
-    dom1 = max(times1, key=lambda c: times1[c][0])
-    dom1_ms = times1[dom1][0] / times1[dom1][1]
-    achieved1 = by_class[dom1][3] / (dom1_ms * 1e-3) / 1e12
-
+    if args.only_headline or args.svp_only:
+        taus = []
+    atr = 0 if args.no_autotune else 1
+    if args.svp_only:
+        svp, _ = workload_builds(32, 10.0, 32, 'def2-svp', args.ithreshold, args.steps, args.warmup, [],
+                                 0 if args.no_autotune else 2, 'water32_svp')
+        print(json.dumps({'svp32': svp}))
+        return
+    head, (mol, bas, scr, dm, g) = workload_builds(64, 12.8, 64, 'def2-tzvp', args.ithreshold, args.steps,
+                                                   args.warmup, taus, atr, 'water64_tzvp')
     result = {
         'metric': 'J+K Fock builds/s + SCF time, mixed vs FP64 on GPU, at |\u0394E|<=1e-6 Eh',
-        'mode': 'fp64 2jk build (value); fp64 RHF SCF (scf)',
-        'value': 1e3 / ms_per_step,
+        'mode': 'fp64 2jk build of the 64-water def2-TZVP cluster (value); mixed sweep on the same density',
+        'value': head['builds_per_s'],
         'unit': 'builds/s',
         'n_gpus': 1,
         'steps': args.steps,
         'warmup': args.warmup,
-        'ms_per_step': ms_per_step,
+        'ms_per_step': head['ms_per_step'],
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': 'f64',
         'data': 'synthetic (seeded water cluster, SAD guess density)',
         'config': {
-            'workload': f'{args.nmol}-water cluster (seed {args.seed}, {args.box} A box), {args.basis}, '
-                        f'{n} AOs, 2jk FP64 build on the SAD density, ithreshold {args.ithreshold}',
-            'nao': n,
-            'shell_quartets': int(sum(w[1] for w in work)),
-            'primitive_quartets': int(sum(w[2] for w in work)),
-            'screener_setup_s': round(t_screen, 3),
-            'autotune_s': round(t_tune, 3),
+            'workload': head['workload'],
+            'nao': head['nao'],
+            'shell_quartets': head['shell_quartets'],
+            'primitive_quartets': head['primitive_quartets'],
+            'screener_setup_s': head['screener_setup_s'],
+            'autotune_s': head['autotune_s'],
         },
-        'roofline': {
-            'bound': 'valu',
-            'kernel': dom_kernel,
-            'achieved': achieved,
-            'peak': FP64_VALU_PEAK_TFLOPS,
-            'unit': 'TFLOP/s',
-            'frac': achieved / FP64_VALU_PEAK_TFLOPS,
-            'traffic': kernel_traffic(dom_kernel),
-            'kernel_ms': dom_ms,
-            'kernel_share_of_build': times[dom][0] / sum(v[0] for v in times.values()),
-            'build_achieved': build_tflops,
-            'build_frac': build_tflops / FP64_VALU_PEAK_TFLOPS,
-        },
-        'roofline_single_stream': {
-            'kernel': _native.class_kernel(scr, args.ithreshold, dom1),
-            'achieved': achieved1,
-            'peak': FP64_VALU_PEAK_TFLOPS,
-            'unit': 'TFLOP/s',
-            'frac': achieved1 / FP64_VALU_PEAK_TFLOPS,
-            'kernel_ms': dom1_ms,
-            'build_ms_one_stream': ms1,
-        },
-        'class_ms': {'%d%d%d%d' % c: round(v[0] / v[1], 4) for c, v in sorted(times.items())},
-        'checksum': float(np.sum(g * dm)),
+        'roofline': head['roofline'],
     }
-    if tuned:
-        result['autotune'] = {'%d%d%d%d' % c: v for c, v in sorted(tuned.items())}
-    if sweep:
-        result['mixed_sweep'] = sweep
-    if not args.no_scf:
-        # the dense-algebra libraries load their kernels for each matrix size on first use: one
-        # untimed SCF of the same molecule first (every run below builds its own screener)
-        run_scf(mol, bas, 1.0e-4)
-        result['scf'] = run_scf(mol, bas, args.scf_conv)
-        inc = run_scf(mol, bas, args.scf_conv, incremental=True)
-        inc['dE_vs_fp64'] = inc['energy'] - result['scf']['energy']
-        result['scf_incremental'] = inc
-        if taus:
-            tau = max(taus)
-            for key, incr in (('scf_mixed', False), ('scf_mixed_incremental', True)):
-                mixed = run_scf(mol, bas, args.scf_conv, 'mixed', tau, incremental=incr)
-                mixed['dE_vs_fp64'] = mixed['energy'] - result['scf']['energy']
-                result[key] = mixed
-    if not args.no_tzvp:
-        result['tzvp64'] = tzvp_builds(args, taus)
-    if not args.no_batch:
-        result['batched_clusters'] = batched_clusters(args)
-    if not args.no_cpu_baseline:
-        result['cpu_baseline'] = cpu_baseline(mol, bas, dm, args.ithreshold, args.cpu_seconds, args.basis, g)
+    for k in ('roofline_single_stream', 'class_ms', 'checksum', 'autotune', 'mixed_sweep'):
+        if k in head:
+            result[k] = head[k]
+    if not args.only_headline:
+        if not args.no_svp:
+            svp, (smol, sbas, _, _, _) = workload_builds(32, 10.0, 32, 'def2-svp', args.ithreshold, args.svp_steps, 2,
+                                                         taus, 0 if args.no_autotune else 2, 'water32_svp')
+            result['svp32'] = svp
+            if not args.no_scf:
+                # the dense-algebra libraries load their kernels for each matrix size on first use: one
+                # untimed SCF of the same molecule first (every run below builds its own screener)
+                run_scf(smol, sbas, 1.0e-4)
+                result['scf'] = run_scf(smol, sbas, args.scf_conv)
+                inc = run_scf(smol, sbas, args.scf_conv, incremental=True)
+                inc['dE_vs_fp64'] = inc['energy'] - result['scf']['energy']
+                result['scf_incremental'] = inc
+                if taus:
+                    tau = max(taus)
+                    for key, incr in (('scf_mixed', False), ('scf_mixed_incremental', True)):
+                        mixed = run_scf(smol, sbas, args.scf_conv, 'mixed', tau, incremental=incr)
+                        mixed['dE_vs_fp64'] = mixed['energy'] - result['scf']['energy']
+                        result[key] = mixed
+        if not args.no_batch:
+            result['batched_clusters'] = batched_clusters(args)
+        if not args.no_cpu_baseline:
+            result['cpu_baseline'] = cpu_baseline(mol, bas, dm, args.ithreshold, args.cpu_seconds, 'def2-tzvp', g,
+                                                  repeats=args.cpu_repeats)
     print(json.dumps(result))
 
 

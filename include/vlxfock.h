@@ -62,6 +62,11 @@ int vlx_context_last_fp32_bin_sum(vlx_context* ctx, int* k32);
 /* Primitive slots of one build run in FP64 and in FP32 at bin-sum threshold k32. */
 int vlx_fock_precision_split(vlx_screener* scr, int ithreshold, int k32, long long* prim_slots_fp64,
                              long long* prim_slots_fp32);
+/* The FP32 primitive slots of the same split per angular class: *nrecords
+ * records of (la, lb, lc, ld) (4 ints each) and their FP32 primitive slots;
+ * at most max_records are written. */
+int vlx_fock_precision_classes(vlx_screener* scr, int ithreshold, int k32, int max_records, int* nrecords,
+                               int* classes, long long* prim_slots_fp32);
 
 /*
  * T4CScreener::partition(basis, molecule, "eri")
@@ -192,10 +197,16 @@ int vlx_fock_schedule_stats(vlx_screener* scr, int ithreshold, int max_records, 
  * the fastest variant per class for later builds with this screener (and as
  * the default of screeners created later on this context).
  * Optional outputs: *nchosen records of classes (4 ints each) and variants
- * (arrays of at least 64 classes).  Results do not depend on the variant.
+ * (arrays of at least 64 classes).  Results do not depend on the variant;
+ * a variant whose warm-up G disagrees with the untiled one is not chosen.
  */
 int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_density, int ithreshold, int repeats,
                       int* nchosen, int* classes, int* variants);
+/* Variants the last vlx_fock_autotune on this context rejected (bit v set for
+ * variant v): the G of a variant's warm-up build is checked on the host against
+ * the untiled variant's (non-finite elements, or a difference above 1e-10), and
+ * a rejected variant is never chosen.  No reference counterpart. */
+int vlx_fock_autotune_rejected(vlx_context* ctx, int* mask);
 
 /* Kernel (template name as profilers report it) that a class (la lb|lc ld)
  * runs with at ithreshold, written to name[0..len). */

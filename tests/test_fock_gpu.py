@@ -4,6 +4,8 @@ FP64 bar: elementwise 1e-10 Hartree against the oracle on the same basis,
 geometry and density; the reference's own rule (rtol = atol = 1e-12,
 src/math/SubMatrix.cpp:105-115) against its fixtures.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -287,17 +289,24 @@ def test_every_variant_matches_oracle():
             tmol, tbas = dimer('def2-tzvp')
             tz = (_screened(tmol, tbas), load('h2o.dimer.tzvp.density.npy'),
                   2 * load('h2o.dimer.tzvp.j.npy') - load('h2o.dimer.tzvp.k.npy'))
-        for v, name in enumerate(FockDriver.VARIANTS):
-            _native.check(_native.lib().vlx_context_set_variant(ctx, v))
-            g = np.asarray(drv.compute(scr, D, '2jk', 1.0, 0.0, 12))
-            err = np.max(np.abs(g - ref))
-            print(f'variant {name}: max|dG| {err:.2e}', flush=True)
-            assert err < TOL, name
-            if tz is not None:  # f shells: the reference's def2-TZVP dimer fixture
-                gt = np.asarray(drv.compute(tz[0], tz[1], '2jk', 0.0, 0.0, 15))
-                assert reference_equal(gt, tz[2]), name
+        # four side streams, then the whole sequence again on one stream (the configuration in
+        # which the def2-TZVP p-d elements once came out non-finite)
+        for nstreams in (4, 1):
+            _native.check(_native.lib().vlx_context_set_streams(ctx, nstreams))
+            for v, name in enumerate(FockDriver.VARIANTS):
+                _native.check(_native.lib().vlx_context_set_variant(ctx, v))
+                g = np.asarray(drv.compute(scr, D, '2jk', 1.0, 0.0, 12))
+                err = np.max(np.abs(g - ref))
+                print(f'{nstreams} streams, variant {name}: max|dG| {err:.2e}', flush=True)
+                assert np.isfinite(g).all(), name
+                assert err < TOL, name
+                if tz is not None:  # f shells: the reference's def2-TZVP dimer fixture
+                    gt = np.asarray(drv.compute(tz[0], tz[1], '2jk', 0.0, 0.0, 15))
+                    assert np.isfinite(gt).all(), name
+                    assert reference_equal(gt, tz[2]), name
     finally:
         _native.check(_native.lib().vlx_context_set_variant(ctx, -1))
+        _native.check(_native.lib().vlx_context_set_streams(ctx, 4))
 
 
 def test_batch_matches_single_builds():
@@ -337,6 +346,11 @@ def test_autotune_choices_keep_parity():
     drv = FockDriver()
     choice = drv._autotune(scr, D, 12, repeats=1)
     assert choice and set(choice.values()) <= set(FockDriver.VARIANTS)
+    # every variant's warm-up G agreed with the untiled one, so none was rejected
+    from veloxchem_amd import _native
+    mask = ctypes.c_int(-1)
+    _native.check(_native.lib().vlx_fock_autotune_rejected(_native.context(), ctypes.byref(mask)))
+    assert mask.value == 0
     g = drv.compute(scr, D, '2jk', 1.0, 0.0, 12)
     ref, _ = orc.fock(orc.basis_from(bas, mol), D, '2jk', 1.0, 12, 0)
     assert np.max(np.abs(np.asarray(g) - ref)) < TOL

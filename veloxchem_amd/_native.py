@@ -80,11 +80,14 @@ def lib():
         L.vlx_eri_quartets.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_double_p]
         L.vlx_fock_autotune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                         _c_int_p, _c_int_p, _c_int_p]
+        L.vlx_fock_autotune_rejected.argtypes = [ctypes.c_void_p, _c_int_p]
         L.vlx_fock_class_kernel.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.vlx_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         L.vlx_context_last_fp32_bin_sum.argtypes = [ctypes.c_void_p, _c_int_p]
         L.vlx_fock_precision_split.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_ll_p, _c_ll_p]
+        L.vlx_fock_precision_classes.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p,
+                                                 _c_int_p, _c_ll_p]
         L.vlx_one_electron.argtypes = [ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_double_p, _c_double_p,
                                        _c_double_p, _c_int_p, _c_int_p, ctypes.c_int, ctypes.c_int, _c_double_p,
                                        _c_double_p, _c_double_p, _c_double_p, _c_double_p]
@@ -93,10 +96,12 @@ def lib():
 
 
 def header_functions():
-    """Names of the functions declared in include/vlxfock.h."""
-    text = HEADER_PATH.read_text()
-    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
-    return sorted(set(re.findall(r'\b(vlx_[a-z_]+)\s*\(', text)))
+    """Names of the functions declared in the library's headers (include/*.h)."""
+    names = set()
+    for path in sorted(HEADER_PATH.parent.glob('*.h')):
+        text = re.sub(r'/\*.*?\*/', '', path.read_text(), flags=re.S)
+        names.update(re.findall(r'\b(vlx_[a-z0-9_]+)\s*\(', text))
+    return sorted(names)
 
 
 def check(rc):

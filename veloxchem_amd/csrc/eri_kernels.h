@@ -660,7 +660,7 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                                     vcd[k][l] = fma(I[((i * nb + jb) * nc + k) * nd + l], dab, vcd[k][l]);
                         }
                     if constexpr (jcd_in_regs) {
-                        // inactive lanes add zeros (I is scaled by 0)
+                        // inactive lanes add zeros (their integrals were replaced by exact zeros)
 #pragma unroll
                         for (int k = 0; k < nc; k++)
 #pragma unroll

@@ -29,6 +29,7 @@ It also writes ``gen/dispatch.inc`` with the table of launchers.
 Usage: python gen_eri.py OUTDIR LMAX
 """
 
+import importlib.util
 import math
 import sys
 from pathlib import Path
@@ -46,20 +47,13 @@ def nsph(lang):
     return 2 * lang + 1
 
 
-# Cartesian -> spherical factors: sph[l][m] = [(cart_index, coefficient)].
-_F3 = 2.0 * math.sqrt(3.0)
-_G5 = math.sqrt(2.5)
-_G15 = 2.0 * math.sqrt(15.0)
-_G3 = math.sqrt(1.5)
-SPH = {
-    0: [[(0, 1.0)]],
-    1: [[(1, 1.0)], [(2, 1.0)], [(0, 1.0)]],
-    2: [[(1, _F3)], [(4, _F3)], [(0, -1.0), (3, -1.0), (5, 2.0)], [(2, _F3)],
-        [(0, 0.5 * _F3), (3, -0.5 * _F3)]],
-    3: [[(1, 3.0 * _G5), (6, -_G5)], [(4, _G15)], [(8, 4.0 * _G3), (1, -_G3), (6, -_G3)],
-        [(9, 2.0), (2, -3.0), (7, -3.0)], [(5, 4.0 * _G3), (0, -_G3), (3, -_G3)],
-        [(2, 0.5 * _G15), (7, -0.5 * _G15)], [(0, _G5), (3, -3.0 * _G5)]],
-}
+# Cartesian -> spherical factors: sph[l][m] = [(cart_index, coefficient)] (veloxchem_amd/sphericals.py)
+# (loaded by file path: the package itself needs the built library)
+_spec = importlib.util.spec_from_file_location('_vlx_sphericals', Path(__file__).resolve().parents[1] / 'sphericals.py')
+_sph = importlib.util.module_from_spec(_spec)
+_spec.loader.exec_module(_sph)
+SPH = _sph.SPH
+
 
 AX = 'xyz'
 

@@ -115,6 +115,8 @@ struct DevView {
 
 }  // namespace
 
+void vlx_set_last_error(const char* msg) { g_last_error = msg ? msg : ""; }
+
 struct PairSet {
     int la = 0, lb = 0;
     std::vector<PairInfo> pairs;
@@ -198,6 +200,7 @@ struct vlx_context {
     int tiled_lmax = 5;             // largest la+lb+lc+ld that may use the tiled kernel (VLX_TILED_LMAX)
     int force_variant = -1;         // autotuning: every class runs this variant (VARIANT_*)
     std::map<int, int> tuned_variant;  // class entry -> VARIANT_* of the last autotune (default for new screeners)
+    int autotune_rejected = 0;      // bit v: variant v's G failed the last autotune's check (never chosen)
     // atoms per group of the tiled kernels (VLX_GROUP_ATOMS); 0: the largest of 12, 8, 6, 4, 3,
     // 2, 1 whose (ps|ss) tiles fit max_tile_lds (12 for def2-SVP waters, 8 for def2-TZVP)
     int group_atoms = 0;
@@ -989,6 +992,28 @@ int vlx_fock_precision_split(vlx_screener* scr, int ithreshold, int k32, long lo
     return VLX_OK;
 }
 
+int vlx_fock_precision_classes(vlx_screener* scr, int ithreshold, int k32, int max_records, int* nrecords,
+                               int* classes, long long* prim_slots_fp32)
+{
+    if (!scr || !nrecords) return fail(VLX_ERR_ARG, "NULL argument");
+    if (k32 < 0) return fail(VLX_ERR_ARG, "bin-sum threshold must be >= 0");
+    std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
+    int rc = get_schedules(scr, ithreshold, std::min(k32, NO_FP32), &list);
+    if (rc) return rc;
+    *nrecords = static_cast<int>(list->size());
+    for (int r = 0; r < std::min(max_records, *nrecords); r++) {
+        const auto& ce = g_class_table[(*list)[r]->entry];
+        if (classes) {
+            classes[4 * r] = ce.la;
+            classes[4 * r + 1] = ce.lb;
+            classes[4 * r + 2] = ce.lc;
+            classes[4 * r + 3] = ce.ld;
+        }
+        if (prim_slots_fp32) prim_slots_fp32[r] = (*list)[r]->prim_slots32;
+    }
+    return VLX_OK;
+}
+
 int vlx_context_synchronize(vlx_context* ctx)
 {
     if (!ctx) return fail(VLX_ERR_ARG, "context is NULL");
@@ -1375,6 +1400,11 @@ int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_densi
     std::map<int, double> best_ms;
     std::map<int, int> best_var;
     int rc = VLX_OK;
+    // the warm-up G of every variant is checked against the untiled variant's (variant 0, run
+    // first): a variant whose G has a non-finite element or differs by more than 1e-10 is never
+    // chosen (the choice is made on timing alone otherwise)
+    std::vector<double> g_ref(nn), g_var(nn);
+    ctx->autotune_rejected = 0;
     for (int v = 0; v < NVARIANTS && rc == VLX_OK; v++) {
         ctx->force_variant = v;
         scr->schedules.clear();
@@ -1384,6 +1414,16 @@ int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_densi
         rc = drain_timing(ctx);
         if (rc) break;
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::vector<double>& g = v == 0 ? g_ref : g_var;
+        HIP_TRY(hipMemcpy(g.data(), ctx->d_fock.ptr, nn * sizeof(double), hipMemcpyDeviceToHost));
+        bool ok = true;
+        for (size_t e = 0; e < nn && ok; e++)
+            ok = std::isfinite(g[e]) && (v == 0 || std::fabs(g[e] - g_ref[e]) <= 1.0e-10);
+        if (!ok) {
+            ctx->autotune_rejected |= 1 << v;
+            if (v == 0) rc = fail(VLX_ERR_HIP, "autotune: the untiled variant's G has non-finite elements");
+            continue;
+        }
         ctx->class_ms.clear();
         ctx->timing = true;
         for (int r = 0; r < repeats && rc == VLX_OK; r++)
@@ -1430,6 +1470,13 @@ int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_densi
         if (variants) variants[r] = kv.second;
         r++;
     }
+    return VLX_OK;
+}
+
+int vlx_fock_autotune_rejected(vlx_context* ctx, int* mask)
+{
+    if (!ctx || !mask) return fail(VLX_ERR_ARG, "NULL argument");
+    *mask = ctx->autotune_rejected;
     return VLX_OK;
 }
 
@@ -1577,6 +1624,16 @@ int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::
     return VLX_OK;
 }
 
+// Makes the context stream wait for everything queued so far on the side streams.
+int join_side_streams(vlx_context* ctx)
+{
+    for (int k = 0; k < NSIDE; k++) {
+        HIP_TRY(hipEventRecord(ctx->ev_done[k], ctx->side[k]));
+        HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_done[k], 0));
+    }
+    return VLX_OK;
+}
+
 int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, double fj, double fk, double omega2,
                int ithreshold, double* d_fock_out, double anti_coef)
 {
@@ -1614,12 +1671,12 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
     if (rc) return rc;
     HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
     for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamWaitEvent(ctx->side[k], ctx->ev_start, 0));
+    // the side streams join the context stream also when a launch fails part-way, so that
+    // work ordered on the context stream (copies, frees) never overtakes a queued kernel
     rc = enqueue_classes(ctx, scr, *list, d_density, ctx->d_faccum.ptr, fj, fk, omega2, ithreshold, nullptr, 0);
+    const int rj = join_side_streams(ctx);
     if (rc) return rc;
-    for (int k = 0; k < NSIDE; k++) {
-        HIP_TRY(hipEventRecord(ctx->ev_done[k], ctx->side[k]));
-        HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_done[k], 0));
-    }
+    if (rj) return rj;
     dim3 sg((n + 255) / 256, n);
     if (anti_coef == 0.0) symmetrize_kernel<<<sg, 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, d_fock_out, n);
     else antisym_add_kernel<<<sg, 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, d_fock_out, n, anti_coef);
@@ -1697,18 +1754,19 @@ int vlx_fock_compute_batch_device(vlx_context* ctx, int nmol, vlx_screener* cons
         const size_t nn = (size_t)n * n;
         zero_kernel<<<std::min<size_t>((nn + 255) / 256, 4096), 256, 0, st>>>(scr->d_faccum.ptr, nn);
         HIP_TRY(hipGetLastError());
-        const int rc = enqueue_classes(ctx, scr, *lists[m], d_densities[m], scr->d_faccum.ptr, fj, fk, omega2,
-                                       ithreshold, st);
-        if (rc) return rc;
-        dim3 sg((n + 255) / 256, n);
-        symmetrize_kernel<<<sg, 256, 0, st>>>(scr->d_faccum.ptr, d_focks[m], n);
-        HIP_TRY(hipGetLastError());
+        int rc = enqueue_classes(ctx, scr, *lists[m], d_densities[m], scr->d_faccum.ptr, fj, fk, omega2,
+                                 ithreshold, st);
+        if (!rc) {
+            dim3 sg((n + 255) / 256, n);
+            symmetrize_kernel<<<sg, 256, 0, st>>>(scr->d_faccum.ptr, d_focks[m], n);
+            if (hipGetLastError() != hipSuccess) rc = fail(VLX_ERR_HIP, "symmetrize_kernel launch failed");
+        }
+        if (rc) {
+            (void)join_side_streams(ctx);
+            return rc;
+        }
     }
-    for (int k = 0; k < NSIDE; k++) {
-        HIP_TRY(hipEventRecord(ctx->ev_done[k], ctx->side[k]));
-        HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_done[k], 0));
-    }
-    return VLX_OK;
+    return join_side_streams(ctx);
 }
 
 int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,

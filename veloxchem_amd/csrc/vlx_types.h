@@ -11,6 +11,10 @@
 
 #include "vlx_boys.h"
 
+// Sets the calling thread's message of vlx_last_error() (every translation unit of the library
+// reports through it).
+void vlx_set_last_error(const char* msg);
+
 // Wavefront width and the block shape of the untiled Fock kernels (4 waves).
 constexpr int VLX_WAVE = 64;
 constexpr int VLX_BLOCK = 256;

@@ -31,7 +31,7 @@ import ctypes
 import math
 import time
 from collections import deque
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import Future, ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -151,6 +151,11 @@ class ScfRestrictedDriver:
         self.incremental_fock = False
         self.full_fock_every = 8
         self.screener_factory = None
+        # starting density: None for the reference's guesses (SAD, L2-DIIS valence stage), or an
+        # object with supports(molecule, basis) and density(molecule, basis, overlap) (the learned
+        # guess, learnedguess.LearnedGuess); the full-basis DIIS then starts from its density, and
+        # molecules it does not support fall back to SAD
+        self.guess = None
         self.device = None
         self.history = []
         self.timings = {}
@@ -215,6 +220,15 @@ class ScfRestrictedDriver:
 
     def _run(self, molecule, ao_basis, min_basis):
         acc = self.acc_type.upper()
+        if self.guess is not None and self.guess.supports(molecule, ao_basis):
+            t = time.perf_counter()
+            one = Future()
+            one.set_result(compute_one_electron_integrals(molecule, ao_basis))
+            self.timings['one_electron'] += time.perf_counter() - t
+            t = time.perf_counter()
+            den = self.guess.density(molecule, ao_basis, one.result()[0])
+            self.timings['guess'] += time.perf_counter() - t
+            return self._comp_diis(molecule, ao_basis, den, self.conv_thresh, self.max_iter, one)
         if acc == 'DIIS':
             t = time.perf_counter()
             den = sad_density(molecule, min_basis, ao_basis)

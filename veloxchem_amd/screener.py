@@ -99,6 +99,19 @@ class T4CScreener:
                                                              ctypes.byref(b)))
         return a.value, b.value
 
+    def precision_classes(self, ithreshold, k32):
+        """{(la, lb, lc, ld): FP32 primitive slots} of one build when bin sums >= k32 run in FP32."""
+        L = _native.lib()
+        n = ctypes.c_int()
+        _native.check(L.vlx_fock_precision_classes(self.handle, int(ithreshold), int(k32), 0, ctypes.byref(n), None,
+                                                   None))
+        cls = np.zeros(4 * max(n.value, 1), dtype=np.int32)
+        p32 = np.zeros(max(n.value, 1), dtype=np.int64)
+        _native.check(L.vlx_fock_precision_classes(self.handle, int(ithreshold), int(k32), n.value, ctypes.byref(n),
+                                                   cls.ctypes.data_as(_native._c_int_p),
+                                                   p32.ctypes.data_as(_native._c_ll_p)))
+        return {tuple(int(x) for x in cls[4 * i:4 * i + 4]): int(p32[i]) for i in range(n.value)}
+
     def schedule_stats(self, ithreshold):
         """Per class (same order as work()): (lane slots, primitive slots) the kernels execute."""
         L = _native.lib()
