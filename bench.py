@@ -40,6 +40,11 @@ TRAFFIC_FILES = {'water64_tzvp': os.path.join(ROOT, 'profiles', 'fock_build_wate
 REF_FLOPS_FILE = os.path.join(ROOT, 'tests', 'golden', 'ref_scheme_flops.json')
 
 
+def log(msg):
+    """Progress line on stderr (the JSON line alone goes to stdout)."""
+    print(f'bench: {msg}', file=sys.stderr, flush=True)
+
+
 def kernel_traffic(workload, name):
     try:
         with open(TRAFFIC_FILES[workload]) as fh:
@@ -224,12 +229,15 @@ def workload_builds(nmol, box, seed, basis, ithr, steps, warmup, taus, autotune_
     from veloxchem_amd import FockDriver, _native
     mol, bas, scr, dm, t_screen = build_workload(nmol, box, seed, basis)
     n = bas.get_dimensions_of_basis()
+    log(f'{workload_key}: {n} AOs, screener {t_screen:.2f} s')
     tuned, t_tune = None, 0.0
     if autotune_repeats > 0:
         t0 = time.perf_counter()
         tuned = FockDriver()._autotune(scr, dm, ithr, autotune_repeats)
         t_tune = time.perf_counter() - t0
+        log(f'{workload_key}: autotuned in {t_tune:.1f} s')
     ms, times, g, sweep = measure_builds(scr, dm, ithr, steps, warmup, taus)
+    log(f'{workload_key}: {1e3 / ms:.3f} builds/s over {steps} builds; mixed sweep of {len(taus)} thresholds done')
     work = scr.work(ithr)
     rf = roofline(scr, ithr, times, ms, workload_key)
     # the classes timed alone: one stream serialises the class kernels (in the timed region four
@@ -405,6 +413,7 @@ def main():
             if not args.no_scf:
                 # the dense-algebra libraries load their kernels for each matrix size on first use: one
                 # untimed SCF of the same molecule first (every run below builds its own screener)
+                log('SCF runs')
                 run_scf(smol, sbas, 1.0e-4)
                 result['scf'] = run_scf(smol, sbas, args.scf_conv)
                 inc = run_scf(smol, sbas, args.scf_conv, incremental=True)
@@ -417,8 +426,10 @@ def main():
                         mixed['dE_vs_fp64'] = mixed['energy'] - result['scf']['energy']
                         result[key] = mixed
         if not args.no_batch:
+            log('batched clusters')
             result['batched_clusters'] = batched_clusters(args)
         if not args.no_cpu_baseline:
+            log('reference CPU build of the headline workload')
             result['cpu_baseline'] = cpu_baseline(mol, bas, dm, args.ithreshold, args.cpu_seconds, 'def2-tzvp', g,
                                                   repeats=args.cpu_repeats)
     print(json.dumps(result))
