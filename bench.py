@@ -269,8 +269,9 @@ def workload_builds(nmol, box, seed, basis, ithr, steps, warmup, taus, autotune_
 def batched_clusters(args):
     """Batched uneven builds: 256 water clusters of 1-12 molecules (sizes drawn with seed 4,
     cluster i with seed 4000 + i, box scaled to the 32-water density), def2-SVP, SAD densities,
-    resident in HBM.  Aggregate builds/s of one batched call (each molecule whole on one stream,
-    molecules concurrent) against the same molecules built one after another."""
+    resident in HBM.  Aggregate builds/s of one batched call (one launch per angular class over
+    every molecule, replayed as a captured HIP graph) against the same molecules built one after
+    another, in FP64 and in mixed precision."""
     import ctypes
     from veloxchem_amd import MolecularBasis, T4CScreener, _native
     from veloxchem_amd.sadguess import sad_density
@@ -321,6 +322,19 @@ def batched_clusters(args):
     t_bat = timed(batched, args.batch_reps)
     g_bat = [b.download() for b in d_g]
     diff = max(float(np.max(np.abs(x - y))) for x, y in zip(g_seq, g_bat))
+    # mixed precision at the loosest swept threshold: one-by-one (host reads max|D| per build) and
+    # batched (split on the device), both against the FP64 one-by-one G
+    tau = max([float(x) for x in args.mixed.split(',') if x.strip()] or [1e-6])
+    try:
+        _native.set_precision('mixed', tau)
+        t_seq32 = timed(one_by_one, args.batch_reps)
+        g_seq32 = [b.download() for b in d_g]
+        t_bat32 = timed(batched, args.batch_reps)
+        g_bat32 = [b.download() for b in d_g]
+    finally:
+        _native.set_precision('fp64')
+    diff32_seq = max(float(np.max(np.abs(x - y))) for x, y in zip(g_seq, g_seq32))
+    diff32_bat = max(float(np.max(np.abs(x - y))) for x, y in zip(g_seq, g_bat32))
     quartets = sum(sum(w[1] for w in s_.work(args.ithreshold)) for s_ in scrs)
     for a, b in zip(d_dm, d_g):
         a.free()
@@ -331,7 +345,65 @@ def batched_clusters(args):
             'batched_builds_per_s': nmol / t_bat, 'batched_ms_per_batch': t_bat * 1e3,
             'one_by_one_builds_per_s': nmol / t_seq, 'one_by_one_ms_per_batch': t_seq * 1e3,
             'speedup': t_seq / t_bat, 'max_abs_dG_batched_vs_one_by_one': diff,
+            'mixed': {'threshold': tau, 'batched_builds_per_s': nmol / t_bat32,
+                      'one_by_one_builds_per_s': nmol / t_seq32, 'speedup': t_seq32 / t_bat32,
+                      'batched_vs_fp64_one_by_one': t_seq / t_bat32,
+                      'max_abs_dG_batched_vs_fp64': diff32_bat, 'max_abs_dG_one_by_one_vs_fp64': diff32_seq},
             'shell_quartets': int(quartets), 'reps': args.batch_reps, 'setup_s': round(t_setup, 2)}
+
+
+def learned_guess(args):
+    """The learned SCF guess: a network trained on the GPU (BF16 matrix-core kernels) on converged
+    RHF densities of seeded water clusters of 4-16 molecules (sizes seed 5, cluster i seed 10000 + i,
+    box scaled to the 32-water density), from this package's own SCF; then the SCF of held-out
+    clusters of 16-32 molecules (cluster i seed 20000 + i) from SAD and from the learned guess, with
+    iterations and the converged energies."""
+    from veloxchem_amd import MolecularBasis
+    from veloxchem_amd.learnedguess import LearnedGuess, make_dataset
+    from veloxchem_amd.scfdriver import ScfRestrictedDriver
+    from veloxchem_amd.workloads import water_cluster
+
+    def cluster(nw, seed):
+        return water_cluster(int(nw), 10.0 * (nw / 32.0) ** (1.0 / 3.0), seed)
+
+    t0 = time.perf_counter()
+    sizes = np.random.Generator(np.random.PCG64(5)).integers(4, 17, args.guess_train)
+    X, T, S = make_dataset([cluster(nw, 10000 + i) for i, nw in enumerate(sizes)])
+    t_data = time.perf_counter() - t0
+    log(f'learned guess: {len(X)} pair samples from {len(sizes)} converged SCFs in {t_data:.1f} s')
+    guess = LearnedGuess(max_batch=4096, seed=1)
+    t0 = time.perf_counter()
+    loss = guess.train(X, T, S, epochs=args.guess_epochs, batch=2048, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    log(f'learned guess: trained {args.guess_epochs} epochs in {t_train:.1f} s, loss {loss:.3e}')
+    esizes = np.random.Generator(np.random.PCG64(6)).integers(16, 33, args.guess_eval)
+    rows = []
+    for i, nw in enumerate(esizes):
+        mol = cluster(nw, 20000 + i)
+        bas = MolecularBasis.read(mol, 'def2-svp')
+        res = {}
+        for name, g in (('sad', None), ('learned', guess)):
+            drv = ScfRestrictedDriver()
+            drv.acc_type = 'DIIS'
+            drv.conv_thresh = 1.0e-8
+            drv.guess = g
+            r = drv.compute(mol, bas)
+            res[name] = (r['num_iter'], r['scf_energy'], r['is_converged'], r['timings']['total'])
+        rows.append({'waters': int(nw), 'seed': 20000 + i, 'sad_iterations': res['sad'][0],
+                     'learned_iterations': res['learned'][0], 'dE': res['learned'][1] - res['sad'][1],
+                     'converged': bool(res['sad'][2] and res['learned'][2]),
+                     'sad_scf_s': round(res['sad'][3], 3), 'learned_scf_s': round(res['learned'][3], 3)})
+        log(f'learned guess: {nw} waters, SAD {res["sad"][0]} vs learned {res["learned"][0]} iterations')
+    it_sad = sum(r['sad_iterations'] for r in rows)
+    it_learned = sum(r['learned_iterations'] for r in rows)
+    return {'workload': f'train: {len(sizes)} water clusters of 4-16 molecules (sizes seed 5, seeds 10000+i), '
+                        f'def2-SVP converged RHF densities of this package\'s SCF; evaluate: {len(esizes)} held-out '
+                        f'clusters of 16-32 molecules (sizes seed 6, seeds 20000+i), SCF to 1e-8',
+            'pair_samples': int(len(X)), 'data_s': round(t_data, 2), 'train_s': round(t_train, 2),
+            'epochs': args.guess_epochs, 'final_loss': loss,
+            'iteration_cut': 1.0 - it_learned / max(1, it_sad), 'sad_iterations': it_sad,
+            'learned_iterations': it_learned, 'max_abs_dE': max(abs(r['dE']) for r in rows),
+            'clusters': rows}
 
 
 def main():
@@ -359,6 +431,10 @@ def main():
                     help='skip the batched workload of 256 small water clusters')
     ap.add_argument('--batch-clusters', type=int, default=256)
     ap.add_argument('--batch-reps', type=int, default=3)
+    ap.add_argument('--no-guess', action='store_true', help='skip the learned-guess training and evaluation')
+    ap.add_argument('--guess-train', type=int, default=96, help='training clusters of the learned guess')
+    ap.add_argument('--guess-eval', type=int, default=6, help='held-out evaluation clusters of the learned guess')
+    ap.add_argument('--guess-epochs', type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU; the multi-GPU sharding is design only '
@@ -428,6 +504,9 @@ def main():
         if not args.no_batch:
             log('batched clusters')
             result['batched_clusters'] = batched_clusters(args)
+        if not args.no_guess:
+            log('learned guess')
+            result['learned_guess'] = learned_guess(args)
         if not args.no_cpu_baseline:
             log('reference CPU build of the headline workload')
             result['cpu_baseline'] = cpu_baseline(mol, bas, dm, args.ithreshold, args.cpu_seconds, 'def2-tzvp', g,

@@ -116,12 +116,18 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
  * one molecule per CFockDriver::compute call, src/t4c_func/FockDriver.cpp:120-171).
  * Molecule m has its own screener scrs[m] (same context), symmetric density
  * d_densities[m] and output d_focks[m] (device buffers, nao_m x nao_m); label,
- * exchange_factor, omega and ithreshold as in vlx_fock_compute, FP64 whatever
- * the context's precision mode.  Each molecule runs whole on one side stream
- * with its own F' accumulator (held by its screener), longest first onto the
- * least-loaded stream, so that small molecules run concurrently (a screener
- * listed more than once keeps one stream).  The d_focks must not overlap.
- * Runs on the context's streams and returns without synchronising.
+ * exchange_factor, omega and ithreshold as in vlx_fock_compute, in the
+ * context's precision mode (mixed: each molecule's FP32 split from its own
+ * max|D|, decided on the device).  The molecules' pair lists are concatenated
+ * per angular class, so that one kernel launch per (class, kernel variant)
+ * covers the whole batch; the build (density gather, F' clear, precision
+ * split, class launches, symmetrisation into d_focks) is captured into a HIP
+ * graph on the first call with these screeners, label, thresholds and
+ * precision, and replayed by later calls (up to 8 such plans are kept per
+ * context; destroying a screener drops the plans that hold it).  A screener
+ * may be listed more than once.  At most 65535 molecules.  The d_focks must
+ * not overlap.  Runs on the context's stream and returns without
+ * synchronising.
  */
 int vlx_fock_compute_batch_device(vlx_context* ctx, int nmol, vlx_screener* const* scrs,
                                   const double* const* d_densities, const char* label, double exchange_factor,

@@ -337,6 +337,48 @@ def test_batch_matches_single_builds():
         drv.compute_batch(scrs[:1], [dens[0] + np.triu(np.ones_like(dens[0]), 1)], '2jk', 1.0, 0.0, 12)
 
 
+def test_batch_graph_replays_and_mixed_precision():
+    """The batched build is one captured graph per (screeners, label, threshold, precision): every
+    replay, with single eager builds run in between, gives the single builds' G.  In mixed mode the
+    precision split is decided on the device per molecule (max|D_m|) and matches the single mixed
+    build's split, so the batched and single mixed G agree, and both stay within 1e-3 tau of FP64."""
+    from veloxchem_amd import _native
+    drv = FockDriver()
+    scrs, dens = [], []
+    for i, nw in enumerate((2, 5, 1, 3)):
+        mol = water_cluster(nw, 6.0, 70 + i)
+        bas = MolecularBasis.read(mol, 'def2-svp')
+        scrs.append(_screened(mol, bas))
+        dens.append((1.0 + i) * pseudo_density(bas.get_dimensions_of_basis(), 5 * nw, seed=80 + i))
+    single = [np.asarray(drv.compute(s_, D, '2jk', 1.0, 0.0, 12)) for s_, D in zip(scrs, dens)]
+    for rep in range(3):
+        batch = drv.compute_batch(scrs, dens, '2jk', 1.0, 0.0, 12)
+        err = max(float(np.max(np.abs(g - s_))) for g, s_ in zip(batch, single))
+        print(f'replay {rep}: max|G_batch - G_single| {err:.2e}', flush=True)
+        assert err < 1e-12
+        np.asarray(drv.compute(scrs[1], dens[1], 'j', 0.0, 0.0, 12))  # eager work between replays
+    tau = 1e-7
+    _native.set_precision('mixed', tau)
+    try:
+        single32 = [np.asarray(drv.compute(s_, D, '2jk', 1.0, 0.0, 12)) for s_, D in zip(scrs, dens)]
+        for rep in range(2):
+            batch32 = drv.compute_batch(scrs, dens, '2jk', 1.0, 0.0, 12)
+            for g, g1, g64 in zip(batch32, single32, single):
+                assert np.max(np.abs(g - g1)) < 1e-10
+                assert np.max(np.abs(g - g64)) <= 1e-3 * tau
+    finally:
+        _native.set_precision('fp64')
+    # FP32 work was present: the all-FP32 split moves G away from FP64 by more than rounding
+    _native.set_precision('mixed', 1e6)
+    try:
+        batch_all32 = drv.compute_batch(scrs, dens, '2jk', 1.0, 0.0, 12)
+    finally:
+        _native.set_precision('fp64')
+    dev = max(float(np.max(np.abs(g - g64))) for g, g64 in zip(batch_all32, single))
+    print(f'all-FP32 batched build: max|dG| vs FP64 {dev:.2e}', flush=True)
+    assert 1e-12 < dev < 1e-4
+
+
 def test_autotune_choices_keep_parity():
     """The autotuner times every kernel variant per class; the build it leaves behind still matches the oracle."""
     mol = water_cluster(4, 6.0, 7)

@@ -228,14 +228,14 @@ __global__ void __launch_bounds__(VLX_BLOCK)
     fock_kernel(const PairInfo* __restrict__ bra, const PrimPair* __restrict__ bra_prims,
                 const PairInfo* __restrict__ ket, const PrimPair* __restrict__ ket_prims,
                 const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
-                const double* __restrict__ D, double* __restrict__ F, const FockScalars a)
+                const double* __restrict__ D0, double* __restrict__ F0, const MolDesc* __restrict__ mols,
+                const FockScalars a)
 {
     constexpr int na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
     constexpr int nsph = na * nb * nc * nd;
     __shared__ double tab[BOYS_TAB_SIZE];
     stage_boys(tab, boys_tab);
 
-    const int n = a.nao;
     const bool do_j = a.fj != 0.0, do_k = a.fk != 0.0;
     const int lane = threadIdx.x & (VLX_WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / VLX_WAVE);
@@ -245,8 +245,12 @@ __global__ void __launch_bounds__(VLX_BLOCK)
 
     for (long long it = w0; it < w1; it++) {
         const WorkItem wi = items[it];
+        const MolDesc md = mols[wi.mol];
+        const double* __restrict__ D = D0 + md.base;
+        double* __restrict__ F = F0 + md.base;
+        const int n = md.nao;
         const int j = wi.j0 + lane;
-        const bool lane_ok = j < a.nket;
+        const bool lane_ok = j < wi.jend;
         const PairInfo kp = ket[lane_ok ? j : wi.j0];
         int r[nc], t[nd];
 #pragma unroll
@@ -399,14 +403,14 @@ __global__ void __launch_bounds__(VLX_BLOCK)
     fock_kernel_bs(const PairInfo* __restrict__ bra, const PrimPair* __restrict__ bra_prims,
                    const PairInfo* __restrict__ ket, const PrimPair* __restrict__ ket_prims,
                    const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
-                   const double* __restrict__ D, double* __restrict__ F, const FockScalars a)
+                   const double* __restrict__ D0, double* __restrict__ F0, const MolDesc* __restrict__ mols,
+                   const FockScalars a)
 {
     constexpr int na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
     constexpr int nsph = na * nb * nc * nd;
     __shared__ double tab[BOYS_TAB_SIZE];
     stage_boys(tab, boys_tab);
 
-    const int n = a.nao;
     const int lane = threadIdx.x & (VLX_WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / VLX_WAVE);
     const long long gw = (long long)blockIdx.x * VLX_WAVES_PER_BLOCK + wave;
@@ -415,6 +419,10 @@ __global__ void __launch_bounds__(VLX_BLOCK)
 
     for (long long it = w0; it < w1; it++) {
         const WorkItem wi = items[it];
+        const MolDesc md = mols[wi.mol];
+        const double* __restrict__ D = D0 + md.base;
+        double* __restrict__ F = F0 + md.base;
+        const int n = md.nao;
         const PairInfo bp = bra[wi.i0];
         const int j = wi.j0 + lane;
         const bool active = j < wi.i1;
@@ -541,7 +549,8 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
     fock_kernel_tiled(const PairInfo* __restrict__ pairs_bra, const PrimPair* __restrict__ bra_prims,
                       const PairInfo* __restrict__ pairs_ket, const PrimPair* __restrict__ ket_prims,
                       const TileItem* __restrict__ items, const int2* __restrict__ kentries,
-                      const R* __restrict__ boys_tab, const double* __restrict__ D, double* __restrict__ F,
+                      const R* __restrict__ boys_tab, const double* __restrict__ D0, double* __restrict__ F0,
+                      const MolDesc* __restrict__ mols, const int* __restrict__ k32,
                       const int* __restrict__ grp_off, const int* __restrict__ grp_n,
                       const int* __restrict__ grp_ao, const FockScalars a)
 {
@@ -551,13 +560,32 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
     extern __shared__ double tiles[];
     stage_boys<R>(tab, boys_tab);
 
-    const int n = a.nao;
     const bool do_j = a.fj != 0.0, do_k = a.fk != 0.0;
     const int lane = threadIdx.x & (VLX_WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / VLX_WAVE);
 
     for (long long it = blockIdx.x; it < a.nitems; it += gridDim.x) {
         const TileItem ti = items[it];
+        const MolDesc md = mols[ti.mol];
+        const double* __restrict__ D = D0 + md.base;
+        double* __restrict__ F = F0 + md.base;
+        const int n = md.nao;
+        // bras [b0, b1) of the item: all of them, or, with the precision split on the device, those
+        // whose bin plus the smallest ket bin of the item is below k32 (FP64 kernel) or not (FP32
+        // kernel).  The item's bras are sorted by bin, so the split is one point, found by bisection.
+        int b0 = ti.i0, b1 = ti.i1;
+        if (k32 != nullptr) {
+            const int lim = k32[ti.mol] - ti.bin_lo;  // FP32 from the first bra with bin >= lim
+            int lo = ti.i0, hi = ti.i1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (pairs_bra[mid].bin < lim) lo = mid + 1;
+                else hi = mid;
+            }
+            if (sizeof(R) == sizeof(double)) b1 = lo;
+            else b0 = lo;
+            if (b0 >= b1) continue;  // block-uniform: every thread skips the item together
+        }
         const int nA = grp_n[ti.ga * 4 + la], nB = grp_n[ti.gb * 4 + lb];
         const int nC = grp_n[ti.gc * 4 + lc], nD = grp_n[ti.gd * 4 + ld];
         double* t_jab = tiles;
@@ -603,7 +631,7 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
 #pragma unroll
             for (int l = 0; l < (jcd_in_regs ? nd : 1); l++) jcd[k][l] = 0.0;
 
-        for (int ib = ti.i0 + wave; ib < ti.i1; ib += NW) {
+        for (int ib = b0 + wave; ib < b1; ib += NW) {
             const PairInfo bp = pairs_bra[ib];
             bool active = lane_ok;
             if (a.ithreshold >= 0) active = active && (bp.bin + kp.bin <= a.ithreshold);
@@ -801,11 +829,12 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
     {                                                                                                   \
         if (a.sc.omega2 > 0.0)                                                                          \
             hipLaunchKernelGGL((kernel<la, lb, lc, ld, true>), dim3(grid), dim3(VLX_BLOCK), 0, s, a.bra, \
-                               a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F, a.sc);   \
+                               a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F, a.mols, \
+                               a.sc);                                                                   \
         else                                                                                            \
             hipLaunchKernelGGL((kernel<la, lb, lc, ld, false>), dim3(grid), dim3(VLX_BLOCK), 0, s,      \
                                a.bra, a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F,   \
-                               a.sc);                                                                   \
+                               a.mols, a.sc);                                                           \
     }                                                                                                   \
     void launch_eri_##la##lb##lc##ld(const EriArgs& a, hipStream_t s)                                   \
     {                                                                                                   \
@@ -822,7 +851,7 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
 #define VLX_TILED_LAUNCH(la, lb, lc, ld, R, NW, RS, tab)                                                \
     hipLaunchKernelGGL((fock_kernel_tiled<la, lb, lc, ld, R, NW, RS>), dim3(grid), dim3((NW) * VLX_WAVE), \
                        a.tile_lds_bytes, s, a.bra, a.bra_prims, a.ket, a.ket_prims, a.titems, a.kentries,  \
-                       tab, a.D, a.F, a.grp_off, a.grp_n, a.grp_ao, a.sc)
+                       tab, a.D, a.F, a.mols, a.k32, a.grp_off, a.grp_n, a.grp_ao, a.sc)
 #define VLX_INSTANTIATE_TILED(la, lb, lc, ld, NWB)                                                      \
     void launch_tiled_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                      \
     {                                                                                                   \

@@ -278,11 +278,10 @@ __device__ void quartet(const PrimPair* __restrict__ bra_prims, int bra_np, cons
 // Fock digestion of one block in LDS (factor s included), lanes over outputs.
 template <int la, int lb, int lc, int ld>
 __device__ void digest(const PairInfo& bp, const PairInfo& kp, double s, const double* __restrict__ out,
-                       const double* __restrict__ D, double* __restrict__ F, const FockScalars& a)
+                       const double* __restrict__ D, double* __restrict__ F, const int n, const FockScalars& a)
 {
     using S = Sizes<la, lb, lc, ld>;
     constexpr int na = S::nsa, nb = S::nsb, nc = S::nsc, nd = S::nsd;
-    const int n = a.nao;
     auto I = [&](int i, int j, int k, int l) { return out[(i * nb + j) * S::nscd + k * nd + l]; };
     auto P = [&](int i) { return bp.ao_a + i * bp.str_a; };
     auto Q = [&](int j) { return bp.ao_b + j * bp.str_b; };
@@ -403,7 +402,8 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
                     const double* __restrict__ bra_u, const PairInfo* __restrict__ ket,
                     const PrimPair* __restrict__ ket_prims, const double* __restrict__ ket_u,
                     const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
-                    const double* __restrict__ D, double* __restrict__ F, const FockScalars a)
+                    const double* __restrict__ D0, double* __restrict__ F0, const MolDesc* __restrict__ mols,
+                    const FockScalars a)
 {
     using S = MmdSizes<la, lb, lc, ld>;
     constexpr int L = S::L;
@@ -415,6 +415,9 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
     const int lr = lane & 15, lk = lane >> 4;
     for (long long it = blockIdx.x; it < a.nitems; it += gridDim.x) {
         const WorkItem wi = items[it];
+        const MolDesc md = mols[wi.mol];
+        const double* __restrict__ D = D0 + md.base;
+        double* __restrict__ F = F0 + md.base;
         const PairInfo bp = bra[wi.i0];
         for (int j = wi.j0; j < wi.i1; j++) {
             const PairInfo kp = ket[j];
@@ -572,7 +575,7 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
             if (bp.same) sf *= 0.5;
             if (kp.same) sf *= 0.5;
             if (a.same_class && bp.rank == j) sf *= 0.5;
-            md::digest<la, lb, lc, ld>(bp, kp, sf, outs, D, F, a);
+            md::digest<la, lb, lc, ld>(bp, kp, sf, outs, D, F, md.nao, a);
             __syncthreads();
         }
     }
@@ -631,7 +634,7 @@ __global__ void __launch_bounds__(VLX_WAVE) diag_kernel_md(const DiagArgs a)
 #define VLX_MMD_LAUNCH(la, lb, lc, ld, RS)                                                              \
     hipLaunchKernelGGL((fock_kernel_mmd<la, lb, lc, ld, RS>), dim3(g), dim3(VLX_WAVE), 0, s, a.bra,     \
                        a.bra_prims, a.bra_u, a.ket, a.ket_prims, a.ket_u, a.items, a.boys_tab, a.D, a.F, \
-                       a.sc)
+                       a.mols, a.sc)
 // launch_mmd_: the matrix-core kernel as an alternative variant of a class
 // whose default kernel is generated (same MD work items).
 #define VLX_INSTANTIATE_MMD(la, lb, lc, ld)                                                             \

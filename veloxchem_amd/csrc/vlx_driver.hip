@@ -167,6 +167,62 @@ struct ClassSchedule {
     DevBuf<WorkItem> d_items;
 };
 
+struct vlx_screener;
+
+// One launch of a batched build: the work items of one (class, kernel variant) of every
+// molecule of the batch, their indices moved into the batch's concatenated pair lists.
+struct BatchLaunch {
+    int entry = -1;       // index into g_class_table
+    int variant = 0, tiled_waves = 0, tile_lds = 0, stream = 0;
+    bool tiled = false, singles = false, mmd = false;
+    std::vector<TileItem> titems;
+    std::vector<WorkItem> items;
+    DevBuf<TileItem> d_titems;
+    DevBuf<WorkItem> d_items;
+    double cost = 0.0;    // estimated FLOPs (stream balance, launch order)
+};
+
+// A batched build of many molecules.  Per pair class and order (ket, bra, tiled ket, tiled
+// bra) the molecules' pair lists are concatenated, with primitive ranges, ranks (positions in
+// the ket orders), atom groups and tiled lane entries moved by each molecule's offsets.  Work
+// items move the same way and name their molecule, so one launch per (class, variant) covers
+// the batch; D and F' of molecule m sit at offset mols[m].base of packed arrays.  The whole
+// build (density gather, F' clear, precision split, class launches on the side streams,
+// symmetrisation into the callers' outputs) is captured once into a HIP graph and replayed by
+// later calls with the same screeners and settings.
+struct BatchPlan {
+    std::vector<vlx_screener*> scrs;
+    std::vector<int> epochs;                 // the screeners' variant epochs the plan was built at
+    int ithreshold = 0, force_variant = -1, nstreams = 0;
+    double fj = 0.0, fk = 0.0, omega2 = 0.0; // label factors (kernel arguments of the graph)
+    double tau = 0.0;                        // mixed precision threshold; 0: FP64
+    std::vector<PairSet> ket, bra, tket, tbra;  // concatenated pair sets per pair class
+    std::vector<int> grp_off, grp_n, grp_ao;
+    DevBuf<int> d_grp_off, d_grp_n, d_grp_ao;
+    std::vector<MolDesc> mols;
+    DevBuf<MolDesc> d_mols;
+    size_t total = 0;                        // elements of the packed D and F'
+    int max_nao = 0;
+    DevBuf<double> d_D, d_F;
+    std::vector<const double*> h_src;        // the callers' densities and outputs of the last call
+    std::vector<double*> h_dst;
+    DevBuf<const double*> d_src;
+    DevBuf<double*> d_dst;
+    DevBuf<unsigned long long> d_dmax;       // mixed: max|D| per molecule (bit patterns)
+    DevBuf<int> d_k32;                       // mixed: FP32 bin-sum threshold per molecule
+    std::vector<std::unique_ptr<BatchLaunch>> launches;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    BatchPlan() = default;
+    BatchPlan(const BatchPlan&) = delete;
+    BatchPlan& operator=(const BatchPlan&) = delete;
+    ~BatchPlan()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
 constexpr int BRA_CHUNK = 128;  // default upper bound of bras per work item
 
 constexpr int NSIDE = 4;  // side streams that run class kernels concurrently
@@ -248,6 +304,7 @@ struct vlx_context {
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<int, std::pair<double, long long>> class_ms;  // entry -> (ms, launches)
+    std::vector<std::unique_ptr<BatchPlan>> plans;  // batched builds, most recently used last
 };
 
 struct vlx_screener {
@@ -278,6 +335,8 @@ struct vlx_screener {
     std::map<int, int> variant;  // class entry -> VARIANT_* chosen by vlx_fock_autotune
     std::map<int, double> tuned_ms;  // class entry -> its measured time (ms) under that variant
     DevBuf<double> d_faccum;  // F' of this screener's molecule in batched builds
+    DevBuf<MolDesc> d_mol;    // {0, nao}: the molecule table of a single-molecule build
+    int epoch = 0;            // incremented when the autotuner changes the variants (batch plans)
 };
 
 namespace {
@@ -708,8 +767,9 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
                             im = (b < 0) ? i0 : std::min(i1, bs.start + bs.cnt_le[std::min(b, NBINS - 1)]);
                             im = std::max(im, i0);
                         }
-                        if (im > i0) sch->titems.push_back({i0, im, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb});
-                        if (i1 > im) sch->titems32.push_back({im, i1, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb});
+                        if (im > i0) sch->titems.push_back({i0, im, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb, 0, bin_lo, 0, 0});
+                        if (i1 > im)
+                            sch->titems32.push_back({im, i1, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb, 0, bin_lo, 0, 0});
                     }
                 }
             }
@@ -736,7 +796,8 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
                 std::max<long long>(1, std::min<long long>(s->ctx->max_chunk, span / s->ctx->min_items)));
             for (size_t t = 0; t < ranges.size(); t++)
                 for (int i0 = ranges[t].first; i0 < ranges[t].second; i0 += chunk)
-                    sch->items.push_back({static_cast<int>(64 * t), i0, std::min(ranges[t].second, i0 + chunk), 0});
+                    sch->items.push_back({static_cast<int>(64 * t), i0, std::min(ranges[t].second, i0 + chunk), 0,
+                                          ket.size(), 0, 0, 0});
         } else {
             // work items: one bra x a run of the ket prefix its bins allow
             // (64 kets, one per lane; or a few kets for the matrix-core MD kernel)
@@ -750,8 +811,10 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
                     nk = (lim < 0) ? 0 : ket.cnt_le[std::min(lim, NBINS - 1)];
                 }
                 if (same) nk = std::min(nk, bra.pairs[i].rank + 1);
-                for (int j0 = 0; j0 < nk; j0 += kstep)
-                    sch->items.push_back({j0, i, sch->mmd ? std::min(nk, j0 + kstep) : nk, 0});
+                for (int j0 = 0; j0 < nk; j0 += kstep) {
+                    const int j1 = sch->mmd ? std::min(nk, j0 + kstep) : nk;
+                    sch->items.push_back({j0, i, j1, 0, j1, 0, 0, 0});
+                }
             }
         }
         // schedule efficiency: slots the wavefronts execute, masked lanes included
@@ -1161,6 +1224,7 @@ int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* 
         HIP_TRY(s->d_grp_n.upload(s->grp_n));
         HIP_TRY(s->d_grp_ao.upload(s->grp_ao));
     }
+    HIP_TRY(s->d_mol.upload(std::vector<MolDesc>{MolDesc{0, s->nao, 0}}));
     const int npc = npair_classes();
     s->all.resize(npc);
     s->kept_tiled.resize(npc);
@@ -1274,6 +1338,13 @@ void vlx_screener_destroy(vlx_screener* scr)
     if (!scr) return;
     (void)hipSetDevice(scr->ctx->device);
     (void)hipStreamSynchronize(scr->ctx->stream);
+    // batch plans that hold this screener's pairs go with it
+    auto& plans = scr->ctx->plans;
+    plans.erase(std::remove_if(plans.begin(), plans.end(),
+                               [scr](const std::unique_ptr<BatchPlan>& p) {
+                                   return std::find(p->scrs.begin(), p->scrs.end(), scr) != p->scrs.end();
+                               }),
+                plans.end());
     delete scr;
 }
 
@@ -1456,6 +1527,7 @@ int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_densi
     if (rc) return rc;
     scr->variant = best_var;
     scr->tuned_ms = best_ms;
+    scr->epoch++;
     for (const auto& kv : best_var) ctx->tuned_variant[kv.first] = kv.second;
     if (nchosen) *nchosen = static_cast<int>(best_var.size());
     int r = 0;
@@ -1570,6 +1642,8 @@ int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::
         a.boys_tab_f = ctx->d_boys_f.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;
         a.D = d_density;
         a.F = F;
+        a.mols = scr->d_mol.ptr;
+        a.k32 = nullptr;
         a.titems = sch.d_titems.ptr;
         a.kentries = sch.singles ? ket.singles.d_kentries.ptr : ket.runs.d_kentries.ptr;
         a.grp_off = scr->d_grp_off.ptr;
@@ -1642,7 +1716,8 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
     const int n = scr->nao;
     const size_t nn = (size_t)n * n;
     HIP_TRY(ctx->d_faccum.alloc(nn));
-    zero_kernel<<<std::min<size_t>((nn + 255) / 256, 4096), 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, nn);
+    double* facc = ctx->d_faccum.ptr;
+    zero_kernel<<<std::min<size_t>((nn + 255) / 256, 4096), 256, 0, ctx->stream>>>(facc, nn);
     HIP_TRY(hipGetLastError());
     // mixed precision: FP32 where Schwarz bound x max|D| < tau, i.e. for bin sums
     // >= k32 = ceil(2 + log10(max|D| / tau)) (bins: sqrt(max) < 10^(1 - bin));
@@ -1650,11 +1725,12 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
     int k32 = NO_FP32;
     if (ctx->mixed_tau > 0.0 && ithreshold >= 0 && omega2 == 0.0) {
         HIP_TRY(ctx->d_dmax.alloc(1));
-        clear_u64_kernel<<<1, 1, 0, ctx->stream>>>(ctx->d_dmax.ptr);
-        absmax_kernel<<<std::min<size_t>((nn + 255) / 256, 1024), 256, 0, ctx->stream>>>(d_density, nn, ctx->d_dmax.ptr);
+        unsigned long long* dmax_slot = ctx->d_dmax.ptr;
+        clear_u64_kernel<<<1, 1, 0, ctx->stream>>>(dmax_slot);
+        absmax_kernel<<<std::min<size_t>((nn + 255) / 256, 1024), 256, 0, ctx->stream>>>(d_density, nn, dmax_slot);
         HIP_TRY(hipGetLastError());
         unsigned long long bits = 0;
-        HIP_TRY(hipMemcpyAsync(&bits, ctx->d_dmax.ptr, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(&bits, dmax_slot, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         double dmax;
         std::memcpy(&dmax, &bits, sizeof(dmax));
@@ -1667,19 +1743,19 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
     std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
     rc = get_schedules(scr, ithreshold, k32, &list);
     if (rc) return rc;
-    rc = enqueue_classes(ctx, scr, *list, d_density, ctx->d_faccum.ptr, fj, fk, omega2, ithreshold, nullptr, 1);
+    rc = enqueue_classes(ctx, scr, *list, d_density, facc, fj, fk, omega2, ithreshold, nullptr, 1);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
     for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamWaitEvent(ctx->side[k], ctx->ev_start, 0));
     // the side streams join the context stream also when a launch fails part-way, so that
     // work ordered on the context stream (copies, frees) never overtakes a queued kernel
-    rc = enqueue_classes(ctx, scr, *list, d_density, ctx->d_faccum.ptr, fj, fk, omega2, ithreshold, nullptr, 0);
+    rc = enqueue_classes(ctx, scr, *list, d_density, facc, fj, fk, omega2, ithreshold, nullptr, 0);
     const int rj = join_side_streams(ctx);
     if (rc) return rc;
     if (rj) return rj;
     dim3 sg((n + 255) / 256, n);
-    if (anti_coef == 0.0) symmetrize_kernel<<<sg, 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, d_fock_out, n);
-    else antisym_add_kernel<<<sg, 256, 0, ctx->stream>>>(ctx->d_faccum.ptr, d_fock_out, n, anti_coef);
+    if (anti_coef == 0.0) symmetrize_kernel<<<sg, 256, 0, ctx->stream>>>(facc, d_fock_out, n);
+    else antisym_add_kernel<<<sg, 256, 0, ctx->stream>>>(facc, d_fock_out, n, anti_coef);
     HIP_TRY(hipGetLastError());
     return VLX_OK;
 }
@@ -1700,6 +1776,367 @@ int parse_build(vlx_context* ctx, vlx_screener* scr, const void* d_density, cons
     return VLX_OK;
 }
 
+// ------------------------------------------------------------- batched builds
+
+// Packed densities: molecule blockIdx.y's D into its MolDesc slot.
+__global__ void batch_gather_kernel(const double* const* __restrict__ src, const MolDesc* __restrict__ mols,
+                                    double* __restrict__ dst)
+{
+    const MolDesc md = mols[blockIdx.y];
+    const double* __restrict__ d = src[blockIdx.y];
+    const size_t nn = (size_t)md.nao * md.nao;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x)
+        dst[md.base + i] = d[i];
+}
+
+// G_m = F'_m + F'_m^T into the caller's output of molecule blockIdx.y.
+__global__ void batch_symmetrize_kernel(const double* __restrict__ f, const MolDesc* __restrict__ mols,
+                                        double* const* __restrict__ dst)
+{
+    const MolDesc md = mols[blockIdx.y];
+    double* __restrict__ g = dst[blockIdx.y];
+    const int n = md.nao;
+    const double* __restrict__ fm = f + md.base;
+    const size_t nn = (size_t)n * n;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = i / n, q = i % n;
+        g[i] = fm[i] + fm[q * n + p];
+    }
+}
+
+__global__ void clear_u64_n_kernel(unsigned long long* __restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// max |D_m| per molecule (bit patterns of non-negative doubles order like their values)
+__global__ void batch_absmax_kernel(const double* __restrict__ d, const MolDesc* __restrict__ mols,
+                                    unsigned long long* __restrict__ out)
+{
+    const MolDesc md = mols[blockIdx.y];
+    const size_t nn = (size_t)md.nao * md.nao;
+    double m = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(d[md.base + i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out + blockIdx.y, static_cast<unsigned long long>(__double_as_longlong(m)));
+}
+
+// k32_m = ceil(2 + log10(max|D_m| / tau)) clamped to [0, NO_FP32] (NO_FP32 for a zero density),
+// the bin-sum threshold of the single-molecule build
+__global__ void batch_k32_kernel(const unsigned long long* __restrict__ dmax, int nmol, double tau, int* __restrict__ k32)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    const double v = __longlong_as_double(static_cast<long long>(dmax[m]));
+    int k = NO_FP32;
+    if (v > 0.0) k = static_cast<int>(fmax(0.0, fmin(static_cast<double>(NO_FP32), ceil(2.0 + log10(v / tau)))));
+    k32[m] = k;
+}
+
+// Concatenated pair lists, moved items and packed layout of a batch (see BatchPlan).
+int build_batch_plan(vlx_context* ctx, BatchPlan& P)
+{
+    const int nmol = static_cast<int>(P.scrs.size());
+    const int npc = npair_classes();
+    const bool unscreened = P.ithreshold < 0;
+    std::vector<std::vector<std::unique_ptr<ClassSchedule>>*> lists(nmol, nullptr);
+    for (int m = 0; m < nmol; m++) {
+        const int rc = get_schedules(P.scrs[m], P.ithreshold, NO_FP32, &lists[m]);
+        if (rc) return rc;
+    }
+    // the four orders of every pair class, and each molecule's offsets in them
+    enum { KET = 0, BRA = 1, TKET = 2, TBRA = 3, NKIND = 4 };
+    auto sets_of = [&](vlx_screener* sc, int kind) -> std::vector<PairSet>& {
+        switch (kind) {
+        case KET: return unscreened ? sc->all : sc->kept;
+        case BRA: return unscreened ? sc->all_bra : sc->kept_bra;
+        case TKET: return unscreened ? sc->all_tiled : sc->kept_tiled;
+        default: return unscreened ? sc->all_tiled_bra : sc->kept_tiled_bra;
+        }
+    };
+    std::vector<PairSet>* dst_sets[NKIND] = {&P.ket, &P.bra, &P.tket, &P.tbra};
+    for (auto* v : dst_sets) {
+        v->clear();
+        v->resize(npc);
+    }
+    // poff[kind][m * npc + c]: offset of molecule m's pairs of class c in the concatenated set
+    std::vector<int> poff[NKIND], eoff_runs(nmol * npc), eoff_singles(nmol * npc), goff(nmol);
+    for (int k = 0; k < NKIND; k++) poff[k].assign(nmol * npc, 0);
+    for (int c = 0; c < npc; c++)
+        for (int k = 0; k < NKIND; k++) {
+            int off = 0;
+            for (int m = 0; m < nmol; m++) {
+                const std::vector<PairSet>& v = sets_of(P.scrs[m], k);
+                poff[k][m * npc + c] = off;
+                off += c < static_cast<int>(v.size()) ? v[c].size() : 0;
+            }
+        }
+    int ngroups = 0;
+    for (int m = 0; m < nmol; m++) {
+        goff[m] = ngroups;
+        ngroups += P.scrs[m]->ngroups;
+    }
+    for (int c = 0; c < npc; c++)
+        for (int k = 0; k < NKIND; k++) {
+            PairSet& dst = (*dst_sets[k])[c];
+            for (int m = 0; m < nmol; m++) {
+                const std::vector<PairSet>& v = sets_of(P.scrs[m], k);
+                if (c >= static_cast<int>(v.size())) continue;
+                const PairSet& src = v[c];
+                dst.la = src.la;
+                dst.lb = src.lb;
+                const int pair0 = static_cast<int>(dst.pairs.size()), prim0 = static_cast<int>(dst.prims.size());
+                // ranks are positions in the ket order of the same kind (untiled or tiled)
+                const int rank0 = poff[(k == KET || k == BRA) ? KET : TKET][m * npc + c];
+                for (const PairInfo& pi0 : src.pairs) {
+                    PairInfo pi = pi0;
+                    pi.prim_start += prim0;
+                    pi.rank += rank0;
+                    pi.grp_a += goff[m];
+                    pi.grp_b += goff[m];
+                    dst.pairs.push_back(pi);
+                }
+                dst.prims.insert(dst.prims.end(), src.prims.begin(), src.prims.end());
+                if (k == TKET) {
+                    eoff_runs[m * npc + c] = static_cast<int>(dst.runs.kentries.size());
+                    eoff_singles[m * npc + c] = static_cast<int>(dst.singles.kentries.size());
+                    for (int2 e : src.runs.kentries) dst.runs.kentries.push_back(make_int2(e.x + pair0, e.y));
+                    for (int2 e : src.singles.kentries) dst.singles.kentries.push_back(make_int2(e.x + pair0, e.y));
+                }
+            }
+            finalize_set(dst);
+            if (dst.size() > 0) {
+                const int rc = upload_set(dst);
+                if (rc) return rc;
+            }
+        }
+    // atom groups: (group, l) blocks of every molecule, AO lists local to the molecule
+    for (int m = 0; m < nmol; m++) {
+        const vlx_screener* sc = P.scrs[m];
+        const int ao0 = static_cast<int>(P.grp_ao.size());
+        for (size_t i = 0; i < sc->grp_off.size(); i++) {
+            P.grp_off.push_back(sc->grp_off[i] + ao0);
+            P.grp_n.push_back(sc->grp_n[i]);
+        }
+        P.grp_ao.insert(P.grp_ao.end(), sc->grp_ao.begin(), sc->grp_ao.end());
+    }
+    HIP_TRY(P.d_grp_off.upload(P.grp_off));
+    HIP_TRY(P.d_grp_n.upload(P.grp_n));
+    HIP_TRY(P.d_grp_ao.upload(P.grp_ao));
+    // packed D and F'
+    for (int m = 0; m < nmol; m++) {
+        const int n = P.scrs[m]->nao;
+        P.mols.push_back(MolDesc{static_cast<long long>(P.total), n, 0});
+        P.total += (size_t)n * n;
+        P.max_nao = std::max(P.max_nao, n);
+    }
+    HIP_TRY(P.d_mols.upload(P.mols));
+    HIP_TRY(P.d_D.alloc(P.total));
+    HIP_TRY(P.d_F.alloc(P.total));
+    P.h_src.assign(nmol, nullptr);
+    P.h_dst.assign(nmol, nullptr);
+    HIP_TRY(P.d_src.alloc(nmol));
+    HIP_TRY(P.d_dst.alloc(nmol));
+    if (P.tau > 0.0) {
+        HIP_TRY(P.d_dmax.alloc(nmol));
+        HIP_TRY(P.d_k32.alloc(nmol));
+    }
+    // launches: every molecule's class schedules, grouped by (class, variant, lane entries, block)
+    std::map<std::array<int, 4>, BatchLaunch*> by_key;
+    for (int m = 0; m < nmol; m++) {
+        for (const auto& sp : *lists[m]) {
+            const ClassSchedule& sch = *sp;
+            const auto& ce = g_class_table[sch.entry];
+            const std::array<int, 4> key{sch.entry, sch.variant, sch.singles ? 1 : 0, sch.tiled_waves};
+            BatchLaunch*& L = by_key[key];
+            if (!L) {
+                P.launches.push_back(std::make_unique<BatchLaunch>());
+                L = P.launches.back().get();
+                L->entry = sch.entry;
+                L->variant = sch.variant;
+                L->tiled = sch.tiled;
+                L->singles = sch.singles;
+                L->mmd = sch.mmd;
+                L->tiled_waves = sch.tiled_waves;
+            }
+            L->tile_lds = std::max(L->tile_lds, sch.tile_lds);
+            L->cost += (double)sch.prim_quartets * (ce.vrr_flops + 60) + (double)sch.quartets * ce.hrr_flops;
+            const int x = m * npc + ce.x, y = m * npc + ce.y;
+            if (sch.tiled) {
+                const int eo = sch.singles ? eoff_singles[y] : eoff_runs[y];
+                for (const auto* lst : {&sch.titems, &sch.titems32})
+                    for (TileItem ti : *lst) {
+                        ti.i0 += poff[TBRA][x];
+                        ti.i1 += poff[TBRA][x];
+                        ti.j0 += eo;
+                        ti.j1 += eo;
+                        ti.ga += goff[m];
+                        ti.gb += goff[m];
+                        ti.gc += goff[m];
+                        ti.gd += goff[m];
+                        ti.mol = m;
+                        L->titems.push_back(ti);
+                    }
+            } else {
+                const bool ks = !sch.mmd && ce.variant == VLX_KET_STATIONARY;  // i1 is a bra index
+                for (WorkItem wi : sch.items) {
+                    wi.j0 += poff[KET][y];
+                    wi.i0 += poff[BRA][x];
+                    wi.i1 += ks ? poff[BRA][x] : poff[KET][y];
+                    wi.jend += poff[KET][y];
+                    wi.mol = m;
+                    L->items.push_back(wi);
+                }
+            }
+        }
+    }
+    for (auto& Lp : P.launches) {
+        BatchLaunch& L = *Lp;
+        const auto& ce = g_class_table[L.entry];
+        if (L.tiled) {
+            HIP_TRY(L.d_titems.upload(L.titems));
+            continue;
+        }
+        const PairSet& bra = P.bra[ce.x];
+        const PairSet& ket = P.ket[ce.y];
+        if (L.mmd) {
+            int rc = ensure_u(ctx, P.bra[ce.x]);
+            if (!rc) rc = ensure_u(ctx, P.ket[ce.y]);
+            if (rc) return rc;
+        } else if (ctx->balance_items) {
+            // the wavefronts' contiguous ranges of equal estimated cost, over the whole batch
+            const int nsph = (2 * ce.la + 1) * (2 * ce.lb + 1) * (2 * ce.lc + 1) * (2 * ce.ld + 1);
+            const double pcost = ce.vrr_flops + 60.0, qcost = ce.hrr_flops + 12.0 * nsph;
+            const bool ks = ce.variant == VLX_KET_STATIONARY;
+            std::vector<double> cost;
+            cost.reserve(L.items.size());
+            for (const auto& wi : L.items) {
+                const int j1 = std::min(wi.j0 + 64, wi.jend);
+                int kmax = 0;
+                for (int j = wi.j0; j < j1; j++) kmax = std::max(kmax, ket.pairs[j].nprim);
+                const long long bprims = ks ? bra.nprim_prefix[wi.i1] - bra.nprim_prefix[wi.i0] : bra.pairs[wi.i0].nprim;
+                const long long nbra = ks ? wi.i1 - wi.i0 : 1;
+                cost.push_back(kmax * bprims * pcost + nbra * qcost);
+            }
+            balance_items(L.items, cost, untiled_waves(static_cast<long long>(L.items.size()), ctx->untiled_wave_cap));
+        }
+        HIP_TRY(L.d_items.upload(L.items));
+    }
+    // side streams by estimated cost, most expensive launches first
+    std::sort(P.launches.begin(), P.launches.end(),
+              [](const std::unique_ptr<BatchLaunch>& a, const std::unique_ptr<BatchLaunch>& b) { return a->cost > b->cost; });
+    std::vector<double> load(std::max(1, ctx->nstreams), 0.0);
+    for (auto& Lp : P.launches) {
+        const int k = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        Lp->stream = k;
+        load[k] += Lp->cost;
+    }
+    return VLX_OK;
+}
+
+// Enqueues one batched build (see BatchPlan) on the context stream and its side streams.
+int record_batch(vlx_context* ctx, BatchPlan& P)
+{
+    const int nmol = static_cast<int>(P.scrs.size());
+    const size_t nn_max = (size_t)P.max_nao * P.max_nao;
+    const dim3 gm(static_cast<unsigned>(std::min<size_t>(std::max<size_t>((nn_max + 255) / 256, 1), 256)), nmol);
+    hipStream_t st = ctx->stream;
+    batch_gather_kernel<<<gm, 256, 0, st>>>(P.d_src.ptr, P.d_mols.ptr, P.d_D.ptr);
+    zero_kernel<<<std::min<size_t>((P.total + 255) / 256, 8192), 256, 0, st>>>(P.d_F.ptr, P.total);
+    HIP_TRY(hipGetLastError());
+    const bool mixed = P.tau > 0.0;
+    if (mixed) {
+        clear_u64_n_kernel<<<(nmol + 255) / 256, 256, 0, st>>>(P.d_dmax.ptr, nmol);
+        batch_absmax_kernel<<<gm, 256, 0, st>>>(P.d_D.ptr, P.d_mols.ptr, P.d_dmax.ptr);
+        batch_k32_kernel<<<(nmol + 255) / 256, 256, 0, st>>>(P.d_dmax.ptr, nmol, P.tau, P.d_k32.ptr);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(ctx->ev_start, st));
+    for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamWaitEvent(ctx->side[k], ctx->ev_start, 0));
+    int rc = VLX_OK;
+    for (auto& Lp : P.launches) {
+        const BatchLaunch& L = *Lp;
+        const auto& ce = g_class_table[L.entry];
+        const PairSet& bra = L.tiled ? P.tbra[ce.x] : P.bra[ce.x];
+        const PairSet& ket = L.tiled ? P.tket[ce.y] : P.ket[ce.y];
+        FockArgs a{};
+        a.bra = bra.d_pairs.ptr;
+        a.bra_prims = bra.d_prims.ptr;
+        a.ket = ket.d_pairs.ptr;
+        a.ket_prims = ket.d_prims.ptr;
+        a.bra_u = bra.d_u.ptr;
+        a.ket_u = ket.d_u.ptr;
+        a.items = L.d_items.ptr;
+        a.titems = L.d_titems.ptr;
+        a.kentries = L.singles ? ket.singles.d_kentries.ptr : ket.runs.d_kentries.ptr;
+        a.grp_off = P.d_grp_off.ptr;
+        a.grp_n = P.d_grp_n.ptr;
+        a.grp_ao = P.d_grp_ao.ptr;
+        a.tile_lds_bytes = L.tile_lds;
+        a.tiled_waves = L.tiled_waves;
+        a.boys_tab = ctx->d_boys.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;
+        a.boys_tab_f = ctx->d_boys_f.ptr + (size_t)(ce.la + ce.lb + ce.lc + ce.ld) * BOYS_TAB_SIZE;
+        a.D = P.d_D.ptr;
+        a.F = P.d_F.ptr;
+        a.mols = P.d_mols.ptr;
+        a.k32 = nullptr;
+        a.sc.nitems = static_cast<long long>(L.tiled ? L.titems.size() : L.items.size());
+        a.sc.nket = ket.size();
+        a.sc.ithreshold = P.ithreshold;
+        a.sc.same_class = (ce.x == ce.y) ? 1 : 0;
+        a.sc.nao = 0;
+        a.sc.fj = P.fj;
+        a.sc.fk = P.fk;
+        a.sc.omega2 = P.omega2;
+        if (a.sc.nitems == 0) continue;
+        hipStream_t s = ctx->side[L.stream % NSIDE];
+        if (L.tiled) {
+            const int grid = static_cast<int>(std::min<long long>(a.sc.nitems, ctx->tiled_grid));
+            if (mixed && ce.tiled32) {
+                // both precisions over every item: each kernel runs its side of the device split
+                a.k32 = P.d_k32.ptr;
+                ce.tiled(a, grid, s);
+                ce.tiled32(a, grid, s);
+            } else {
+                ce.tiled(a, grid, s);
+            }
+        } else if (L.mmd) {
+            ce.mmd(a, static_cast<int>(std::min<long long>(a.sc.nitems, ctx->mmd_grid)), s);
+        } else {
+            ce.fock(a, static_cast<int>(untiled_waves(a.sc.nitems, ctx->untiled_wave_cap) / VLX_WAVES_PER_BLOCK), s);
+        }
+        if (hipGetLastError() != hipSuccess) {
+            rc = fail(VLX_ERR_HIP, "batched class launch failed");
+            break;
+        }
+    }
+    const int rj = join_side_streams(ctx);
+    if (rc) return rc;
+    if (rj) return rj;
+    batch_symmetrize_kernel<<<gm, 256, 0, st>>>(P.d_F.ptr, P.d_mols.ptr, P.d_dst.ptr);
+    HIP_TRY(hipGetLastError());
+    return VLX_OK;
+}
+
+// Captures the plan's build into a HIP graph (no allocation, copy or host wait inside).
+int capture_batch(vlx_context* ctx, BatchPlan& P)
+{
+    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = record_batch(ctx, P);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(VLX_ERR_HIP, "graph capture of the batched build: %s", hipGetErrorString(e));
+    P.graph = g;
+    HIP_TRY(hipGraphInstantiate(&P.exec, g, nullptr, nullptr, 0));
+    return VLX_OK;
+}
+
 }  // namespace
 
 int vlx_fock_compute_batch_device(vlx_context* ctx, int nmol, vlx_screener* const* scrs, const double* const* d_densities,
@@ -1709,6 +2146,7 @@ int vlx_fock_compute_batch_device(vlx_context* ctx, int nmol, vlx_screener* cons
     if (!ctx || nmol < 0 || (nmol > 0 && (!scrs || !d_densities || !d_focks)))
         return fail(VLX_ERR_ARG, "NULL argument to vlx_fock_compute_batch_device");
     if (nmol == 0) return VLX_OK;
+    if (nmol > 65535) return fail(VLX_ERR_ARG, "%d molecules in one batch (at most 65535)", nmol);
     double fj = 0.0, fk = 0.0, omega2 = 0.0;
     for (int m = 0; m < nmol; m++) {
         const int rc = parse_build(ctx, scrs[m], d_densities[m], d_focks[m], label, exchange_factor, omega, ithreshold,
@@ -1716,57 +2154,60 @@ int vlx_fock_compute_batch_device(vlx_context* ctx, int nmol, vlx_screener* cons
         if (rc) return rc;
     }
     HIP_TRY(hipSetDevice(ctx->device));
-    // host side first (schedules, accumulators), then the launches: molecules go whole to one
-    // side stream each, longest first onto the least-loaded stream
-    std::vector<std::vector<std::unique_ptr<ClassSchedule>>*> lists(nmol, nullptr);
-    std::vector<std::pair<double, int>> cost(nmol);
-    for (int m = 0; m < nmol; m++) {
-        vlx_screener* scr = scrs[m];
-        const int rc = get_schedules(scr, ithreshold, NO_FP32, &lists[m]);
+    // mixed precision as in single builds (FP32 recursions below tau, Coulomb builds with Schwarz
+    // selection only), the split decided on the device per molecule
+    const double tau = (ctx->mixed_tau > 0.0 && ithreshold >= 0 && omega2 == 0.0) ? ctx->mixed_tau : 0.0;
+    BatchPlan* plan = nullptr;
+    for (auto it = ctx->plans.begin(); it != ctx->plans.end(); ++it) {
+        BatchPlan& P = **it;
+        if (static_cast<int>(P.scrs.size()) != nmol || P.ithreshold != ithreshold || P.tau != tau || P.fj != fj ||
+            P.fk != fk || P.omega2 != omega2 || P.force_variant != ctx->force_variant || P.nstreams != ctx->nstreams)
+            continue;
+        bool same = true;
+        for (int m = 0; m < nmol && same; m++) same = P.scrs[m] == scrs[m] && P.epochs[m] == scrs[m]->epoch;
+        if (!same) continue;
+        // most recently used last
+        std::unique_ptr<BatchPlan> keep = std::move(*it);
+        ctx->plans.erase(it);
+        ctx->plans.push_back(std::move(keep));
+        plan = ctx->plans.back().get();
+        break;
+    }
+    if (!plan) {
+        auto P = std::make_unique<BatchPlan>();
+        P->scrs.assign(scrs, scrs + nmol);
+        for (int m = 0; m < nmol; m++) P->epochs.push_back(scrs[m]->epoch);
+        P->ithreshold = ithreshold;
+        P->force_variant = ctx->force_variant;
+        P->nstreams = ctx->nstreams;
+        P->fj = fj;
+        P->fk = fk;
+        P->omega2 = omega2;
+        P->tau = tau;
+        int rc = build_batch_plan(ctx, *P);
+        if (!rc) rc = capture_batch(ctx, *P);
         if (rc) return rc;
-        HIP_TRY(scr->d_faccum.alloc((size_t)scr->nao * scr->nao));
-        double c = 0.0;
-        for (const auto& sp : *lists[m]) {
-            const auto& ce = g_class_table[sp->entry];
-            c += (double)sp->prim_quartets * (ce.vrr_flops + 60) + (double)sp->quartets * ce.hrr_flops;
+        constexpr size_t MAX_PLANS = 8;
+        if (ctx->plans.size() >= MAX_PLANS) {
+            HIP_TRY(hipStreamSynchronize(ctx->stream));  // the oldest plan's graph may still run
+            ctx->plans.erase(ctx->plans.begin());
         }
-        cost[m] = {c + 1.0e4 * lists[m]->size(), m};  // plus a per-launch term
+        ctx->plans.push_back(std::move(P));
+        plan = ctx->plans.back().get();
     }
-    std::sort(cost.begin(), cost.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
-    const int nst = ctx->nstreams;
-    std::vector<double> load(nst, 0.0);
-    HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
-    for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamWaitEvent(ctx->side[k], ctx->ev_start, 0));
-    // a screener listed more than once keeps one stream: its F' accumulator is then reused in
-    // stream order, never by two streams at once
-    std::map<const vlx_screener*, int> stream_of;
-    for (const auto& cm : cost) {
-        const int m = cm.second;
-        const auto seen = stream_of.find(scrs[m]);
-        const int k = seen != stream_of.end()
-                          ? seen->second
-                          : static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
-        stream_of[scrs[m]] = k;
-        load[k] += cm.first;
-        hipStream_t st = ctx->side[k];
-        vlx_screener* scr = scrs[m];
-        const int n = scr->nao;
-        const size_t nn = (size_t)n * n;
-        zero_kernel<<<std::min<size_t>((nn + 255) / 256, 4096), 256, 0, st>>>(scr->d_faccum.ptr, nn);
-        HIP_TRY(hipGetLastError());
-        int rc = enqueue_classes(ctx, scr, *lists[m], d_densities[m], scr->d_faccum.ptr, fj, fk, omega2,
-                                 ithreshold, st);
-        if (!rc) {
-            dim3 sg((n + 255) / 256, n);
-            symmetrize_kernel<<<sg, 256, 0, st>>>(scr->d_faccum.ptr, d_focks[m], n);
-            if (hipGetLastError() != hipSuccess) rc = fail(VLX_ERR_HIP, "symmetrize_kernel launch failed");
-        }
-        if (rc) {
-            (void)join_side_streams(ctx);
-            return rc;
-        }
+    // the callers' buffers: pointer tables in HBM, read by the graph's gather and scatter kernels
+    bool changed = false;
+    for (int m = 0; m < nmol; m++) changed = changed || plan->h_src[m] != d_densities[m] || plan->h_dst[m] != d_focks[m];
+    if (changed) {
+        plan->h_src.assign(d_densities, d_densities + nmol);
+        plan->h_dst.assign(d_focks, d_focks + nmol);
+        HIP_TRY(hipMemcpyAsync(plan->d_src.ptr, plan->h_src.data(), nmol * sizeof(const double*), hipMemcpyHostToDevice,
+                               ctx->stream));
+        HIP_TRY(hipMemcpyAsync(plan->d_dst.ptr, plan->h_dst.data(), nmol * sizeof(double*), hipMemcpyHostToDevice,
+                               ctx->stream));
     }
-    return join_side_streams(ctx);
+    HIP_TRY(hipGraphLaunch(plan->exec, ctx->stream));
+    return VLX_OK;
 }
 
 int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,

@@ -44,16 +44,29 @@ struct alignas(16) PairInfo {
 };
 
 // Tiled Fock work: bras [i0, i1) of atom groups (ga, gb) against ket
-// primitive entries [j0, j1) (at most 64, one per lane) of atom groups (gc, gd).
+// primitive entries [j0, j1) (at most 64, one per lane) of atom groups (gc, gd),
+// of molecule `mol` of the launch (MolDesc).  bin_lo is the smallest Schwarz
+// bin of the item's kets (the precision split on the device, FockArgs::k32).
 struct TileItem {
     int i0, i1, j0, j1;
     int ga, gb, gc, gd;
+    int mol, bin_lo, pad0, pad1;
 };
 
-// One unit of Fock work: kets [j0, j0 + 64) of the ket list (one per lane)
-// against bras [i0, i1) of the bra list.
+// One unit of Fock work: kets [j0, j0 + 64) of the ket list (one per lane,
+// lanes at or beyond jend idle) against bras [i0, i1) of the bra list, of
+// molecule `mol` of the launch.
 struct WorkItem {
-    int j0, i0, i1, pad;
+    int j0, i0, i1, mol;
+    int jend, pad0, pad1, pad2;
+};
+
+// Where one molecule's D and F' live: nao x nao row-major at element offset
+// `base` of the launch's D and F arrays.  A single-molecule build has one
+// entry {0, nao}; a batched build packs its molecules back to back.
+struct MolDesc {
+    long long base;
+    int nao, pad;
 };
 
 // Scalar parameters of one Fock class launch.
@@ -85,8 +98,12 @@ struct FockArgs {
     const float* boys_tab_f;       // the same table in FP32 (mixed-precision tiled kernel)
     const double* bra_u;           // matrix-core MD kernel: Hermite-to-spherical U of every bra primitive pair
     const double* ket_u;           // ... and of every ket primitive pair
-    const double* D;               // nao x nao, row-major
-    double* F;                     // nao x nao accumulator F' (G = F' + F'^T)
+    const double* D;               // densities of the launch's molecules (MolDesc layout)
+    double* F;                     // F' accumulators (G = F' + F'^T), same layout
+    const MolDesc* mols;           // per molecule: offset and size of its D and F'
+    const int* k32;                // per molecule: FP32 bin-sum threshold on the device, or nullptr
+                                   // (tiled items: the FP64 kernel runs the item's bras with
+                                   // bin + bin_lo < k32, the FP32 kernel the others)
     FockScalars sc;
 };
 
