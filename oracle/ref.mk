@@ -17,7 +17,14 @@ SKIP := $(REF)/math/DenseDiagonalizer.cpp $(REF)/math/DenseLinearAlgebra.cpp \
 SRCS := $(filter-out $(SKIP),$(foreach d,$(DIRS),$(wildcard $(REF)/$(d)/*.cpp)))
 OBJS := $(patsubst $(REF)/%.cpp,$(OUT)/obj/%.o,$(SRCS))
 
-all: $(OUT)/vlx_ref_fock
+# the reference-side drop-in (veloxchem_amd/refbind): CGpuFockDriver against the reference headers,
+# its pybind11 export (compiled, not linked: the reference's Python module registers the argument
+# types) and a harness that runs it beside CFockDriver on the same CT4CScreener and CMatrix
+BIND := veloxchem_amd/refbind
+PYBIND_INC := $(shell python3 -c "import pybind11; print(pybind11.get_include())")
+PY_INC := $(shell python3 -c "import sysconfig; print(sysconfig.get_paths()['include'])")
+
+all: $(OUT)/vlx_ref_fock $(OUT)/vlx_ref_gpu_fock $(OUT)/refbind/ExportGpuFock.o
 
 $(OUT)/obj/%.o: $(REF)/%.cpp
 	@mkdir -p $(dir $@)
@@ -32,3 +39,21 @@ $(OUT)/vlx_ref_fock: $(OUT)/harness.o $(OBJS)
 	strip $@
 
 .PHONY: all
+
+$(OUT)/refbind/GpuFockDriver.o: $(BIND)/GpuFockDriver.cpp $(BIND)/GpuFockDriver.hpp include/vlxfock.h
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) $(INC) -c -o $@ $<
+
+$(OUT)/refbind/ExportGpuFock.o: $(BIND)/ExportGpuFock.cpp $(BIND)/GpuFockDriver.hpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -fPIC $(INC) -I$(REF)/python -I$(PYBIND_INC) -I$(PY_INC) -c -o $@ $<
+
+$(OUT)/ref_gpu_harness.o: oracle/ref_gpu_harness.cpp $(BIND)/GpuFockDriver.hpp
+	@mkdir -p $(OUT)
+	$(CXX) $(CXXFLAGS) $(INC) -c -o $@ $<
+
+# libvlxfock.so is found next to the package at run time ($ORIGIN = oracle/_ref)
+$(OUT)/vlx_ref_gpu_fock: $(OUT)/ref_gpu_harness.o $(OUT)/refbind/GpuFockDriver.o $(OBJS) veloxchem_amd/libvlxfock.so
+	$(CXX) -fopenmp -o $@ $(OUT)/ref_gpu_harness.o $(OUT)/refbind/GpuFockDriver.o $(OBJS) \
+	    -Lveloxchem_amd -lvlxfock -Wl,-rpath,'$$ORIGIN/../../veloxchem_amd'
+	strip $@

@@ -67,3 +67,61 @@ def test_cluster_generator_is_seeded_and_respects_min_distance():
     assert d.min() >= 2.6
     h = a.get_coordinates_in_angstrom()
     assert abs(np.linalg.norm(h[1] - h[0]) - 0.9572) < 1e-12
+
+
+from veloxchem_amd import T4CScreener as _T4CScreener  # noqa: E402
+
+
+class _FakeScreener(_T4CScreener):
+    """The screener's host bookkeeping with pair maxima from a closed form (no library)."""
+
+    def _partition(self, basis, molecule, label='eri'):
+        if label != 'eri':
+            raise ValueError(label)
+        self._inputs = (basis, molecule, label)
+        sh = basis.shell_arrays(molecule.get_coordinates_in_bohr())
+        self._shell_l, self._shell_atom = np.asarray(sh['l']), np.asarray(sh['atom'])
+        self._nshells = len(sh['l'])
+        xyz = np.asarray(sh['centers'])
+        self._pm = np.exp(-np.sum((xyz[:, None] - xyz[None]) ** 2, axis=-1))
+        self._key = ('fake', xyz.tobytes())
+
+    def pair_maxima(self):
+        return self._pm
+
+
+def test_screener_surface_copy_pickle_pair_blocks():
+    """T4CScreener as exported by the reference (src/python/ExportT4CIntegrals.cpp:127-140): copy,
+    pickle, gto_pair_blocks and the atom-restricted partitions, on the host side of the screener
+    (no build)."""
+    import copy
+    import pickle
+
+    import numpy as np
+    import pytest
+
+    from tests.common import dimer
+    mol, bas = dimer('def2-svp')
+    s = _FakeScreener().partition(bas, mol, 'eri')
+    blocks = s.gto_pair_blocks()
+    npairs = sum(len(b['pairs']) for b in blocks)
+    assert npairs == s._nshells * (s._nshells + 1) // 2  # exp(-r^2) >= 1e-30 for every dimer pair
+    for b in blocks:
+        la, lb = b['angular_momentums']
+        assert la <= lb
+        for (p, q), v in zip(b['pairs'], b['max_values']):
+            assert (s._shell_l[p], s._shell_l[q]) == (la, lb)
+            assert b['bin'] == (0 if v >= 1.0 else int(-np.floor(np.log10(np.sqrt(v)))))
+    a0 = _FakeScreener().partition_atom(bas, mol, 'eri', 0)
+    assert a0.is_restricted and not s.is_restricted
+    touched = [(a0._shell_atom[p], a0._shell_atom[q]) for b in a0.gto_pair_blocks() for p, q in b['pairs']]
+    assert touched and all(0 in t for t in touched)
+    ap = _FakeScreener().partition_atom_pair(bas, mol, 'eri', 0, 3)
+    pairs = [tuple(sorted((ap._shell_atom[p], ap._shell_atom[q]))) for b in ap.gto_pair_blocks() for p, q in b['pairs']]
+    assert pairs and set(pairs) == {(0, 3)}
+    assert s != a0
+    with pytest.raises(ValueError):
+        _FakeScreener().partition(bas, mol, 'ovl')
+    # pickle and copy keep the inputs and the restriction (and re-partition)
+    for c in (copy.copy(a0), copy.deepcopy(a0), pickle.loads(pickle.dumps(a0))):
+        assert isinstance(c, _FakeScreener) and c == a0 and c._atoms == (0,)

@@ -153,9 +153,11 @@ class FockDriver:
         """G of many molecules in one call (vlx_fock_compute_batch_device).
 
         ``screeners[m]`` and ``densities[m]`` (symmetric, nao_m x nao_m, Matrix or array) describe
-        molecule m; returns the list of G as numpy arrays.  Every molecule runs whole on one of the
-        context's streams, so that small molecules run concurrently; FP64.  No reference
-        counterpart: the reference builds one molecule per ``compute`` call.
+        molecule m; returns the list of G as numpy arrays.  One kernel launch per angular class
+        covers every molecule, and the whole build is a HIP graph captured on the first call with
+        these screeners and settings and replayed afterwards; the precision mode is the context's
+        (mixed: the FP32 split per molecule from its max|D|).  No reference counterpart: the
+        reference builds one molecule per ``compute`` call.
         """
         if len(screeners) != len(densities):
             raise ValueError('compute_batch: one density per screener')
@@ -221,13 +223,22 @@ class FockDriver:
             keys = density.keys()
             if len(keys) != len(label):
                 raise ValueError('number of densities and labels differ')
+            # one set of integrals serves every density: erf(omega r)/r only when every label is
+            # range-separated, else 1/r for all, the *_rs labels digesting like their Coulomb forms
+            # (CT4CMatricesDistributor::need_omega, src/t4c_func/T4CMatricesDistributor.cpp:55-66)
+            all_rs = all(str(lab).endswith('_rs') for lab in label)
             out = Matrices()
             for key, lab in zip(keys, label):
+                if not all_rs and str(lab).endswith('_rs'):
+                    lab = lab[:-3]
                 out.add(self._one(screener, rank, nodes, density.matrix(key), lab, xf, omega, ithreshold), key)
             return out
         return self._one(screener, rank, nodes, density, label, xf, omega, ithreshold)
 
     def _one(self, screener, rank, nodes, density, label, xf, omega, ithreshold):
+        if screener.is_restricted:
+            raise ValueError('FockDriver: screeners of partition_atom / partition_atom_pair feed the geometric-'
+                             'derivative drivers; the Fock build takes a partition() screener')
         dm, mat = _dense(density)
         n = screener.number_of_ao()
         if dm.shape != (n, n):

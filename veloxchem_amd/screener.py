@@ -16,17 +16,49 @@ from . import _native
 
 class T4CScreener:
 
-    def __init__(self):
+    def __init__(self, other=None):
         self._handle = None
         self._nshells = 0
         self._nao = 0
         self._device = _native.current_device()
         self._key = None
+        self._inputs = None       # (basis, molecule, label) of the last partition
+        self._atoms = None        # partition_atom / partition_atom_pair: atoms the pairs must touch
+        if other is not None:     # T4CScreener(other): a copy (src/python/ExportT4CIntegrals.cpp:130)
+            self.__setstate__(other.__getstate__())
 
     def partition(self, basis, molecule, label='eri'):
+        """Every shell pair of the basis (CT4CScreener::partition, src/t4c_func/T4CScreener.cpp:63-118)."""
+        self._partition(basis, molecule, label)
+        self._atoms = None
+        return self
+
+    def partition_atom(self, basis, molecule, label, iatom):
+        """The pairs with a shell on atom ``iatom`` (CT4CScreener::partition_atom,
+        src/t4c_func/T4CScreener.cpp:120-185).  The reference feeds such screeners to its
+        geometric-derivative Fock drivers, not to FockDriver.compute, which refuses them here."""
+        self._partition(basis, molecule, label)
+        self._atoms = (int(iatom),)
+        return self
+
+    def partition_atom_pair(self, basis, molecule, label, iatom, jatom):
+        """The pairs of a shell on atom ``iatom`` with a shell on atom ``jatom``
+        (CT4CScreener::partition_atom_pair, src/t4c_func/T4CScreener.cpp:187-254); like
+        partition_atom, for the derivative drivers only."""
+        self._partition(basis, molecule, label)
+        self._atoms = (int(iatom), int(jatom))
+        return self
+
+    @property
+    def is_restricted(self):
+        """True after partition_atom / partition_atom_pair."""
+        return self._atoms is not None
+
+    def _partition(self, basis, molecule, label='eri'):
         if label != 'eri':
             raise ValueError(f"T4CScreener.partition: label {label!r} is not supported (only 'eri')")
         self._release()
+        self._inputs = (basis, molecule, label)
         sh = basis.shell_arrays(molecule.get_coordinates_in_bohr())
         self._keep = {}
         ptrs = {}
@@ -44,6 +76,8 @@ class T4CScreener:
         self._nshells = len(sh['l'])
         self._nao = nao
         self._key = (basis.get_label(), np.asarray(molecule.get_coordinates_in_bohr()).tobytes())
+        self._shell_l = np.asarray(sh['l'])
+        self._shell_atom = np.asarray(sh['atom'])
         return self
 
     def _release(self):
@@ -126,5 +160,62 @@ class T4CScreener:
                                                 ps.ctypes.data_as(_native._c_ll_p)))
         return [(int(ls[i]), int(ps[i])) for i in range(n.value)]
 
+    def gto_pair_blocks(self):
+        """The kept shell pairs by angular class and Schwarz bin, as CT4CScreener::gto_pair_blocks
+        lists them (src/t4c_func/T4CScreener.hpp:67; bins of src/orbdata/BlockedGtoPairBlock.cpp:32-42):
+        one dict per (l_a <= l_b, bin) with the shell pairs (a, b), their max over components of
+        (ab|ab) and the bin; a restricted screener lists only the pairs touching its atoms."""
+        pm = self.pair_maxima()
+        la, at = self._shell_l, self._shell_atom
+        blocks = {}
+        n = self._nshells
+        for a in range(n):
+            for b in range(a, n):
+                if la[a] > la[b] or (la[a] == la[b] and a > b):
+                    p, q = b, a
+                else:
+                    p, q = a, b
+                if self._atoms is not None:
+                    if len(self._atoms) == 1 and self._atoms[0] not in (at[p], at[q]):
+                        continue
+                    if len(self._atoms) == 2 and {at[p], at[q]} != set(self._atoms) and not (
+                            self._atoms[0] == self._atoms[1] == at[p] == at[q]):
+                        continue
+                fact = float(np.sqrt(pm[p, q]))
+                if not fact > 1.0e-15:
+                    continue
+                b_ = 0 if fact >= 1.0 else int(-np.floor(np.log10(fact)))
+                blk = blocks.setdefault((int(la[p]), int(la[q]), b_), {
+                    'angular_momentums': (int(la[p]), int(la[q])), 'bin': b_, 'pairs': [], 'max_values': []})
+                blk['pairs'].append((p, q))
+                blk['max_values'].append(float(pm[p, q]))
+        return [blocks[k] for k in sorted(blocks)]
+
+    def __getstate__(self):
+        """Pickled as its inputs (src/python/ExportT4CIntegrals.cpp:131-132 pickles the pair
+        blocks); unpickling partitions again on the current device."""
+        return {'inputs': self._inputs, 'atoms': self._atoms}
+
+    def __setstate__(self, state):
+        self._handle = None
+        self._nshells = 0
+        self._nao = 0
+        self._device = _native.current_device()
+        self._key = None
+        self._inputs = None
+        self._atoms = None
+        if state.get('inputs') is not None:
+            self._partition(*state['inputs'])
+            self._atoms = state.get('atoms')
+
+    def __copy__(self):
+        return type(self)(self)
+
+    def __deepcopy__(self, memo):
+        return type(self)(self)
+
     def __eq__(self, other):
-        return isinstance(other, T4CScreener) and self._key == other._key
+        return isinstance(other, T4CScreener) and self._key == other._key and self._atoms == other._atoms
+
+    def __ne__(self, other):
+        return not self.__eq__(other)
