@@ -112,6 +112,26 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
                             double exchange_factor, double omega, int ithreshold, double* d_fock_out);
 
 /*
+ * vlx_fock_compute_device without any allocation (no reference counterpart:
+ * it is for callers that cannot allocate device memory, such as code under
+ * HIP graph capture or tight serving loops).  The F' accumulator and the
+ * max|D| slot of a mixed build live in a file-scope device array of the
+ * library (VLX_NOALLOC_MAX_NAO^2 doubles + 256 B, 72 MiB per device, held
+ * until the process ends), so nao must not exceed VLX_NOALLOC_MAX_NAO
+ * (VLX_ERR_ARG otherwise).  The work schedule of (screener, ithreshold,
+ * precision split) must exist: one vlx_fock_compute_device call with the
+ * same screener and settings (and, in mixed mode, a density of the same
+ * max|D| decade) comes first, else VLX_ERR_ARG.  Per-class timing is not
+ * recorded.  Every call on one device shares the array: calls that overlap
+ * on one device (two contexts, or two host threads) race and must be
+ * serialised by the caller.  Same arguments, results and stream behaviour
+ * as vlx_fock_compute_device.
+ */
+#define VLX_NOALLOC_MAX_NAO 3072
+int vlx_fock_compute_device_noalloc(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,
+                                    double exchange_factor, double omega, int ithreshold, double* d_fock_out);
+
+/*
  * Many molecules in one call (no reference counterpart: the reference builds
  * one molecule per CFockDriver::compute call, src/t4c_func/FockDriver.cpp:120-171).
  * Molecule m has its own screener scrs[m] (same context), symmetric density

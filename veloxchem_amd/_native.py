@@ -63,6 +63,7 @@ def lib():
                                        ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_double_p]
         L.vlx_fock_compute_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
                                               ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+        L.vlx_fock_compute_device_noalloc.argtypes = L.vlx_fock_compute_device.argtypes
         L.vlx_fock_compute_general.argtypes = L.vlx_fock_compute.argtypes
         L.vlx_fock_compute_batch_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
                                                     ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_double,

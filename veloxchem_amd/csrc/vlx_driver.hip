@@ -54,6 +54,16 @@ int fail(int code, const char* fmt, ...)
 
 constexpr int NBINS = 16;
 
+// Scratch of vlx_fock_compute_device_noalloc (callers that cannot allocate): the F' accumulator
+// (up to VLX_NOALLOC_MAX_NAO^2 doubles) and one 8-byte slot for max|D|, 72 MiB held by every
+// device the library runs on for the life of the process.  Host code never passes `scratch`
+// itself to a kernel: it takes the device address from hipGetSymbolAddress (g_scratch_ptr).
+constexpr size_t SCRATCH_FACC = (size_t)VLX_NOALLOC_MAX_NAO * VLX_NOALLOC_MAX_NAO;
+__device__ __align__(256) double scratch[SCRATCH_FACC + 32];
+constexpr int MAX_DEVICES = 64;
+double* g_scratch_ptr[MAX_DEVICES] = {};  // device address of `scratch` per device (set at context creation)
+std::mutex g_scratch_mutex;
+
 int npair_classes() { return (VLX_GEN_LMAX + 1) * (VLX_GEN_LMAX + 2) / 2; }
 
 int pair_class_index(int la, int lb)  // la >= lb; order (0,0),(1,0),(1,1),(2,0),...
@@ -672,7 +682,10 @@ __global__ void split_kernel(const double* __restrict__ d, double* __restrict__ 
 // tiled items: the bras of an item whose bin plus the smallest ket bin of
 // its chunk reaches k32 form an FP32 item (every quartet in it has
 // Schwarz bound < 10^(2 - k32)); the rest stay FP64.
-int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_ptr<ClassSchedule>>** out)
+// create = false: only a schedule built earlier is returned (no host or device allocation);
+// without one the call fails.
+int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_ptr<ClassSchedule>>** out,
+                  bool create = true)
 {
     const int key = ((ithr + 1) * 128 + k32) * (NVARIANTS + 1) + (s->ctx->force_variant + 1);
     auto it = s->schedules.find(key);
@@ -680,6 +693,9 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
         *out = &it->second;
         return VLX_OK;
     }
+    if (!create)
+        return fail(VLX_ERR_ARG, "no work schedule of this screener for ithreshold %d and FP32 bin sum %d: one "
+                                 "allocating build of the screener at these settings must come first", ithr, k32);
     std::vector<std::unique_ptr<ClassSchedule>> list;
     const int nentries = static_cast<int>(sizeof(g_class_table) / sizeof(g_class_table[0]));
     const bool unscreened = ithr < 0;
@@ -974,6 +990,16 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_KET_ORDER")) ctx->ket_order = std::max(0, std::min(2, std::atoi(e)));
     if (const char* e = std::getenv("VLX_VARIANT")) ctx->force_variant = std::max(-1, std::min(NVARIANTS - 1, std::atoi(e)));
     HIP_TRY(hipSetDevice(device));
+    if (device >= MAX_DEVICES) return fail(VLX_ERR_ARG, "device %d beyond the %d the library supports", device, MAX_DEVICES);
+    {
+        // each device holds its own copy of `scratch`: its address, once per device, with the device current
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        if (!g_scratch_ptr[device]) {
+            void* ptr = nullptr;
+            HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            g_scratch_ptr[device] = static_cast<double*>(ptr);
+        }
+    }
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (int k = 0; k < NSIDE; k++) {
         HIP_TRY(hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
@@ -1618,8 +1644,9 @@ namespace {
 // that run alone (stream -1), 0 only the others, -1 all.  No synchronisation.
 int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::unique_ptr<ClassSchedule>>& list,
                     const double* d_density, double* F, double fj, double fk, double omega2, int ithreshold,
-                    hipStream_t stream, int solo = -1)
+                    hipStream_t stream, int solo = -1, bool allow_timing = true)
 {
+    const bool timing = ctx->timing && allow_timing;
     const bool unscreened = ithreshold < 0;
     for (const auto& schp : list) {
         if (fj == 0.0 && fk == 0.0) break;  // nothing to digest: G = 0
@@ -1664,7 +1691,7 @@ int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::
                                    : static_cast<int>(untiled_waves(a.sc.nitems, ctx->untiled_wave_cap) / VLX_WAVES_PER_BLOCK);
         hipStream_t st = stream ? stream : (sch.stream < 0 ? ctx->stream : ctx->side[sch.stream % ctx->nstreams]);
         vlx_context::Pending pend{sch.entry, nullptr, nullptr};
-        if (ctx->timing) {
+        if (timing) {
             for (hipEvent_t* ev : {&pend.start, &pend.stop}) {
                 if (ctx->event_pool.empty()) {
                     HIP_TRY(hipEventCreate(ev));
@@ -1690,7 +1717,7 @@ int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::
             ce.fock(a, grid, st);
         }
         HIP_TRY(hipGetLastError());
-        if (ctx->timing) {
+        if (timing) {
             HIP_TRY(hipEventRecord(pend.stop, st));
             ctx->pending.push_back(pend);
         }
@@ -1708,15 +1735,20 @@ int join_side_streams(vlx_context* ctx)
     return VLX_OK;
 }
 
+// scratch = nullptr: F' and the max|D| slot live in context buffers (allocated on first use) and
+// missing schedules are built.  Otherwise F' is `scratch` (nao x nao doubles) and max|D| the
+// slot `scratch_dmax`, the schedules must exist already and nothing is allocated.
 int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, double fj, double fk, double omega2,
-               int ithreshold, double* d_fock_out, double anti_coef)
+               int ithreshold, double* d_fock_out, double anti_coef, double* scratch = nullptr,
+               unsigned long long* scratch_dmax = nullptr)
 {
     int rc = VLX_OK;
     HIP_TRY(hipSetDevice(ctx->device));
+    const bool no_alloc = scratch != nullptr;
     const int n = scr->nao;
     const size_t nn = (size_t)n * n;
-    HIP_TRY(ctx->d_faccum.alloc(nn));
-    double* facc = ctx->d_faccum.ptr;
+    if (!no_alloc) HIP_TRY(ctx->d_faccum.alloc(nn));
+    double* facc = no_alloc ? scratch : ctx->d_faccum.ptr;
     zero_kernel<<<std::min<size_t>((nn + 255) / 256, 4096), 256, 0, ctx->stream>>>(facc, nn);
     HIP_TRY(hipGetLastError());
     // mixed precision: FP32 where Schwarz bound x max|D| < tau, i.e. for bin sums
@@ -1724,8 +1756,8 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
     // range-separated builds stay in FP64 (the FP32 kernels are Coulomb-only)
     int k32 = NO_FP32;
     if (ctx->mixed_tau > 0.0 && ithreshold >= 0 && omega2 == 0.0) {
-        HIP_TRY(ctx->d_dmax.alloc(1));
-        unsigned long long* dmax_slot = ctx->d_dmax.ptr;
+        if (!no_alloc) HIP_TRY(ctx->d_dmax.alloc(1));
+        unsigned long long* dmax_slot = no_alloc ? scratch_dmax : ctx->d_dmax.ptr;
         clear_u64_kernel<<<1, 1, 0, ctx->stream>>>(dmax_slot);
         absmax_kernel<<<std::min<size_t>((nn + 255) / 256, 1024), 256, 0, ctx->stream>>>(d_density, nn, dmax_slot);
         HIP_TRY(hipGetLastError());
@@ -1741,15 +1773,15 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
     }
     ctx->last_k32 = k32;
     std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
-    rc = get_schedules(scr, ithreshold, k32, &list);
+    rc = get_schedules(scr, ithreshold, k32, &list, !no_alloc);
     if (rc) return rc;
-    rc = enqueue_classes(ctx, scr, *list, d_density, facc, fj, fk, omega2, ithreshold, nullptr, 1);
+    rc = enqueue_classes(ctx, scr, *list, d_density, facc, fj, fk, omega2, ithreshold, nullptr, 1, !no_alloc);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
     for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamWaitEvent(ctx->side[k], ctx->ev_start, 0));
     // the side streams join the context stream also when a launch fails part-way, so that
     // work ordered on the context stream (copies, frees) never overtakes a queued kernel
-    rc = enqueue_classes(ctx, scr, *list, d_density, facc, fj, fk, omega2, ithreshold, nullptr, 0);
+    rc = enqueue_classes(ctx, scr, *list, d_density, facc, fj, fk, omega2, ithreshold, nullptr, 0, !no_alloc);
     const int rj = join_side_streams(ctx);
     if (rc) return rc;
     if (rj) return rj;
@@ -2217,6 +2249,21 @@ int vlx_fock_compute_device(vlx_context* ctx, vlx_screener* scr, const double* d
     int rc = parse_build(ctx, scr, d_density, d_fock_out, label, exchange_factor, omega, ithreshold, &fj, &fk, &omega2);
     if (rc) return rc;
     return fock_build(ctx, scr, d_density, fj, fk, omega2, ithreshold, d_fock_out, 0.0);
+}
+
+int vlx_fock_compute_device_noalloc(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,
+                                    double exchange_factor, double omega, int ithreshold, double* d_fock_out)
+{
+    double fj = 0.0, fk = 0.0, omega2 = 0.0;
+    int rc = parse_build(ctx, scr, d_density, d_fock_out, label, exchange_factor, omega, ithreshold, &fj, &fk, &omega2);
+    if (rc) return rc;
+    if (scr->nao > VLX_NOALLOC_MAX_NAO)
+        return fail(VLX_ERR_ARG, "vlx_fock_compute_device_noalloc: %d AOs exceed its scratch limit of %d", scr->nao,
+                    VLX_NOALLOC_MAX_NAO);
+    double* base = ctx->device < MAX_DEVICES ? g_scratch_ptr[ctx->device] : nullptr;
+    if (!base) return fail(VLX_ERR_HIP, "no scratch address for device %d", ctx->device);
+    return fock_build(ctx, scr, d_density, fj, fk, omega2, ithreshold, d_fock_out, 0.0, base,
+                      reinterpret_cast<unsigned long long*>(base + SCRATCH_FACC));
 }
 
 int vlx_fock_compute_general_device(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,
