@@ -110,13 +110,14 @@ def build_workload(nmol, box, seed, basis):
     return mol, bas, scr, dm, t_screen
 
 
-def run_scf(mol, bas, conv_thresh, precision='fp64', tau=0.0, incremental=False):
+def run_scf(mol, bas, conv_thresh, precision='fp64', tau=0.0, incremental=False, density_screening=0.0):
     from veloxchem_amd.scfdriver import ScfRestrictedDriver
     drv = ScfRestrictedDriver()
     drv.conv_thresh = conv_thresh
     drv.precision = precision
     drv.mixed_threshold = tau
     drv.incremental_fock = incremental
+    drv.density_screening = density_screening
     res = drv.compute(mol, bas)
     tm = res['timings']
     out = {'scf_time_s': round(tm['total'], 3), 'iterations': res['num_iter'], 'converged': res['is_converged'],
@@ -128,7 +129,26 @@ def run_scf(mol, bas, conv_thresh, precision='fp64', tau=0.0, incremental=False)
         out['incremental_builds'] = sum(1 for h in drv.history if h['incremental'])
     if precision == 'mixed':
         out['mixed_threshold'] = tau
+    if density_screening > 0.0:
+        out['density_screening'] = density_screening
+    # per-SCF average rate of its Fock builds (every build of the run, on the SCF's own densities)
+    out['fock_builds'] = len(drv.history)
+    out['fock_builds_per_s'] = len(drv.history) / max(tm['fock'], 1e-12)
     return out
+
+
+def fast_vs_fp64_scf(mol, bas, conv_thresh, tau_mixed, tau_dens):
+    """The plain FP64 SCF (full builds) against the fast one (mixed precision, incremental
+    difference-density builds, density-weighted screening; confirmed by an unscreened full build):
+    per-SCF-average Fock builds/s, Fock time, iterations and the energy difference."""
+    ref = run_scf(mol, bas, conv_thresh)
+    fast = run_scf(mol, bas, conv_thresh, 'mixed', tau_mixed, incremental=True, density_screening=tau_dens)
+    screened = run_scf(mol, bas, conv_thresh, incremental=True, density_screening=tau_dens)
+    for r in (fast, screened):
+        r['dE_vs_fp64'] = r['energy'] - ref['energy']
+        r['builds_per_s_vs_fp64'] = r['fock_builds_per_s'] / ref['fock_builds_per_s']
+        r['fock_time_vs_fp64'] = ref['fock_time_s'] / max(r['fock_time_s'], 1e-12)
+    return {'fp64': ref, 'fast': fast, 'fp64_incremental_density_screened': screened}
 
 
 def measure_builds(scr, dm, ithr, steps, warmup, taus, label=b'2jk', precision='fp64', tau=0.0):
@@ -222,7 +242,38 @@ def cpu_baseline(mol, bas, dm, ithr, target_s, basis_name, g_gpu, repeats=1):
                       f'{dt:.1f} s), extrapolated by quartet count'}
 
 
-def workload_builds(nmol, box, seed, basis, ithr, steps, warmup, taus, autotune_repeats, workload_key):
+def save_variants(scr, path):
+    """The screener's per-class kernel variants and measured class times, as JSON."""
+    import ctypes
+    from veloxchem_amd import _native
+    lib = _native.lib()
+    n = ctypes.c_int()
+    _native.check(lib.vlx_fock_get_variants(scr.handle, 0, ctypes.byref(n), None, None, None))
+    cls = np.zeros(4 * max(n.value, 1), dtype=np.int32)
+    var = np.zeros(max(n.value, 1), dtype=np.int32)
+    ms = np.zeros(max(n.value, 1))
+    _native.check(lib.vlx_fock_get_variants(scr.handle, n.value, ctypes.byref(n), cls.ctypes.data_as(_native._c_int_p),
+                                            var.ctypes.data_as(_native._c_int_p), ms.ctypes.data_as(_native._c_double_p)))
+    with open(path, 'w') as fh:
+        json.dump({'classes': cls[:4 * n.value].tolist(), 'variants': var[:n.value].tolist(),
+                   'ms': ms[:n.value].tolist()}, fh)
+
+
+def load_variants(scr, path):
+    """Applies variants saved by save_variants (a profiling run then times the same kernels
+    without an autotuner pass of its own)."""
+    from veloxchem_amd import _native
+    with open(path) as fh:
+        d = json.load(fh)
+    cls, pc = _native.as_int(d['classes'])
+    var, pv = _native.as_int(d['variants'])
+    ms, pm = _native.as_double(d['ms'])
+    _native.check(_native.lib().vlx_fock_set_variants(_native.context(0), scr.handle, len(var), pc, pv, pm))
+    return {tuple(int(x) for x in cls[4 * r:4 * r + 4]): int(var[r]) for r in range(len(var))}
+
+
+def workload_builds(nmol, box, seed, basis, ithr, steps, warmup, taus, autotune_repeats, workload_key,
+                    save_tune=None, load_tune=None, single_stream=True):
     """FP64 builds of one seeded water cluster after its own autotuner pass: rate, per-class kernel
     times and the roofline of the dominant class kernel (timed region, four streams), the same
     kernel alone on one stream, and the mixed sweep with its error against FP64."""
@@ -231,11 +282,16 @@ def workload_builds(nmol, box, seed, basis, ithr, steps, warmup, taus, autotune_
     n = bas.get_dimensions_of_basis()
     log(f'{workload_key}: {n} AOs, screener {t_screen:.2f} s')
     tuned, t_tune = None, 0.0
-    if autotune_repeats > 0:
+    if load_tune:
+        tuned = load_variants(scr, load_tune)
+        log(f'{workload_key}: kernel variants of {len(tuned)} classes from {load_tune}')
+    elif autotune_repeats > 0:
         t0 = time.perf_counter()
         tuned = FockDriver()._autotune(scr, dm, ithr, autotune_repeats)
         t_tune = time.perf_counter() - t0
         log(f'{workload_key}: autotuned in {t_tune:.1f} s')
+        if save_tune:
+            save_variants(scr, save_tune)
     ms, times, g, sweep = measure_builds(scr, dm, ithr, steps, warmup, taus)
     log(f'{workload_key}: {1e3 / ms:.3f} builds/s over {steps} builds; mixed sweep of {len(taus)} thresholds done')
     work = scr.work(ithr)
@@ -243,22 +299,25 @@ def workload_builds(nmol, box, seed, basis, ithr, steps, warmup, taus, autotune_
     # the classes timed alone: one stream serialises the class kernels (in the timed region four
     # streams share the chip, so each launch's time includes its neighbours)
     lib, ctx = _native.lib(), _native.context(0)
-    _native.check(lib.vlx_context_set_streams(ctx, 1))
-    try:
-        ms1, times1, _, _ = measure_builds(scr, dm, ithr, 1, 1, [])
-    finally:
-        _native.check(lib.vlx_context_set_streams(ctx, max(1, min(4, int(os.environ.get('VLX_STREAMS', 4))))))
-    rf1 = roofline(scr, ithr, times1, ms1, workload_key)
+    rf1 = None
+    if single_stream:
+        _native.check(lib.vlx_context_set_streams(ctx, 1))
+        try:
+            ms1, times1, _, _ = measure_builds(scr, dm, ithr, 1, 1, [])
+        finally:
+            _native.check(lib.vlx_context_set_streams(ctx, max(1, min(4, int(os.environ.get('VLX_STREAMS', 4))))))
+        rf1 = roofline(scr, ithr, times1, ms1, workload_key)
     out = {'workload': f'{nmol}-water cluster (seed {seed}, {box} A box), {basis}, {n} AOs, 2jk FP64 build on the '
                        f'SAD density, ithreshold {ithr}',
            'nao': n, 'builds_per_s': 1e3 / ms, 'ms_per_step': ms, 'steps': steps, 'warmup': warmup,
            'shell_quartets': int(sum(w[1] for w in work)), 'primitive_quartets': int(sum(w[2] for w in work)),
            'screener_setup_s': round(t_screen, 3), 'autotune_s': round(t_tune, 2),
            'roofline': rf,
-           'roofline_single_stream': {k: rf1[k] for k in ('kernel', 'class', 'achieved', 'frac', 'kernel_ms')}
-           | {'build_ms_one_stream': ms1},
            'class_ms': {'%d%d%d%d' % c: round(v[0] / v[1], 4) for c, v in sorted(times.items())},
            'checksum': float(np.sum(g * dm))}
+    if rf1 is not None:
+        out['roofline_single_stream'] = {k: rf1[k] for k in ('kernel', 'class', 'achieved', 'frac', 'kernel_ms')} | {
+            'build_ms_one_stream': ms1}
     if tuned:
         out['autotune'] = {'%d%d%d%d' % c: v for c, v in sorted(tuned.items())}
     if sweep:
@@ -427,11 +486,21 @@ def main():
     ap.add_argument('--no-svp', action='store_true', help='skip the 32-water def2-SVP builds and SCF')
     ap.add_argument('--no-scf', action='store_true')
     ap.add_argument('--svp-steps', type=int, default=10)
+    ap.add_argument('--save-tune', default=None, help='write the headline workload\'s autotuned variants here')
+    ap.add_argument('--load-tune', default=None,
+                    help='take the headline workload\'s variants from this file instead of an autotuner pass '
+                         '(profiling runs: the profile then holds the timed builds only)')
+    ap.add_argument('--no-single-stream', action='store_true',
+                    help='skip the single-stream measurement of the dominant class (profiling runs)')
     ap.add_argument('--no-batch', action='store_true',
                     help='skip the batched workload of 256 small water clusters')
     ap.add_argument('--batch-clusters', type=int, default=256)
     ap.add_argument('--batch-reps', type=int, default=3)
     ap.add_argument('--no-guess', action='store_true', help='skip the learned-guess training and evaluation')
+    ap.add_argument('--density-tau', type=float, default=1e-10,
+                    help='density-weighted screening threshold of the fast SCF mode')
+    ap.add_argument('--no-tzvp-scf', action='store_true',
+                    help='skip the FP64 and fast SCFs of the 64-water def2-TZVP cluster')
     ap.add_argument('--guess-train', type=int, default=96, help='training clusters of the learned guess')
     ap.add_argument('--guess-eval', type=int, default=6, help='held-out evaluation clusters of the learned guess')
     ap.add_argument('--guess-epochs', type=int, default=60)
@@ -449,11 +518,13 @@ def main():
     atr = 0 if args.no_autotune else 1
     if args.svp_only:
         svp, _ = workload_builds(32, 10.0, 32, 'def2-svp', args.ithreshold, args.steps, args.warmup, [],
-                                 0 if args.no_autotune else 2, 'water32_svp')
+                                 0 if args.no_autotune else 2, 'water32_svp', args.save_tune, args.load_tune,
+                                 not args.no_single_stream)
         print(json.dumps({'svp32': svp}))
         return
     head, (mol, bas, scr, dm, g) = workload_builds(64, 12.8, 64, 'def2-tzvp', args.ithreshold, args.steps,
-                                                   args.warmup, taus, atr, 'water64_tzvp')
+                                                   args.warmup, taus, atr, 'water64_tzvp', args.save_tune,
+                                                   args.load_tune, not args.no_single_stream)
     result = {
         'metric': 'J+K Fock builds/s + SCF time, mixed vs FP64 on GPU, at |\u0394E|<=1e-6 Eh',
         'mode': 'fp64 2jk build of the 64-water def2-TZVP cluster (value); mixed sweep on the same density',
@@ -501,6 +572,10 @@ def main():
                         mixed = run_scf(smol, sbas, args.scf_conv, 'mixed', tau, incremental=incr)
                         mixed['dE_vs_fp64'] = mixed['energy'] - result['scf']['energy']
                         result[key] = mixed
+                    result['scf_fast_vs_fp64'] = fast_vs_fp64_scf(smol, sbas, args.scf_conv, tau, args.density_tau)
+        if not args.no_tzvp_scf and taus:
+            log('64-water def2-TZVP SCFs: FP64 and fast')
+            result['scf_tzvp_fast_vs_fp64'] = fast_vs_fp64_scf(mol, bas, args.scf_conv, max(taus), args.density_tau)
         if not args.no_batch:
             log('batched clusters')
             result['batched_clusters'] = batched_clusters(args)

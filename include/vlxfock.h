@@ -57,6 +57,19 @@ int vlx_set_block_size_factor(vlx_context* ctx, int factor);
  */
 enum { VLX_PRECISION_FP64 = 0, VLX_PRECISION_MIXED = 1 };
 int vlx_set_precision(vlx_context* ctx, int mode, double threshold);
+/*
+ * Density-weighted screening (no reference counterpart: a fast mode whose
+ * error is stated against the FP64 build).  threshold > 0: a quartet
+ * (ab|cd) is skipped when its Schwarz bound 10^(2 - bin_ab - bin_cd) times
+ * the largest |D| of the six D blocks its J and K terms read (D_cd, D_ab,
+ * D_bd, D_bc, D_ad, D_ac, taken per pair of the screener's atom groups) is
+ * below threshold; 0 (default) turns it off, and the build is the
+ * reference's again.  Applies to single-molecule builds with ithreshold >= 0
+ * (not to batched builds, nor to vlx_fock_compute_device_noalloc, which
+ * refuses it).  With a difference density (incremental SCF builds) the
+ * bound follows |Delta D| block by block.
+ */
+int vlx_set_density_screening(vlx_context* ctx, double threshold);
 /* Bin-sum threshold the last build used (99 = no FP32 work). */
 int vlx_context_last_fp32_bin_sum(vlx_context* ctx, int* k32);
 /* Primitive slots of one build run in FP64 and in FP32 at bin-sum threshold k32. */
@@ -228,6 +241,17 @@ int vlx_fock_schedule_stats(vlx_screener* scr, int ithreshold, int max_records, 
  */
 int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_density, int ithreshold, int repeats,
                       int* nchosen, int* classes, int* variants);
+/* The per-class kernel variants a screener uses (set by vlx_fock_autotune or
+ * vlx_fock_set_variants): *nrecords records of classes (4 ints), variants and
+ * the measured milliseconds of each class under its variant (0 when not
+ * measured); at most max_records are written.  No reference counterpart. */
+int vlx_fock_get_variants(vlx_screener* scr, int max_records, int* nrecords, int* classes, int* variants, double* ms);
+/* Applies saved autotuner choices to a screener (e.g. from another process of
+ * the same workload): variants per class and, optionally (ms may be NULL),
+ * their measured times, which balance the classes over the side streams.
+ * Classes not listed use the default rule.  No reference counterpart. */
+int vlx_fock_set_variants(vlx_context* ctx, vlx_screener* scr, int nrecords, const int* classes, const int* variants,
+                          const double* ms);
 /* Variants the last vlx_fock_autotune on this context rejected (bit v set for
  * variant v): the G of a variant's warm-up build is checked on the host against
  * the untiled variant's (non-finite elements, or a difference above 1e-10), and

@@ -82,9 +82,13 @@ def lib():
         L.vlx_fock_autotune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                         _c_int_p, _c_int_p, _c_int_p]
         L.vlx_fock_autotune_rejected.argtypes = [ctypes.c_void_p, _c_int_p]
+        L.vlx_fock_get_variants.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_double_p]
+        L.vlx_fock_set_variants.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_int_p,
+                                            _c_double_p]
         L.vlx_fock_class_kernel.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.vlx_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.vlx_set_density_screening.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.vlx_context_last_fp32_bin_sum.argtypes = [ctypes.c_void_p, _c_int_p]
         L.vlx_fock_precision_split.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_ll_p, _c_ll_p]
         L.vlx_fock_precision_classes.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p,
@@ -132,6 +136,12 @@ def set_precision(mode='fp64', threshold=0.0, device=0):
     if mode not in PRECISION_MODES:
         raise ValueError(f'unknown precision mode {mode!r}')
     check(lib().vlx_set_precision(context(device), PRECISION_MODES[mode], float(threshold)))
+
+
+def set_density_screening(threshold=0.0, device=0):
+    """Density-weighted screening of later builds: quartets whose Schwarz bound times the largest |D|
+    of the blocks they read is below threshold are skipped (0: off, the reference's screening)."""
+    check(lib().vlx_set_density_screening(context(device), float(threshold)))
 
 
 def last_fp32_bin_sum(device=0):

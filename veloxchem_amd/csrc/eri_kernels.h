@@ -193,6 +193,19 @@ __device__ __forceinline__ void stage_boys(R* lds, const R* __restrict__ tab)
     __syncthreads();
 }
 
+// Density-weighted screening (a non-reference fast mode): glim[g1 * ng + g2] is the largest bin sum
+// a quartet may have when it reads the D block of atom groups (g1, g2), from that block's max|D|;
+// a quartet of groups (A, B|C, D) reads D_CD, D_AB (Coulomb) and D_BD, D_BC, D_AD, D_AC (exchange),
+// so it is kept up to the largest of those six limits.
+__device__ __forceinline__ int quartet_bin_limit(const int* __restrict__ glim, const int ng, const int A, const int B,
+                                                 const int C, const int D)
+{
+    const int l0 = max(glim[C * ng + D], glim[A * ng + B]);
+    const int l1 = max(glim[B * ng + D], glim[B * ng + C]);
+    const int l2 = max(glim[A * ng + D], glim[A * ng + C]);
+    return max(l0, max(l1, l2));
+}
+
 // ---------------------------------------------------------------- Fock
 //
 // Ket-stationary digestion.  Lane j owns ket pair (c, d) for a whole work
@@ -229,7 +242,7 @@ __global__ void __launch_bounds__(VLX_BLOCK)
                 const PairInfo* __restrict__ ket, const PrimPair* __restrict__ ket_prims,
                 const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
                 const double* __restrict__ D0, double* __restrict__ F0, const MolDesc* __restrict__ mols,
-                const FockScalars a)
+                const int* __restrict__ glim, const FockScalars a)
 {
     constexpr int na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
     constexpr int nsph = na * nb * nc * nd;
@@ -291,6 +304,9 @@ __global__ void __launch_bounds__(VLX_BLOCK)
             }
             bool active = lane_ok;
             if (a.ithreshold >= 0) active = active && (bp.bin + kp.bin <= a.ithreshold);
+            if (glim != nullptr)
+                active = active && (bp.bin + kp.bin <=
+                                     quartet_bin_limit(glim, a.ngroups, bp.grp_a, bp.grp_b, kp.grp_a, kp.grp_b));
             if (a.same_class) active = active && (bp.rank >= j);
             double s = active ? 1.0 : 0.0;
             if (bp.same) s *= 0.5;
@@ -404,7 +420,7 @@ __global__ void __launch_bounds__(VLX_BLOCK)
                    const PairInfo* __restrict__ ket, const PrimPair* __restrict__ ket_prims,
                    const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
                    const double* __restrict__ D0, double* __restrict__ F0, const MolDesc* __restrict__ mols,
-                   const FockScalars a)
+                   const int* __restrict__ glim, const FockScalars a)
 {
     constexpr int na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
     constexpr int nsph = na * nb * nc * nd;
@@ -425,8 +441,13 @@ __global__ void __launch_bounds__(VLX_BLOCK)
         const int n = md.nao;
         const PairInfo bp = bra[wi.i0];
         const int j = wi.j0 + lane;
-        const bool active = j < wi.i1;
+        bool active = j < wi.i1;
         const PairInfo kp = ket[active ? j : wi.j0];
+        if (glim != nullptr) {
+            active = active &&
+                     (bp.bin + kp.bin <= quartet_bin_limit(glim, a.ngroups, bp.grp_a, bp.grp_b, kp.grp_a, kp.grp_b));
+            if (__ballot(active) == 0) continue;  // wave-uniform: the item has no quartet left
+        }
         double s = active ? 1.0 : 0.0;
         if (bp.same) s *= 0.5;
         if (kp.same) s *= 0.5;
@@ -550,7 +571,7 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                       const PairInfo* __restrict__ pairs_ket, const PrimPair* __restrict__ ket_prims,
                       const TileItem* __restrict__ items, const int2* __restrict__ kentries,
                       const R* __restrict__ boys_tab, const double* __restrict__ D0, double* __restrict__ F0,
-                      const MolDesc* __restrict__ mols, const int* __restrict__ k32,
+                      const MolDesc* __restrict__ mols, const int* __restrict__ k32, const int* __restrict__ glim,
                       const int* __restrict__ grp_off, const int* __restrict__ grp_n,
                       const int* __restrict__ grp_ao, const FockScalars a)
 {
@@ -585,6 +606,21 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
             if (sizeof(R) == sizeof(double)) b1 = lo;
             else b0 = lo;
             if (b0 >= b1) continue;  // block-uniform: every thread skips the item together
+        }
+        // density-weighted screening: every quartet of the item reads the D blocks of its four atom
+        // groups, so the item has one bin-sum limit; bras past it (sorted by bin) are cut off
+        int ilim = a.ithreshold;
+        if (glim != nullptr) {
+            ilim = min(ilim, quartet_bin_limit(glim, a.ngroups, ti.ga, ti.gb, ti.gc, ti.gd));
+            const int bl = ilim - ti.bin_lo;  // a bra needs bin <= bl for one quartet with the item's kets
+            int lo = b0, hi = b1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (pairs_bra[mid].bin <= bl) lo = mid + 1;
+                else hi = mid;
+            }
+            b1 = lo;
+            if (b0 >= b1) continue;  // block-uniform
         }
         const int nA = grp_n[ti.ga * 4 + la], nB = grp_n[ti.gb * 4 + lb];
         const int nC = grp_n[ti.gc * 4 + lc], nD = grp_n[ti.gd * 4 + ld];
@@ -634,7 +670,7 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
         for (int ib = b0 + wave; ib < b1; ib += NW) {
             const PairInfo bp = pairs_bra[ib];
             bool active = lane_ok;
-            if (a.ithreshold >= 0) active = active && (bp.bin + kp.bin <= a.ithreshold);
+            if (a.ithreshold >= 0) active = active && (bp.bin + kp.bin <= ilim);
             if (a.same_class) active = active && (bp.rank >= kp.rank);
             R s = active ? R(1) : R(0);
             if (bp.same) s *= R(0.5);
@@ -830,11 +866,11 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
         if (a.sc.omega2 > 0.0)                                                                          \
             hipLaunchKernelGGL((kernel<la, lb, lc, ld, true>), dim3(grid), dim3(VLX_BLOCK), 0, s, a.bra, \
                                a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F, a.mols, \
-                               a.sc);                                                                   \
+                               a.glim, a.sc);                                                           \
         else                                                                                            \
             hipLaunchKernelGGL((kernel<la, lb, lc, ld, false>), dim3(grid), dim3(VLX_BLOCK), 0, s,      \
                                a.bra, a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F,   \
-                               a.mols, a.sc);                                                           \
+                               a.mols, a.glim, a.sc);                                                   \
     }                                                                                                   \
     void launch_eri_##la##lb##lc##ld(const EriArgs& a, hipStream_t s)                                   \
     {                                                                                                   \
@@ -851,7 +887,7 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
 #define VLX_TILED_LAUNCH(la, lb, lc, ld, R, NW, RS, tab)                                                \
     hipLaunchKernelGGL((fock_kernel_tiled<la, lb, lc, ld, R, NW, RS>), dim3(grid), dim3((NW) * VLX_WAVE), \
                        a.tile_lds_bytes, s, a.bra, a.bra_prims, a.ket, a.ket_prims, a.titems, a.kentries,  \
-                       tab, a.D, a.F, a.mols, a.k32, a.grp_off, a.grp_n, a.grp_ao, a.sc)
+                       tab, a.D, a.F, a.mols, a.k32, a.glim, a.grp_off, a.grp_n, a.grp_ao, a.sc)
 #define VLX_INSTANTIATE_TILED(la, lb, lc, ld, NWB)                                                      \
     void launch_tiled_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                      \
     {                                                                                                   \

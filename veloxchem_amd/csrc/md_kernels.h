@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
                     const PrimPair* __restrict__ ket_prims, const double* __restrict__ ket_u,
                     const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
                     const double* __restrict__ D0, double* __restrict__ F0, const MolDesc* __restrict__ mols,
-                    const FockScalars a)
+                    const int* __restrict__ glim, const FockScalars a)
 {
     using S = MmdSizes<la, lb, lc, ld>;
     constexpr int L = S::L;
@@ -421,6 +421,10 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
         const PairInfo bp = bra[wi.i0];
         for (int j = wi.j0; j < wi.i1; j++) {
             const PairInfo kp = ket[j];
+            // density-weighted screening (wave-uniform: one quartet per wavefront)
+            if (glim != nullptr &&
+                bp.bin + kp.bin > quartet_bin_limit(glim, a.ngroups, bp.grp_a, bp.grp_b, kp.grp_a, kp.grp_b))
+                continue;
             // one 16-wide cd tile at a time: its accumulators persist over all primitive quartets
             for (int nt = 0; nt < S::NT; nt++) {
                 vlx_d4 acc[S::MT];
@@ -634,7 +638,7 @@ __global__ void __launch_bounds__(VLX_WAVE) diag_kernel_md(const DiagArgs a)
 #define VLX_MMD_LAUNCH(la, lb, lc, ld, RS)                                                              \
     hipLaunchKernelGGL((fock_kernel_mmd<la, lb, lc, ld, RS>), dim3(g), dim3(VLX_WAVE), 0, s, a.bra,     \
                        a.bra_prims, a.bra_u, a.ket, a.ket_prims, a.ket_u, a.items, a.boys_tab, a.D, a.F, \
-                       a.mols, a.sc)
+                       a.mols, a.glim, a.sc)
 // launch_mmd_: the matrix-core kernel as an alternative variant of a class
 // whose default kernel is generated (same MD work items).
 #define VLX_INSTANTIATE_MMD(la, lb, lc, ld)                                                             \

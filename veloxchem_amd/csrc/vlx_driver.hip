@@ -304,6 +304,11 @@ struct vlx_context {
     DevBuf<unsigned long long> d_dmax;  // max |D| of the current build (bit pattern)
     double mixed_tau = 0.0;         // mixed precision: FP32 below this Schwarz x max|D| bound; 0 = FP64 only
     int last_k32 = NO_FP32;         // bin-sum threshold of the last build
+    // density-weighted screening (a non-reference fast mode): quartets whose Schwarz bound times the
+    // largest |D| of the blocks they read is below dscreen_tau are skipped; 0 = off
+    double dscreen_tau = 0.0;
+    DevBuf<unsigned long long> d_gmax;  // per pair of atom groups: max|D| (bit patterns)
+    DevBuf<int> d_glim;                 // per pair of atom groups: largest bin sum kept
     int block_size_factor = 8;
     // optional per-class timing
     bool timing = false;
@@ -346,6 +351,7 @@ struct vlx_screener {
     std::map<int, double> tuned_ms;  // class entry -> its measured time (ms) under that variant
     DevBuf<double> d_faccum;  // F' of this screener's molecule in batched builds
     DevBuf<MolDesc> d_mol;    // {0, nao}: the molecule table of a single-molecule build
+    DevBuf<int> d_ao_group;   // atom group of every AO (density-weighted screening)
     int epoch = 0;            // incremented when the autotuner changes the variants (batch plans)
 };
 
@@ -649,6 +655,41 @@ __global__ void absmax_kernel(const double* __restrict__ x, size_t n, unsigned l
 }
 
 __global__ void clear_u64_kernel(unsigned long long* __restrict__ p) { *p = 0ull; }
+
+__global__ void clear_u64_n_kernel(unsigned long long* __restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// max |D_pq| per pair of atom groups (both orders, bit patterns of non-negative doubles)
+__global__ void group_dmax_kernel(const double* __restrict__ d, int n, const int* __restrict__ ao_group, int ng,
+                                  unsigned long long* __restrict__ gmax)
+{
+    const size_t nn = (size_t)n * n;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nn; i += (size_t)gridDim.x * blockDim.x) {
+        const double v = fabs(d[i]);
+        if (v == 0.0) continue;
+        const int gp = ao_group[i / n], gq = ao_group[i % n];
+        const auto bits = static_cast<unsigned long long>(__double_as_longlong(v));
+        atomicMax(gmax + gp * ng + gq, bits);
+        atomicMax(gmax + gq * ng + gp, bits);
+    }
+}
+
+// The largest bin sum kept for a quartet reading a D block of max|D| = M: Schwarz bins bound the
+// quartet by 10^(2 - bin sum), so it stays while 10^(2 - s) M >= tau, s <= floor(2 + log10(M / tau));
+// never above ithreshold; -1000 (nothing kept) for a zero block.
+__global__ void group_lim_kernel(const unsigned long long* __restrict__ gmax, int n, double tau, int ithreshold,
+                                 int* __restrict__ glim)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double m = __longlong_as_double(static_cast<long long>(gmax[i]));
+    int lim = -1000;
+    if (m > 0.0) lim = static_cast<int>(fmin(static_cast<double>(ithreshold), floor(2.0 + log10(m / tau))));
+    glim[i] = lim;
+}
 
 // G = F' + F'^T
 __global__ void symmetrize_kernel(const double* __restrict__ f, double* __restrict__ g, int n)
@@ -1058,6 +1099,15 @@ int vlx_set_precision(vlx_context* ctx, int mode, double threshold)
     return VLX_OK;
 }
 
+int vlx_set_density_screening(vlx_context* ctx, double threshold)
+{
+    if (!ctx) return fail(VLX_ERR_ARG, "context is NULL");
+    if (!(threshold >= 0.0) || !std::isfinite(threshold))
+        return fail(VLX_ERR_ARG, "density screening threshold must be finite and >= 0 (0 = off)");
+    ctx->dscreen_tau = threshold;
+    return VLX_OK;
+}
+
 int vlx_context_last_fp32_bin_sum(vlx_context* ctx, int* k32)
 {
     if (!ctx || !k32) return fail(VLX_ERR_ARG, "NULL argument");
@@ -1251,6 +1301,12 @@ int vlx_screener_create(vlx_context* ctx, int nshells, const int* l, const int* 
         HIP_TRY(s->d_grp_ao.upload(s->grp_ao));
     }
     HIP_TRY(s->d_mol.upload(std::vector<MolDesc>{MolDesc{0, s->nao, 0}}));
+    {
+        std::vector<int> ao_group(nao, 0);
+        for (int sh = 0; sh < nshells; sh++)
+            for (int m = 0; m < 2 * l[sh] + 1; m++) ao_group[ao_base[sh] + m * ao_stride[sh]] = s->shell_group[sh];
+        HIP_TRY(s->d_ao_group.upload(ao_group));
+    }
     const int npc = npair_classes();
     s->all.resize(npc);
     s->kept_tiled.resize(npc);
@@ -1571,6 +1627,62 @@ int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_densi
     return VLX_OK;
 }
 
+int vlx_fock_get_variants(vlx_screener* scr, int max_records, int* nrecords, int* classes, int* variants, double* ms)
+{
+    if (!scr || !nrecords) return fail(VLX_ERR_ARG, "NULL argument");
+    int r = 0;
+    for (const auto& kv : scr->variant) {
+        if (r < max_records) {
+            const auto& ce = g_class_table[kv.first];
+            if (classes) {
+                classes[4 * r] = ce.la;
+                classes[4 * r + 1] = ce.lb;
+                classes[4 * r + 2] = ce.lc;
+                classes[4 * r + 3] = ce.ld;
+            }
+            if (variants) variants[r] = kv.second;
+            if (ms) {
+                const auto it = scr->tuned_ms.find(kv.first);
+                ms[r] = it != scr->tuned_ms.end() ? it->second : 0.0;
+            }
+        }
+        r++;
+    }
+    *nrecords = r;
+    return VLX_OK;
+}
+
+int vlx_fock_set_variants(vlx_context* ctx, vlx_screener* scr, int nrecords, const int* classes, const int* variants,
+                          const double* ms)
+{
+    if (!ctx || !scr || nrecords < 0 || (nrecords > 0 && (!classes || !variants)))
+        return fail(VLX_ERR_ARG, "NULL argument to vlx_fock_set_variants");
+    if (scr->ctx != ctx) return fail(VLX_ERR_ARG, "screener belongs to another context");
+    std::map<int, int> var;
+    std::map<int, double> tms;
+    const int nentries = static_cast<int>(sizeof(g_class_table) / sizeof(g_class_table[0]));
+    for (int r = 0; r < nrecords; r++) {
+        int e = -1;
+        for (int k = 0; k < nentries && e < 0; k++) {
+            const auto& ce = g_class_table[k];
+            if (ce.la == classes[4 * r] && ce.lb == classes[4 * r + 1] && ce.lc == classes[4 * r + 2] &&
+                ce.ld == classes[4 * r + 3])
+                e = k;
+        }
+        if (e < 0)
+            return fail(VLX_ERR_ARG, "no class (%d %d|%d %d) in the kernel table", classes[4 * r], classes[4 * r + 1],
+                        classes[4 * r + 2], classes[4 * r + 3]);
+        if (variants[r] < 0 || variants[r] >= NVARIANTS) return fail(VLX_ERR_ARG, "variant %d out of range", variants[r]);
+        var[e] = variants[r];
+        if (ms) tms[e] = ms[r];
+    }
+    scr->variant = var;
+    scr->tuned_ms = tms;
+    scr->epoch++;
+    scr->schedules.clear();
+    return VLX_OK;
+}
+
 int vlx_fock_autotune_rejected(vlx_context* ctx, int* mask)
 {
     if (!ctx || !mask) return fail(VLX_ERR_ARG, "NULL argument");
@@ -1644,7 +1756,7 @@ namespace {
 // that run alone (stream -1), 0 only the others, -1 all.  No synchronisation.
 int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::unique_ptr<ClassSchedule>>& list,
                     const double* d_density, double* F, double fj, double fk, double omega2, int ithreshold,
-                    hipStream_t stream, int solo = -1, bool allow_timing = true)
+                    hipStream_t stream, int solo = -1, bool allow_timing = true, const int* glim = nullptr)
 {
     const bool timing = ctx->timing && allow_timing;
     const bool unscreened = ithreshold < 0;
@@ -1671,6 +1783,8 @@ int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::
         a.F = F;
         a.mols = scr->d_mol.ptr;
         a.k32 = nullptr;
+        a.glim = glim;
+        a.sc.ngroups = scr->ngroups;
         a.titems = sch.d_titems.ptr;
         a.kentries = sch.singles ? ket.singles.d_kentries.ptr : ket.runs.d_kentries.ptr;
         a.grp_off = scr->d_grp_off.ptr;
@@ -1772,16 +1886,33 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
         }
     }
     ctx->last_k32 = k32;
+    // density-weighted screening: bin-sum limits per pair of atom groups from this D
+    const int* glim = nullptr;
+    if (ctx->dscreen_tau > 0.0 && ithreshold >= 0) {
+        if (no_alloc)
+            return fail(VLX_ERR_ARG, "density-weighted screening is not available in the no-allocation build");
+        const int ng = scr->ngroups;
+        const size_t ng2 = (size_t)ng * ng;
+        HIP_TRY(ctx->d_gmax.alloc(ng2));
+        HIP_TRY(ctx->d_glim.alloc(ng2));
+        clear_u64_n_kernel<<<static_cast<int>((ng2 + 255) / 256), 256, 0, ctx->stream>>>(ctx->d_gmax.ptr, (int)ng2);
+        group_dmax_kernel<<<std::min<size_t>((nn + 255) / 256, 2048), 256, 0, ctx->stream>>>(
+            d_density, n, scr->d_ao_group.ptr, ng, ctx->d_gmax.ptr);
+        group_lim_kernel<<<static_cast<int>((ng2 + 255) / 256), 256, 0, ctx->stream>>>(
+            ctx->d_gmax.ptr, (int)ng2, ctx->dscreen_tau, ithreshold, ctx->d_glim.ptr);
+        HIP_TRY(hipGetLastError());
+        glim = ctx->d_glim.ptr;
+    }
     std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
     rc = get_schedules(scr, ithreshold, k32, &list, !no_alloc);
     if (rc) return rc;
-    rc = enqueue_classes(ctx, scr, *list, d_density, facc, fj, fk, omega2, ithreshold, nullptr, 1, !no_alloc);
+    rc = enqueue_classes(ctx, scr, *list, d_density, facc, fj, fk, omega2, ithreshold, nullptr, 1, !no_alloc, glim);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
     for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamWaitEvent(ctx->side[k], ctx->ev_start, 0));
     // the side streams join the context stream also when a launch fails part-way, so that
     // work ordered on the context stream (copies, frees) never overtakes a queued kernel
-    rc = enqueue_classes(ctx, scr, *list, d_density, facc, fj, fk, omega2, ithreshold, nullptr, 0, !no_alloc);
+    rc = enqueue_classes(ctx, scr, *list, d_density, facc, fj, fk, omega2, ithreshold, nullptr, 0, !no_alloc, glim);
     const int rj = join_side_streams(ctx);
     if (rc) return rc;
     if (rj) return rj;
@@ -1834,12 +1965,6 @@ __global__ void batch_symmetrize_kernel(const double* __restrict__ f, const MolD
         const size_t p = i / n, q = i % n;
         g[i] = fm[i] + fm[q * n + p];
     }
-}
-
-__global__ void clear_u64_n_kernel(unsigned long long* __restrict__ p, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0ull;
 }
 
 // max |D_m| per molecule (bit patterns of non-negative doubles order like their values)
@@ -2114,6 +2239,8 @@ int record_batch(vlx_context* ctx, BatchPlan& P)
         a.F = P.d_F.ptr;
         a.mols = P.d_mols.ptr;
         a.k32 = nullptr;
+        a.glim = nullptr;  // density-weighted screening: single-molecule builds only
+        a.sc.ngroups = 0;
         a.sc.nitems = static_cast<long long>(L.tiled ? L.titems.size() : L.items.size());
         a.sc.nket = ket.size();
         a.sc.ithreshold = P.ithreshold;

@@ -76,6 +76,7 @@ struct FockScalars {
     int ithreshold;                // < 0: no Schwarz selection
     int same_class;                // bra and ket lists hold the same pairs (quartets with rank_bra >= rank_ket)
     int nao;
+    int ngroups;                   // atom groups of the molecule (density-weighted screening table side)
     double fj;                     // coefficient of the Coulomb terms (0: none)
     double fk;                     // coefficient of the exchange terms (0: none)
     double omega2;                 // omega^2 of the erf(omega r)/r operator (range-separated labels), 0: 1/r
@@ -101,6 +102,8 @@ struct FockArgs {
     const double* D;               // densities of the launch's molecules (MolDesc layout)
     double* F;                     // F' accumulators (G = F' + F'^T), same layout
     const MolDesc* mols;           // per molecule: offset and size of its D and F'
+    const int* glim;               // density-weighted screening: ngroups x ngroups largest bin sums per
+                                   // pair of atom groups (from max|D| of the block), or nullptr (off)
     const int* k32;                // per molecule: FP32 bin-sum threshold on the device, or nullptr
                                    // (tiled items: the FP64 kernel runs the item's bras with
                                    // bin + bin_lo < k32, the FP32 kernel the others)

@@ -150,6 +150,11 @@ class ScfRestrictedDriver:
         # iterations, when the ERI threshold changes, and to confirm convergence
         self.incremental_fock = False
         self.full_fock_every = 8
+        # density-weighted screening of the builds (a fast mode, no reference counterpart): quartets
+        # whose Schwarz bound x the largest |D| (|dD| in incremental builds) of the blocks they read
+        # is below this are skipped; 0 = off.  The converged density is confirmed by one unscreened
+        # full build, which gives the reported energy.
+        self.density_screening = 0.0
         self.screener_factory = None
         # starting density: None for the reference's guesses (SAD, L2-DIIS valence stage), or an
         # object with supports(molecule, basis) and density(molecule, basis, overlap) (the learned
@@ -202,6 +207,7 @@ class ScfRestrictedDriver:
             self.fock_2e = _DeviceFock(dev)
             self.device = torch.device('cuda', dev)
             _native.set_precision(self.precision, self.mixed_threshold if self.precision == 'mixed' else 0.0, dev)
+            _native.set_density_screening(self.density_screening, dev)
         elif self.device is None:
             self.device = torch.device('cpu')
         t0 = time.perf_counter()
@@ -212,6 +218,7 @@ class ScfRestrictedDriver:
         finally:
             if isinstance(self.fock_2e, _DeviceFock):
                 _native.set_precision('fp64', 0.0, self.fock_2e.device)
+                _native.set_density_screening(0.0, self.fock_2e.device)
         self.timings['total'] = time.perf_counter() - t0
         res['timings'] = dict(self.timings)
         res['num_iter'] = self._num_iter
@@ -315,6 +322,14 @@ class ScfRestrictedDriver:
                 build_thresh = thresh_int + math.ceil(math.log10(dmax)) if dmax > 0.0 else -1
                 G = g_prev + self.fock_2e(screener, dd, build_thresh) if build_thresh >= 0 else g_prev.clone()
                 since_full += 1
+            elif confirm and self.density_screening > 0.0 and isinstance(self.fock_2e, _DeviceFock):
+                # the confirming build of a density-screened SCF runs unscreened
+                _native.set_density_screening(0.0, self.fock_2e.device)
+                try:
+                    G = self.fock_2e(screener, den, thresh_int)
+                finally:
+                    _native.set_density_screening(self.density_screening, self.fock_2e.device)
+                full_thresh, since_full = thresh_int, 0
             else:
                 G = self.fock_2e(screener, den, thresh_int)
                 full_thresh, since_full = thresh_int, 0
@@ -334,8 +349,8 @@ class ScfRestrictedDriver:
             self._scf_energy = e_scf
             if it > 0 and e_grad < conv_thresh:
                 self.timings['linear_algebra'] += time.perf_counter() - t
-                if incremental:
-                    confirm = True  # the same density once more with a full build
+                if incremental or (self.density_screening > 0.0 and not confirm):
+                    confirm = True  # the same density once more with a full (unscreened) build
                     continue
                 self._is_converged = True
                 break
