@@ -295,6 +295,7 @@ def workload_builds(nmol, box, seed, basis, ithr, steps, warmup, taus, autotune_
     ms, times, g, sweep = measure_builds(scr, dm, ithr, steps, warmup, taus)
     log(f'{workload_key}: {1e3 / ms:.3f} builds/s over {steps} builds; mixed sweep of {len(taus)} thresholds done')
     work = scr.work(ithr)
+    class_flops = ref_scheme_flops(work)
     rf = roofline(scr, ithr, times, ms, workload_key)
     # the classes timed alone: one stream serialises the class kernels (in the timed region four
     # streams share the chip, so each launch's time includes its neighbours)
@@ -314,10 +315,16 @@ def workload_builds(nmol, box, seed, basis, ithr, steps, warmup, taus, autotune_
            'screener_setup_s': round(t_screen, 3), 'autotune_s': round(t_tune, 2),
            'roofline': rf,
            'class_ms': {'%d%d%d%d' % c: round(v[0] / v[1], 4) for c, v in sorted(times.items())},
+           # reference-scheme TFLOP/s of each class kernel in the timed region (shared chip)
+           'class_tflops': {'%d%d%d%d' % c: round(class_flops[c] / (v[0] / v[1] * 1e-3) / 1e12, 3)
+                            for c, v in sorted(times.items()) if c in class_flops},
            'checksum': float(np.sum(g * dm))}
     if rf1 is not None:
         out['roofline_single_stream'] = {k: rf1[k] for k in ('kernel', 'class', 'achieved', 'frac', 'kernel_ms')} | {
             'build_ms_one_stream': ms1}
+        out['class_ms_one_stream'] = {'%d%d%d%d' % c: round(v[0] / v[1], 4) for c, v in sorted(times1.items())}
+        out['class_tflops_one_stream'] = {'%d%d%d%d' % c: round(class_flops[c] / (v[0] / v[1] * 1e-3) / 1e12, 3)
+                                          for c, v in sorted(times1.items()) if c in class_flops}
     if tuned:
         out['autotune'] = {'%d%d%d%d' % c: v for c, v in sorted(tuned.items())}
     if sweep:
@@ -501,9 +508,9 @@ def main():
                     help='density-weighted screening threshold of the fast SCF mode')
     ap.add_argument('--no-tzvp-scf', action='store_true',
                     help='skip the FP64 and fast SCFs of the 64-water def2-TZVP cluster')
-    ap.add_argument('--guess-train', type=int, default=96, help='training clusters of the learned guess')
+    ap.add_argument('--guess-train', type=int, default=160, help='training clusters of the learned guess')
     ap.add_argument('--guess-eval', type=int, default=6, help='held-out evaluation clusters of the learned guess')
-    ap.add_argument('--guess-epochs', type=int, default=60)
+    ap.add_argument('--guess-epochs', type=int, default=150)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU; the multi-GPU sharding is design only '
@@ -520,7 +527,7 @@ def main():
         svp, _ = workload_builds(32, 10.0, 32, 'def2-svp', args.ithreshold, args.steps, args.warmup, [],
                                  0 if args.no_autotune else 2, 'water32_svp', args.save_tune, args.load_tune,
                                  not args.no_single_stream)
-        print(json.dumps({'svp32': svp}))
+        print(json.dumps({'svp32': svp}, default=_plain))
         return
     head, (mol, bas, scr, dm, g) = workload_builds(64, 12.8, 64, 'def2-tzvp', args.ithreshold, args.steps,
                                                    args.warmup, taus, atr, 'water64_tzvp', args.save_tune,
@@ -586,7 +593,16 @@ def main():
             log('reference CPU build of the headline workload')
             result['cpu_baseline'] = cpu_baseline(mol, bas, dm, args.ithreshold, args.cpu_seconds, 'def2-tzvp', g,
                                                   repeats=args.cpu_repeats)
-    print(json.dumps(result))
+    print(json.dumps(result, default=_plain))
+
+
+def _plain(o):
+    """numpy scalars and arrays as plain JSON values."""
+    if isinstance(o, np.generic):
+        return o.item()
+    if isinstance(o, np.ndarray):
+        return o.tolist()
+    raise TypeError(f'{type(o).__name__} is not JSON serialisable')
 
 
 if __name__ == '__main__':

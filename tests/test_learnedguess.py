@@ -127,3 +127,32 @@ def test_trained_guess_cuts_scf_iterations():
         assert abs(res['learned'][1] - res['sad'][1]) < 1e-8
         cut.append(1.0 - res['learned'][0] / res['sad'][0])
     assert min(cut) > 0.0
+
+
+def test_baseline_means_and_fallback():
+    """The residual baseline of the trained guess: the mean target per (element pair, on-site,
+    nearest radial centre) class, the residual scale per output element, and, for a class unseen in
+    training, the nearest radial centre of its (element pair, on-site) group (host side only)."""
+    from veloxchem_amd.learnedguess import ELEMENTS, N_OUT, NRADIAL, LearnedGuess, sample_keys
+    lg = LearnedGuess.__new__(LearnedGuess)  # the baseline needs no device network
+    rng = np.random.Generator(np.random.PCG64(0))
+    ne = len(ELEMENTS) ** 2
+    X = np.zeros((60, N_IN), dtype=np.float32)
+    X[np.arange(60), rng.integers(0, 3, 60)] = 1.0
+    X[:, ne] = rng.integers(0, 2, 60)
+    X[np.arange(60), ne + 1 + rng.integers(0, 5, 60)] = 1.0
+    T = rng.normal(size=(60, N_OUT)).astype(np.float32)
+    lg.fit_baseline(X, T)
+    keys = sample_keys(X)
+    for k in np.unique(keys):
+        assert np.allclose(lg.baseline(X[keys == k])[0], T[keys == k].mean(axis=0), atol=1e-5)
+    resid = T - lg.baseline(X)
+    assert np.allclose(lg.scale, np.sqrt(np.mean(resid.astype(np.float64) ** 2, axis=0)) + 1e-7, rtol=1e-5)
+    # unseen radial centre 20: the group's nearest seen centre (the largest one drawn, <= 4)
+    Y = X[:1].copy()
+    Y[0, ne + 1:ne + 1 + NRADIAL] = 0.0
+    Y[0, ne + 1 + 20] = 1.0
+    group = sample_keys(Y)[0] // NRADIAL
+    seen = [k for k in np.unique(keys) if k // NRADIAL == group]
+    nearest = max(seen, key=lambda k: k % NRADIAL)
+    assert np.allclose(lg.baseline(Y)[0], T[keys == nearest].mean(axis=0), atol=1e-5)

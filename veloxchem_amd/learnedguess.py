@@ -236,21 +236,69 @@ def converged_density(molecule, basis, conv_thresh=1.0e-6):
     return res['D_alpha']
 
 
+def sample_keys(features):
+    """Baseline class of each sample: (element pair, on-site flag, nearest radial centre), one int."""
+    ne = len(ELEMENTS) ** 2
+    pair = np.argmax(features[:, :ne], axis=1)
+    onsite = (features[:, ne] > 0.5).astype(np.int64)
+    rbin = np.argmax(features[:, ne + 1:ne + 1 + NRADIAL], axis=1)
+    return (pair * 2 + onsite) * NRADIAL + rbin
+
+
 class LearnedGuess:
-    """The trained network and its use as an SCF starting density."""
+    """The trained network and its use as an SCF starting density.
+
+    The network learns a normalised residual: the target block minus the mean block of its
+    baseline class (element pair, on-site flag, distance to the nearest 0.19 Angstrom), divided per
+    output element by the residuals' root mean square over the training set.  BF16 carries about
+    three significant digits, so its rounding then falls on the residual, not on the whole block."""
 
     def __init__(self, n_hidden=256, max_batch=8192, seed=0, device=0):
         from .densitynet import DensityNet
         self.net = DensityNet(N_IN, n_hidden, N_OUT, MAX_BF, max_batch, seed=seed, device=device)
         self.history = []
+        self.base_keys = np.zeros(0, dtype=np.int64)
+        self.base_mean = np.zeros((0, N_OUT), dtype=np.float32)
+        self.scale = np.ones(N_OUT, dtype=np.float32)
+
+    def fit_baseline(self, features, targets):
+        """Mean block per baseline class and the residual scale per output element."""
+        keys = sample_keys(features)
+        self.base_keys, inv = np.unique(keys, return_inverse=True)
+        mean = np.zeros((len(self.base_keys), N_OUT))
+        np.add.at(mean, inv, targets.astype(np.float64))
+        mean /= np.bincount(inv)[:, None]
+        resid = targets - mean[inv]
+        self.base_mean = mean.astype(np.float32)
+        self.scale = (np.sqrt(np.mean(resid.astype(np.float64) ** 2, axis=0)) + 1e-7).astype(np.float32)
+
+    def baseline(self, features):
+        """Mean blocks of the samples' classes; a class unseen in training takes the nearest radial
+        centre of its (element pair, on-site) group, or zeros."""
+        keys = sample_keys(features)
+        out = np.zeros((len(keys), N_OUT), dtype=np.float32)
+        if len(self.base_keys) == 0:
+            return out
+        idx = np.clip(np.searchsorted(self.base_keys, keys), 0, len(self.base_keys) - 1)
+        hit = self.base_keys[idx] == keys
+        out[hit] = self.base_mean[idx[hit]]
+        groups = self.base_keys // NRADIAL
+        for k in np.nonzero(~hit)[0]:
+            same = np.nonzero(groups == keys[k] // NRADIAL)[0]
+            if len(same):
+                near = same[np.argmin(np.abs(self.base_keys[same] % NRADIAL - keys[k] % NRADIAL))]
+                out[k] = self.base_mean[near]
+        return out
 
     def supports(self, molecule, basis):
         return (basis.get_label().upper() == BASIS and
                 set(np.asarray(molecule.get_element_ids()).tolist()) <= set(ELEMENTS))
 
     def train(self, features, targets, shapes, epochs=40, batch=4096, lr=2e-3, seed=0, log=None):
-        """Adam over shuffled mini-batches with a cosine learning-rate decay; returns the last
-        epoch's mean loss."""
+        """Baseline fit, then Adam over shuffled mini-batches of the normalised residuals with a
+        cosine learning-rate decay; returns the last epoch's mean loss (on the normalised scale)."""
+        self.fit_baseline(features, targets)
+        targets = ((targets - self.baseline(features)) / self.scale).astype(np.float32)
         rng = np.random.Generator(np.random.PCG64(seed))
         n = len(features)
         steps = epochs * max(1, n // batch)
@@ -275,16 +323,33 @@ class LearnedGuess:
         return mean
 
     def predict_blocks(self, pdata):
-        return self.net.forward(pdata.features).cpu().numpy()
+        y = self.net.forward(pdata.features).cpu().numpy()
+        return y * self.scale + self.baseline(pdata.features)
 
-    def density(self, molecule, basis, overlap=None):
-        """Predicted alpha density; scaled so that tr(D S) = N_alpha when the overlap is given."""
+    def density(self, molecule, basis, overlap=None, purify=3):
+        """Predicted alpha density; with the overlap given, scaled so that tr(D S) = N_alpha and made
+        idempotent in the overlap metric by up to `purify` McWeeny steps D <- 3 DSD - 2 DSDSD (each
+        kept only while it lowers |DSD - D|; the predicted blocks are close to a converged density, so
+        the eigenvalues of DS start near 0 and 1, where the step converges quadratically)."""
         pd = PairData(molecule, basis)
         D = pd.assemble(self.predict_blocks(pd), basis.get_dimensions_of_basis())
         if overlap is not None:
+            nocc = molecule.number_of_alpha_electrons()
             ne = float(np.sum(D * overlap))
             if ne > 0:
-                D *= molecule.number_of_alpha_electrons() / ne
+                D *= nocc / ne
+            err = np.linalg.norm(D @ overlap @ D - D)
+            for _ in range(purify):
+                DS = D @ overlap
+                DSD = DS @ D
+                Dn = 3.0 * DSD - 2.0 * DS @ DSD
+                en = np.linalg.norm(Dn @ overlap @ Dn - Dn)
+                if not en < err:
+                    break
+                D, err = 0.5 * (Dn + Dn.T), en
+            ne = float(np.sum(D * overlap))
+            if ne > 0:
+                D *= nocc / ne
         return D
 
 
