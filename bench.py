@@ -315,7 +315,8 @@ def workload_builds(nmol, box, seed, basis, ithr, steps, warmup, taus, autotune_
            'screener_setup_s': round(t_screen, 3), 'autotune_s': round(t_tune, 2),
            'roofline': rf,
            'class_ms': {'%d%d%d%d' % c: round(v[0] / v[1], 4) for c, v in sorted(times.items())},
-           # reference-scheme TFLOP/s of each class kernel in the timed region (shared chip)
+           # reference-scheme FLOPs of each class per build, and its TFLOP/s in the timed region
+           'class_flops': {'%d%d%d%d' % c: float(f) for c, f in sorted(class_flops.items())},
            'class_tflops': {'%d%d%d%d' % c: round(class_flops[c] / (v[0] / v[1] * 1e-3) / 1e12, 3)
                             for c, v in sorted(times.items()) if c in class_flops},
            'checksum': float(np.sum(g * dm))}
@@ -556,7 +557,8 @@ def main():
         },
         'roofline': head['roofline'],
     }
-    for k in ('roofline_single_stream', 'class_ms', 'checksum', 'autotune', 'mixed_sweep'):
+    for k in ('roofline_single_stream', 'class_ms', 'class_tflops', 'class_ms_one_stream', 'class_tflops_one_stream',
+              'class_flops', 'checksum', 'autotune', 'mixed_sweep'):
         if k in head:
             result[k] = head[k]
     if not args.only_headline:

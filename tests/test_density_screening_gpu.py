@@ -100,3 +100,26 @@ def test_scf_dimer_with_density_screening(incremental):
     print(f'incremental {incremental}: E {res["scf_energy"]:.10f}, {res["num_iter"]} iterations', flush=True)
     assert res['is_converged']
     assert abs(res['scf_energy'] - (-151.9220860932)) < 1e-8
+
+
+def test_fast_scf_cluster_converges_to_fp64_energy():
+    """A 12-water def2-SVP SCF in the fast mode (mixed precision, incremental difference-density
+    builds, density-weighted screening that follows the gradient down) converges to the FP64 SCF's
+    energy within 1e-8 Eh, in not many more iterations."""
+    from veloxchem_amd.scfdriver import ScfRestrictedDriver
+    mol = water_cluster(12, 10.0 * (12 / 32.0) ** (1 / 3), 91)
+    bas = MolecularBasis.read(mol, 'def2-svp')
+    ref = ScfRestrictedDriver()
+    ref.conv_thresh = 1e-8
+    r0 = ref.compute(mol, bas)
+    fast = ScfRestrictedDriver()
+    fast.conv_thresh = 1e-8
+    fast.precision, fast.mixed_threshold = 'mixed', 1e-6
+    fast.incremental_fock = True
+    fast.density_screening = 1e-10
+    r1 = fast.compute(mol, bas)
+    print(f'FP64 {r0["num_iter"]} iterations, fast {r1["num_iter"]}; dE {r1["scf_energy"] - r0["scf_energy"]:.2e}',
+          flush=True)
+    assert r0['is_converged'] and r1['is_converged']
+    assert abs(r1['scf_energy'] - r0['scf_energy']) < 1e-8
+    assert r1['num_iter'] <= r0['num_iter'] + 6

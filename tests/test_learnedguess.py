@@ -131,7 +131,7 @@ def test_trained_guess_cuts_scf_iterations():
 
 def test_baseline_means_and_fallback():
     """The residual baseline of the trained guess: the mean target per (element pair, on-site,
-    nearest radial centre) class, the residual scale per output element, and, for a class unseen in
+    nearest radial centre) class, the residuals' overall RMS as their scale, and, for a class unseen in
     training, the nearest radial centre of its (element pair, on-site) group (host side only)."""
     from veloxchem_amd.learnedguess import ELEMENTS, N_OUT, NRADIAL, LearnedGuess, sample_keys
     lg = LearnedGuess.__new__(LearnedGuess)  # the baseline needs no device network
@@ -147,7 +147,7 @@ def test_baseline_means_and_fallback():
     for k in np.unique(keys):
         assert np.allclose(lg.baseline(X[keys == k])[0], T[keys == k].mean(axis=0), atol=1e-5)
     resid = T - lg.baseline(X)
-    assert np.allclose(lg.scale, np.sqrt(np.mean(resid.astype(np.float64) ** 2, axis=0)) + 1e-7, rtol=1e-5)
+    assert np.allclose(lg.scale, np.sqrt(np.mean(resid.astype(np.float64) ** 2)), rtol=1e-5)
     # unseen radial centre 20: the group's nearest seen centre (the largest one drawn, <= 4)
     Y = X[:1].copy()
     Y[0, ne + 1:ne + 1 + NRADIAL] = 0.0

@@ -248,10 +248,12 @@ def sample_keys(features):
 class LearnedGuess:
     """The trained network and its use as an SCF starting density.
 
-    The network learns a normalised residual: the target block minus the mean block of its
-    baseline class (element pair, on-site flag, distance to the nearest 0.19 Angstrom), divided per
-    output element by the residuals' root mean square over the training set.  BF16 carries about
-    three significant digits, so its rounding then falls on the residual, not on the whole block."""
+    The network learns a scaled residual: the target block minus the mean block of its baseline
+    class (element pair, on-site flag, distance to the nearest 0.19 Angstrom), divided by the
+    residuals' root mean square over the training set (one scale for all elements: a scale per
+    element would blow up the elements that are zero by symmetry up to rounding).  BF16 carries
+    about three significant digits, so its rounding then falls on the residual, not on the whole
+    block."""
 
     def __init__(self, n_hidden=256, max_batch=8192, seed=0, device=0):
         from .densitynet import DensityNet
@@ -270,7 +272,8 @@ class LearnedGuess:
         mean /= np.bincount(inv)[:, None]
         resid = targets - mean[inv]
         self.base_mean = mean.astype(np.float32)
-        self.scale = (np.sqrt(np.mean(resid.astype(np.float64) ** 2, axis=0)) + 1e-7).astype(np.float32)
+        rms = float(np.sqrt(np.mean(resid.astype(np.float64) ** 2)))
+        self.scale = np.full(N_OUT, max(rms, 1e-7), dtype=np.float32)
 
     def baseline(self, features):
         """Mean blocks of the samples' classes; a class unseen in training takes the nearest radial

@@ -152,9 +152,12 @@ class ScfRestrictedDriver:
         self.full_fock_every = 8
         # density-weighted screening of the builds (a fast mode, no reference counterpart): quartets
         # whose Schwarz bound x the largest |D| (|dD| in incremental builds) of the blocks they read
-        # is below this are skipped; 0 = off.  The converged density is confirmed by one unscreened
-        # full build, which gives the reported energy.
+        # is below min(density_screening, 1e-6 x the last gradient norm) are skipped; 0 = off.  The
+        # skipped terms leave an error in G, so the threshold follows the gradient down and the SCF
+        # keeps converging; the converged density is confirmed by one unscreened full build, which
+        # gives the reported energy.
         self.density_screening = 0.0
+        self.density_screening_gradient_factor = 1.0e-6
         self.screener_factory = None
         # starting density: None for the reference's guesses (SAD, L2-DIIS valence stage), or an
         # object with supports(molecule, basis) and density(molecule, basis, overlap) (the learned
@@ -311,6 +314,11 @@ class ScfRestrictedDriver:
             thresh = self.eri_thresh if e_grad is None else self._get_dyn_threshold(e_grad)
             thresh_int = int(-math.log10(thresh))
             t = time.perf_counter()
+            if self.density_screening > 0.0 and isinstance(self.fock_2e, _DeviceFock) and not confirm:
+                tau_d = self.density_screening
+                if e_grad is not None:
+                    tau_d = min(tau_d, self.density_screening_gradient_factor * e_grad)
+                _native.set_density_screening(tau_d, self.fock_2e.device)
             incremental = (self.incremental_fock and g_prev is not None and not confirm and
                            thresh_int == full_thresh and since_full < self.full_fock_every)
             build_thresh = thresh_int
