@@ -70,12 +70,50 @@ struct EriClass;
 // T >= BOYS_TMAX: F_0 = sqrt(pi/T)/2 (erf(sqrt(T)) = 1 to double precision
 // there) and upward recursion F_{m+1} = ((2m+1) F_m - e^-T) / (2T), stable
 // for T > L.
+// FP32 quartets keep Boys arguments near zero in FP64: below BOYS_T_FP64 the FP32 kernels take F_L
+// from its series at T = 0, F_L(T) = sum_k (-T)^k / (k! (2L + 2k + 1)), and e^-T from its own series
+// (18 terms each: 1^18 / 18! < 2e-16), then the downward recursion in FP64.
+constexpr double BOYS_T_FP64 = 1.0;
+constexpr int BOYS_NEAR0_TERMS = 18;
+
+template <int L>
+__device__ __forceinline__ void boys_near_zero_fp64(const double t, float* __restrict__ F)
+{
+    double f = 0.0, e = 0.0, inv_fact = 1.0;
+    // Horner from the highest term: coefficient k of F_L is 1 / (k! (2L + 2k + 1)) with sign (-1)^k
+    double cf[BOYS_NEAR0_TERMS + 1], ce[BOYS_NEAR0_TERMS + 1];
+#pragma unroll
+    for (int k = 0; k <= BOYS_NEAR0_TERMS; k++) {
+        if (k > 0) inv_fact /= k;
+        cf[k] = inv_fact / (2 * L + 2 * k + 1);
+        ce[k] = inv_fact;
+    }
+#pragma unroll
+    for (int k = BOYS_NEAR0_TERMS; k >= 0; k--) {
+        f = fma(f, -t, cf[k]);
+        e = fma(e, -t, ce[k]);
+    }
+    double fm = f;
+    F[L] = static_cast<float>(fm);
+#pragma unroll
+    for (int m = L - 1; m >= 0; m--) {
+        fm = (2.0 * t * fm + e) * (1.0 / (2 * m + 1));
+        F[m] = static_cast<float>(fm);
+    }
+}
+
 template <int L, typename R>
 __device__ __forceinline__ void boys_eval(const R T, const R* __restrict__ tab, R* __restrict__ F)
 {
     // Taylor orders: full table order in FP64; 5 terms reach FP32 rounding
     // (0.125^6 / 6! ~ 5e-9)
     constexpr int ORD = (sizeof(R) == 8) ? BOYS_ORDER : 5;
+    if constexpr (sizeof(R) == 4) {
+        if (T < R(BOYS_T_FP64)) {
+            boys_near_zero_fp64<L>(static_cast<double>(T), F);
+            return;
+        }
+    }
     if (T < R(BOYS_TMAX)) {
         const int k = static_cast<int>(T * R(BOYS_INV_DT) + R(0.5));
         const R d = T - R(BOYS_DT) * R(k);
