@@ -21,6 +21,7 @@ the reference's recursions (tests/golden/ref_scheme_flops.json).
 """
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -208,6 +209,88 @@ def measure_builds(scr, dm, ithr, steps, warmup, taus, label=b'2jk', precision='
         d_dm.free()
         d_g.free()
     return ms, times, g, sweep
+
+
+def multi_density(scr, dm, ithr, counts=(2, 4, 8), steps=3, seed=7):
+    """One pass over the integrals for k densities (vlx_fock_compute_multi_device) against k single
+    builds of the same densities, all FP64 on device-resident D: times, the cost of the pass in
+    units of one build, and max|dG| of every density against its own single build.  The densities
+    are the workload's D plus seeded symmetric perturbations of 1 % of its largest element."""
+    from veloxchem_amd import _native
+    lib, ctx = _native.lib(), _native.context(0)
+    n = dm.shape[0]
+    kmax = max(counts)
+    rng = np.random.default_rng(seed)
+    dens = np.empty((kmax, n, n))
+    for i in range(kmax):
+        p = rng.standard_normal((n, n))
+        dens[i] = dm + 0.01 * np.max(np.abs(dm)) * 0.5 * (p + p.T) / np.sqrt(n)
+    d_d = _native.DeviceArray((kmax, n, n))
+    d_g = _native.DeviceArray((kmax, n, n))
+    d_one = _native.DeviceArray((n, n))
+    d_d.upload(dens)
+    nn8 = n * n * 8
+
+    def single(i, out):
+        _native.check(lib.vlx_fock_compute_device(ctx, scr.handle, ctypes.c_void_p(d_d.ptr.value + i * nn8), b'2jk',
+                                                  1.0, 0.0, ithr, out))
+
+    def timed(fn):
+        fn()
+        _native.check(lib.vlx_context_synchronize(ctx))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        _native.check(lib.vlx_context_synchronize(ctx))
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    rows, class_rows = [], {}
+    try:
+        ms_one = timed(lambda: single(0, d_one.ptr))
+        singles = []
+        for i in range(kmax):
+            single(i, d_one.ptr)
+            singles.append(d_one.download())
+        for k in counts:
+            labels = (ctypes.c_char_p * k)(*([b'2jk'] * k))
+
+            def multi():
+                _native.check(lib.vlx_fock_compute_multi_device(ctx, scr.handle, k, d_d.ptr, None, labels, 1.0, 0.0,
+                                                                ithr, d_g.ptr))
+            ms_k = timed(multi)
+            ms_sep = timed(lambda: [single(i, d_one.ptr) for i in range(k)])
+            if k == max(counts):
+                # per class, one stream: the pass's kernel time against k single-density launches
+                _native.check(lib.vlx_context_set_streams(ctx, 1))
+                try:
+                    per = {}
+                    for name, fn in (('one', lambda: single(0, d_one.ptr)), ('multi', multi)):
+                        fn()
+                        _native.check(lib.vlx_context_synchronize(ctx))
+                        _native.check(lib.vlx_context_set_timing(ctx, 1))
+                        _native.check(lib.vlx_context_reset_times(ctx))
+                        fn()
+                        _native.check(lib.vlx_context_synchronize(ctx))
+                        _native.check(lib.vlx_context_set_timing(ctx, 0))
+                        per[name] = {'%d%d%d%d' % c: v[0] for c, v in _native.class_times(0).items()}
+                finally:
+                    _native.check(lib.vlx_context_set_streams(ctx, 4))
+                class_rows = {c: {'ms_one_density': round(per['one'][c], 3), 'ms_pass': round(per['multi'].get(c, 0.0), 3),
+                                  'pass_vs_separate': round(per['multi'].get(c, 0.0) / max(k * per['one'][c], 1e-9), 3)}
+                              for c in per['one']}
+            g = d_g.download()
+            err = max(float(np.max(np.abs(g[i] - singles[i]))) for i in range(k))
+            rows.append({'densities': k, 'ms_one_pass': ms_k, 'ms_separate_builds': ms_sep,
+                         'cost_in_single_builds': ms_k / ms_one, 'speedup_vs_separate': ms_sep / ms_k,
+                         'max_abs_dG_vs_single_builds': err})
+            log(f'multi-density: {k} densities in {ms_k:.1f} ms ({ms_k / ms_one:.2f} builds), separately '
+                f'{ms_sep:.1f} ms, max|dG| {err:.1e}')
+    finally:
+        d_d.free()
+        d_g.free()
+        d_one.free()
+    return {'ms_single_build': ms_one, 'label': '2jk', 'steps': steps, 'rows': rows,
+            f'classes_one_stream_{max(counts)}_densities': class_rows}
 
 
 def cpu_baseline(mol, bas, dm, ithr, target_s, basis_name, g_gpu, repeats=1):
@@ -505,6 +588,7 @@ def main():
     ap.add_argument('--batch-clusters', type=int, default=256)
     ap.add_argument('--batch-reps', type=int, default=3)
     ap.add_argument('--no-guess', action='store_true', help='skip the learned-guess training and evaluation')
+    ap.add_argument('--no-multi', action='store_true', help='skip the multi-density measurements')
     ap.add_argument('--density-tau', type=float, default=1e-10,
                     help='density-weighted screening threshold of the fast SCF mode')
     ap.add_argument('--no-tzvp-scf', action='store_true',
@@ -563,8 +647,9 @@ def main():
             result[k] = head[k]
     if not args.only_headline:
         if not args.no_svp:
-            svp, (smol, sbas, _, _, _) = workload_builds(32, 10.0, 32, 'def2-svp', args.ithreshold, args.svp_steps, 2,
-                                                         taus, 0 if args.no_autotune else 2, 'water32_svp')
+            svp, (smol, sbas, scr32, dm32, _) = workload_builds(32, 10.0, 32, 'def2-svp', args.ithreshold,
+                                                                args.svp_steps, 2, taus,
+                                                                0 if args.no_autotune else 2, 'water32_svp')
             result['svp32'] = svp
             if not args.no_scf:
                 # the dense-algebra libraries load their kernels for each matrix size on first use: one
@@ -582,6 +667,12 @@ def main():
                         mixed['dE_vs_fp64'] = mixed['energy'] - result['scf']['energy']
                         result[key] = mixed
                     result['scf_fast_vs_fp64'] = fast_vs_fp64_scf(smol, sbas, args.scf_conv, tau, args.density_tau)
+            if not args.no_multi:
+                log('multi-density pass, 32-water def2-SVP')
+                result['svp32']['multi_density'] = multi_density(scr32, dm32, args.ithreshold)
+        if not args.no_multi:
+            log('multi-density pass, 64-water def2-TZVP')
+            result['multi_density'] = multi_density(scr, dm, args.ithreshold, counts=(2, 4), steps=2)
         if not args.no_tzvp_scf and taus:
             log('64-water def2-TZVP SCFs: FP64 and fast')
             result['scf_tzvp_fast_vs_fp64'] = fast_vs_fp64_scf(mol, bas, args.scf_conv, max(taus), args.density_tau)

@@ -70,6 +70,20 @@ int vlx_set_precision(vlx_context* ctx, int mode, double threshold);
  * bound follows |Delta D| block by block.
  */
 int vlx_set_density_screening(vlx_context* ctx, double threshold);
+/*
+ * Deterministic reduction mode (no reference counterpart: the reference
+ * merges task-local Fock blocks under an OpenMP critical section,
+ * src/t4c_func/T4CMatrixDistributor.cpp:142, whose order varies from run to
+ * run; this mode is for diffing against the atomic path).  enable != 0: every
+ * later build of the context (single, general and multi-density) runs the
+ * direct-digestion kernels and adds each F' contribution as a 128-bit
+ * fixed-point integer (resolution 2^-96) with integer atomics, whose sums do
+ * not depend on their order: the same bits on every run and for any number of
+ * streams.  FP64 with the Schwarz selection only (the mixed-precision and
+ * density-weighted modes do not apply); slower than the default kernels.
+ * Batched builds, the no-allocation build and the autotuner refuse it.
+ */
+int vlx_set_deterministic(vlx_context* ctx, int enable);
 /* Bin-sum threshold the last build used (99 = no FP32 work). */
 int vlx_context_last_fp32_bin_sum(vlx_context* ctx, int* k32);
 /* Primitive slots of one build run in FP64 and in FP32 at bin-sum threshold k32. */
@@ -182,6 +196,28 @@ int vlx_fock_compute_general(vlx_context* ctx, vlx_screener* scr, const double* 
 int vlx_fock_compute_general_device(vlx_context* ctx, vlx_screener* scr, const double* d_density, const char* label,
                                     double exchange_factor, double omega, int ithreshold, double* d_fock_out);
 
+/*
+ * CFockDriver::compute(screener, CMatrices densities, labels, exchange_factor,
+ * omega, ithreshold) (src/t4c_func/FockDriver.cpp:173-229 with
+ * CT4CMatricesDistributor, src/t4c_func/T4CMatricesDistributor.cpp): one pass
+ * over the integrals digests all nmat densities, each with its own label.
+ * densities: nmat x nao x nao, back to back; focks_out the same.  general:
+ * nmat flags (NULL: all symmetric); a general density costs a second digested
+ * density (its antisymmetric part, exchange labels only), not a second pass.
+ * The integrals are erf(omega r)/r only when every label is range-separated
+ * (CT4CMatricesDistributor::need_omega, :55-66), else 1/r for all, *_rs labels
+ * then digesting like j, k, kx.  FP64 with the Schwarz selection; the
+ * context's mixed-precision and density-weighted modes do not apply.  Host
+ * buffers; symmetric densities are checked like vlx_fock_compute's.
+ */
+int vlx_fock_compute_multi(vlx_context* ctx, vlx_screener* scr, int nmat, const double* densities, const int* general,
+                           const char* const* labels, double exchange_factor, double omega, int ithreshold,
+                           double* focks_out);
+/* The same on device buffers (context stream, no synchronisation, no symmetry check). */
+int vlx_fock_compute_multi_device(vlx_context* ctx, vlx_screener* scr, int nmat, const double* d_densities,
+                                  const int* general, const char* const* labels, double exchange_factor, double omega,
+                                  int ithreshold, double* d_focks_out);
+
 /* Device memory helpers (HBM of the context's device) for callers that keep
  * D and G resident between builds. */
 int vlx_device_malloc(vlx_context* ctx, unsigned long long bytes, void** out);
@@ -232,7 +268,9 @@ int vlx_fock_schedule_stats(vlx_screener* scr, int ithreshold, int max_records, 
  * (0 = untiled, 1 = LDS-tiled with 4-wave blocks, 2 = LDS-tiled with the
  * class's wide block, 3 = matrix-core McMurchie-Davidson, for classes with
  * la+lb+lc+ld >= 4, 4 and 5 = variants 1 and 2 with one ket primitive per lane
- * instead of runs of up to three), `repeats` builds each on a single stream, and keeps
+ * instead of runs of up to three, 6 = ket-stationary work items with lane-private
+ * LDS accumulators for the bra-stationary classes), `repeats` builds each on a
+ * single stream, and keeps
  * the fastest variant per class for later builds with this screener (and as
  * the default of screeners created later on this context).
  * Optional outputs: *nchosen records of classes (4 ints each) and variants

@@ -69,6 +69,12 @@ def lib():
                                                     ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_double,
                                                     ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         L.vlx_fock_compute_general_device.argtypes = L.vlx_fock_compute_device.argtypes
+        L.vlx_fock_compute_multi.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_double_p, _c_int_p,
+                                             ctypes.POINTER(ctypes.c_char_p), ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int, _c_double_p]
+        L.vlx_fock_compute_multi_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                    _c_int_p, ctypes.POINTER(ctypes.c_char_p), ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
         L.vlx_fock_work.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_ll_p, _c_ll_p,
                                     _c_double_p]
         L.vlx_context_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -89,6 +95,7 @@ def lib():
                                             ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.vlx_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         L.vlx_set_density_screening.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.vlx_set_deterministic.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.vlx_context_last_fp32_bin_sum.argtypes = [ctypes.c_void_p, _c_int_p]
         L.vlx_fock_precision_split.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_ll_p, _c_ll_p]
         L.vlx_fock_precision_classes.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p,
@@ -142,6 +149,12 @@ def set_density_screening(threshold=0.0, device=0):
     """Density-weighted screening of later builds: quartets whose Schwarz bound times the largest |D|
     of the blocks they read is below threshold are skipped (0: off, the reference's screening)."""
     check(lib().vlx_set_density_screening(context(device), float(threshold)))
+
+
+def set_deterministic(enable=True, device=0):
+    """Deterministic reduction mode of later builds on the device: fixed-point integer accumulation
+    of every F' contribution, bitwise the same result on every run (vlx_set_deterministic)."""
+    check(lib().vlx_set_deterministic(context(device), 1 if enable else 0))
 
 
 def last_fp32_bin_sum(device=0):

@@ -446,6 +446,152 @@ __global__ void __launch_bounds__(VLX_BLOCK)
     }
 }
 
+// Deterministic accumulation (FockScalars::det): an F' element is a 128-bit fixed-point integer of
+// resolution 2^-96, two 64-bit words (low word first), added with integer atomics.  Integer sums do
+// not depend on the order of the additions, so a build gives the same bits on every run and with
+// any number of streams.  x = hi 2^-32 + lo 2^-96 with hi = floor(x 2^32) and lo = (x 2^32 - hi) 2^64:
+// both scalings are by powers of two and x 2^32 - floor(x 2^32) is exact, so only the bits of x
+// below 2^-96 are dropped (|x| < 2^31).  The carry out of the low word is counted by the atomic
+// whose addition wraps it, once per wrap, so it too is independent of the order.
+__device__ __forceinline__ void fx_add(unsigned long long* __restrict__ acc, const size_t idx, const double x)
+{
+    if (x == 0.0) return;
+    const double y = x * 4294967296.0;
+    const double h = floor(y);
+    const unsigned long long lo = static_cast<unsigned long long>((y - h) * 18446744073709551616.0);
+    unsigned long long hi = static_cast<unsigned long long>(static_cast<long long>(h));
+    unsigned long long* p = acc + 2 * idx;
+    if (lo != 0ull) {
+        const unsigned long long old = atomicAdd(p, lo);
+        if (old + lo < old) hi += 1ull;
+    }
+    if (hi != 0ull) atomicAdd(p + 1, hi);
+}
+
+// F'[idx] += v: an FP64 atomic, or the fixed-point add above into the accumulator array that F
+// then points to (two words per element)
+template <bool DET>
+__device__ __forceinline__ void acc_add(double* __restrict__ F, const size_t idx, const double v)
+{
+    if constexpr (DET) fx_add(reinterpret_cast<unsigned long long*>(F), idx, v);
+    else unsafeAtomicAdd(F + idx, v);
+}
+
+// Lane-private LDS accumulators of fock_kernel_ksl: slot e of the calling thread at p[e * stride]
+// (stride = threads per block, so the lanes of a wave hit consecutive banks).
+struct LaneAcc {
+    double* jcd;  // J'_cd of the lane's ket, nc x nd (factor fj applied)
+    double* kac;  // K'_ac for the current shell a, na x nc (factor fk applied)
+    double* kad;  // K'_ad, na x nd
+    int stride;
+};
+
+// Direct digestion of one quartet block per lane (bra pair wave-uniform, ket pair per lane; every
+// lane of the wavefront calls it): F'_ab is reduced over the wavefront, F'_cd, F'_ac, F'_ad, F'_bc
+// and F'_bd are per-lane atomics.  Masked lanes hold exact zeros in I.
+// LACC: J'_cd, K'_ac and K'_ad go to the lane's LDS accumulators instead (no atomics: only the lane
+// touches them).
+template <int la, int lb, int lc, int ld, bool DET = false, bool LACC = false>
+__device__ __forceinline__ void digest_direct(const double (&I)[(2 * la + 1) * (2 * lb + 1) * (2 * lc + 1) * (2 * ld + 1)],
+                                              const PairInfo& bp, const PairInfo& kp, const bool active,
+                                              const int lane, const double* __restrict__ D,
+                                              double* __restrict__ F, const int n, const double fj,
+                                              const double fk, const LaneAcc* lacc = nullptr)
+{
+    constexpr int na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
+    int p[na], q[nb], r[nc], t[nd];
+#pragma unroll
+    for (int i = 0; i < na; i++) p[i] = bp.ao_a + i * bp.str_a;
+#pragma unroll
+    for (int i = 0; i < nb; i++) q[i] = bp.ao_b + i * bp.str_b;
+#pragma unroll
+    for (int i = 0; i < nc; i++) r[i] = kp.ao_a + i * kp.str_a;
+#pragma unroll
+    for (int i = 0; i < nd; i++) t[i] = kp.ao_b + i * kp.str_b;
+    if (fj != 0.0) {
+#pragma unroll
+        for (int i = 0; i < na; i++)
+#pragma unroll
+            for (int jb = 0; jb < nb; jb++) {
+                double v = 0.0;
+#pragma unroll
+                for (int k = 0; k < nc; k++)
+#pragma unroll
+                    for (int l = 0; l < nd; l++)
+                        v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)r[k] * n + t[l]], v);
+                v = wave_sum(v);
+                if (lane == 0 && v != 0.0) acc_add<DET>(F, (size_t)p[i] * n + q[jb], fj * v);
+            }
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < nc; k++)
+#pragma unroll
+                for (int l = 0; l < nd; l++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int i = 0; i < na; i++)
+#pragma unroll
+                        for (int jb = 0; jb < nb; jb++)
+                            v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)p[i] * n + q[jb]], v);
+                    if constexpr (LACC) lacc->jcd[(k * nd + l) * lacc->stride] += fj * v;
+                    else acc_add<DET>(F, (size_t)r[k] * n + t[l], fj * v);
+                }
+        }
+    }
+    if (fk != 0.0 && active) {
+#pragma unroll
+        for (int i = 0; i < na; i++)
+#pragma unroll
+            for (int k = 0; k < nc; k++) {
+                double v = 0.0;
+#pragma unroll
+                for (int jb = 0; jb < nb; jb++)
+#pragma unroll
+                    for (int l = 0; l < nd; l++)
+                        v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)q[jb] * n + t[l]], v);
+                if constexpr (LACC) lacc->kac[(i * nc + k) * lacc->stride] += fk * v;
+                else acc_add<DET>(F, (size_t)p[i] * n + r[k], fk * v);
+            }
+#pragma unroll
+        for (int i = 0; i < na; i++)
+#pragma unroll
+            for (int l = 0; l < nd; l++) {
+                double v = 0.0;
+#pragma unroll
+                for (int jb = 0; jb < nb; jb++)
+#pragma unroll
+                    for (int k = 0; k < nc; k++)
+                        v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)q[jb] * n + r[k]], v);
+                if constexpr (LACC) lacc->kad[(i * nd + l) * lacc->stride] += fk * v;
+                else acc_add<DET>(F, (size_t)p[i] * n + t[l], fk * v);
+            }
+#pragma unroll
+        for (int jb = 0; jb < nb; jb++)
+#pragma unroll
+            for (int k = 0; k < nc; k++) {
+                double v = 0.0;
+#pragma unroll
+                for (int i = 0; i < na; i++)
+#pragma unroll
+                    for (int l = 0; l < nd; l++)
+                        v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)p[i] * n + t[l]], v);
+                acc_add<DET>(F, (size_t)q[jb] * n + r[k], fk * v);
+            }
+#pragma unroll
+        for (int jb = 0; jb < nb; jb++)
+#pragma unroll
+            for (int l = 0; l < nd; l++) {
+                double v = 0.0;
+#pragma unroll
+                for (int i = 0; i < na; i++)
+#pragma unroll
+                    for (int k = 0; k < nc; k++)
+                        v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)p[i] * n + r[k]], v);
+                acc_add<DET>(F, (size_t)q[jb] * n + t[l], fk * v);
+            }
+    }
+}
+
 // Bra-stationary digestion, for classes whose integrals need most of the
 // register file.  A wavefront takes one bra pair (wave-uniform) and 64 kets
 // (one per lane) from the prefix of the ket list its bins allow; work item
@@ -496,94 +642,191 @@ __global__ void __launch_bounds__(VLX_BLOCK)
                                         a.omega2, I);
 #pragma unroll
         for (int x = 0; x < nsph; x++) I[x] = active ? I[x] * s : 0.0;
-        int p[na], q[nb], r[nc], t[nd];
+        digest_direct<la, lb, lc, ld>(I, bp, kp, active, lane, D, F, n, a.fj, a.fk);
+    }
+}
+
+// Multi-density digestion: the integrals of each quartet, computed once, are digested into
+// a.ndens densities (density d at D0 + d * dstride, its F' at F0 + d * dstride, factors
+// dfac[2 d] (Coulomb) and dfac[2 d + 1] (exchange)), every contribution by direct digestion:
+// per-density register accumulators would not fit beside the integrals.  The work items are
+// those of the class's untiled schedule: ket-stationary (a.bs_layout = 0, bras [i0, i1) against kets
+// [j0, jend)) or bra-stationary (a.bs_layout = 1, bra i0 against kets [j0, i1)).  a.det: F0 is the
+// fixed-point accumulator array of fx_add (two words per element, same offsets doubled).
+template <int la, int lb, int lc, int ld, bool RS>
+__global__ void __launch_bounds__(VLX_BLOCK)
+    fock_kernel_multi(const PairInfo* __restrict__ bra, const PrimPair* __restrict__ bra_prims,
+                      const PairInfo* __restrict__ ket, const PrimPair* __restrict__ ket_prims,
+                      const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
+                      const double* __restrict__ D0, double* __restrict__ F0, const MolDesc* __restrict__ mols,
+                      const double* __restrict__ dfac, const FockScalars a)
+{
+    constexpr int nsph = (2 * la + 1) * (2 * lb + 1) * (2 * lc + 1) * (2 * ld + 1);
+    __shared__ double tab[BOYS_TAB_SIZE];
+    stage_boys(tab, boys_tab);
+
+    const int lane = threadIdx.x & (VLX_WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / VLX_WAVE);
+    const long long gw = (long long)blockIdx.x * VLX_WAVES_PER_BLOCK + wave;
+    const long long nw = (long long)gridDim.x * VLX_WAVES_PER_BLOCK;
+    const long long w0 = a.nitems * gw / nw, w1 = a.nitems * (gw + 1) / nw;
+    const int nds = a.ndens > 1 ? a.ndens : 1;
+    const bool BSL = a.bs_layout != 0;
+
+    for (long long it = w0; it < w1; it++) {
+        const WorkItem wi = items[it];
+        const MolDesc md = mols[wi.mol];
+        const int n = md.nao;
+        const int j = wi.j0 + lane;
+        const bool lane_ok = j < (BSL ? wi.i1 : wi.jend);
+        const PairInfo kp = ket[lane_ok ? j : wi.j0];
+        const int ib1 = BSL ? wi.i0 + 1 : wi.i1;
+        for (int ib = wi.i0; ib < ib1; ib++) {
+            const PairInfo bp = bra[ib];
+            bool active = lane_ok;
+            // bra-stationary items hold only kets within the bin and rank limits
+            if (!BSL && a.ithreshold >= 0) active = active && (bp.bin + kp.bin <= a.ithreshold);
+            if (!BSL && a.same_class) active = active && (bp.rank >= j);
+            double s = 1.0;
+            if (bp.same) s *= 0.5;
+            if (kp.same) s *= 0.5;
+            if (a.same_class && bp.rank == j) s *= 0.5;
+            double I[nsph];
+            eri_quartet<la, lb, lc, ld, RS>(bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
+                                            ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz,
+                                            tab, a.omega2, I);
 #pragma unroll
-        for (int i = 0; i < na; i++) p[i] = bp.ao_a + i * bp.str_a;
-#pragma unroll
-        for (int i = 0; i < nb; i++) q[i] = bp.ao_b + i * bp.str_b;
-#pragma unroll
-        for (int i = 0; i < nc; i++) r[i] = kp.ao_a + i * kp.str_a;
-#pragma unroll
-        for (int i = 0; i < nd; i++) t[i] = kp.ao_b + i * kp.str_b;
-        if (a.fj != 0.0) {
-#pragma unroll
-            for (int i = 0; i < na; i++)
-#pragma unroll
-                for (int jb = 0; jb < nb; jb++) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int k = 0; k < nc; k++)
-#pragma unroll
-                        for (int l = 0; l < nd; l++)
-                            v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)r[k] * n + t[l]], v);
-                    v = wave_sum(v);
-                    if (lane == 0 && v != 0.0) unsafeAtomicAdd(&F[(size_t)p[i] * n + q[jb]], a.fj * v);
-                }
-            if (active) {
-#pragma unroll
-                for (int k = 0; k < nc; k++)
-#pragma unroll
-                    for (int l = 0; l < nd; l++) {
-                        double v = 0.0;
-#pragma unroll
-                        for (int i = 0; i < na; i++)
-#pragma unroll
-                            for (int jb = 0; jb < nb; jb++)
-                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)p[i] * n + q[jb]], v);
-                        unsafeAtomicAdd(&F[(size_t)r[k] * n + t[l]], a.fj * v);
-                    }
+            for (int x = 0; x < nsph; x++) I[x] = active ? I[x] * s : 0.0;
+            for (int dd = 0; dd < nds; dd++) {
+                const long long off = md.base + dd * a.dstride;
+                if (a.det)  // F0 is the fixed-point accumulator array, two words per element
+                    digest_direct<la, lb, lc, ld, true>(I, bp, kp, active, lane, D0 + off, F0 + 2 * off, n,
+                                                        dfac[2 * dd], dfac[2 * dd + 1]);
+                else
+                    digest_direct<la, lb, lc, ld>(I, bp, kp, active, lane, D0 + off, F0 + off, n, dfac[2 * dd],
+                                                  dfac[2 * dd + 1]);
             }
         }
-        if (a.fk != 0.0 && active) {
+    }
+}
+
+// Ket-stationary digestion with lane-private LDS accumulators, for classes whose integrals leave no
+// registers for the accumulators of fock_kernel.  Work items as in fock_kernel (64 kets, one per
+// lane, against a range of bras sorted by shell A).  J'_cd, K'_ac and K'_ad of a lane's ket
+// accumulate in LDS slots only that lane touches (no atomics); K'_ac and K'_ad go to F' when shell A
+// changes, J'_cd at the end of the item.  F'_ab is reduced over the wavefront, F'_bc and F'_bd are
+// per-lane atomics, as in the bra-stationary kernel, which sends all six per quartet.  Dynamic LDS:
+// (na nc + na nd + nc nd) doubles per thread.
+template <int la, int lb, int lc, int ld, bool RS, int NW>
+__global__ void __launch_bounds__(NW * VLX_WAVE)
+    fock_kernel_ksl(const PairInfo* __restrict__ bra, const PrimPair* __restrict__ bra_prims,
+                    const PairInfo* __restrict__ ket, const PrimPair* __restrict__ ket_prims,
+                    const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
+                    const double* __restrict__ D0, double* __restrict__ F0, const MolDesc* __restrict__ mols,
+                    const int* __restrict__ glim, const FockScalars a)
+{
+    constexpr int na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
+    constexpr int nsph = na * nb * nc * nd;
+    constexpr int NT = NW * VLX_WAVE;
+    constexpr int NACC = na * nc + na * nd + nc * nd;
+    __shared__ double tab[BOYS_TAB_SIZE];
+    extern __shared__ double lslots[];
+    const LaneAcc acc{lslots + (na * nc + na * nd) * NT + threadIdx.x, lslots + threadIdx.x,
+                      lslots + na * nc * NT + threadIdx.x, NT};
+    for (int e = 0; e < NACC; e++) lslots[e * NT + threadIdx.x] = 0.0;
+    stage_boys(tab, boys_tab);
+
+    const int lane = threadIdx.x & (VLX_WAVE - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / VLX_WAVE);
+    const long long gw = (long long)blockIdx.x * NW + wave;
+    const long long nw = (long long)gridDim.x * NW;
+    const long long w0 = a.nitems * gw / nw, w1 = a.nitems * (gw + 1) / nw;
+
+    for (long long it = w0; it < w1; it++) {
+        const WorkItem wi = items[it];
+        const MolDesc md = mols[wi.mol];
+        const double* __restrict__ D = D0 + md.base;
+        double* __restrict__ F = F0 + md.base;
+        const int n = md.nao;
+        const int j = wi.j0 + lane;
+        const bool lane_ok = j < wi.jend;
+        const PairInfo kp = ket[lane_ok ? j : wi.j0];
+        int r[nc], t[nd], p[na];
 #pragma unroll
-            for (int i = 0; i < na; i++)
+        for (int k = 0; k < nc; k++) r[k] = kp.ao_a + k * kp.str_a;
 #pragma unroll
-                for (int k = 0; k < nc; k++) {
-                    double v = 0.0;
+        for (int l = 0; l < nd; l++) t[l] = kp.ao_b + l * kp.str_b;
+        int cur_a = -1;
+        bool pending = false;  // this lane's K slots hold contributions for shell cur_a
+        for (int ib = wi.i0; ib < wi.i1; ib++) {
+            const PairInfo bp = bra[ib];
+            if (bp.shell_a != cur_a) {
+                if (pending) {
 #pragma unroll
-                    for (int jb = 0; jb < nb; jb++)
+                    for (int i = 0; i < na; i++) {
 #pragma unroll
-                        for (int l = 0; l < nd; l++)
-                            v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)q[jb] * n + t[l]], v);
-                    unsafeAtomicAdd(&F[(size_t)p[i] * n + r[k]], a.fk * v);
+                        for (int k = 0; k < nc; k++) {
+                            double& v = acc.kac[(i * nc + k) * NT];
+                            if (v != 0.0) unsafeAtomicAdd(&F[(size_t)p[i] * n + r[k]], v);
+                            v = 0.0;
+                        }
+#pragma unroll
+                        for (int l = 0; l < nd; l++) {
+                            double& v = acc.kad[(i * nd + l) * NT];
+                            if (v != 0.0) unsafeAtomicAdd(&F[(size_t)p[i] * n + t[l]], v);
+                            v = 0.0;
+                        }
+                    }
                 }
+                pending = false;
+                cur_a = bp.shell_a;
 #pragma unroll
-            for (int i = 0; i < na; i++)
+                for (int i = 0; i < na; i++) p[i] = bp.ao_a + i * bp.str_a;
+            }
+            bool active = lane_ok;
+            if (a.ithreshold >= 0) active = active && (bp.bin + kp.bin <= a.ithreshold);
+            if (glim != nullptr)
+                active = active && (bp.bin + kp.bin <=
+                                     quartet_bin_limit(glim, a.ngroups, bp.grp_a, bp.grp_b, kp.grp_a, kp.grp_b));
+            if (a.same_class) active = active && (bp.rank >= j);
+            double s = 1.0;
+            if (bp.same) s *= 0.5;
+            if (kp.same) s *= 0.5;
+            if (a.same_class && bp.rank == j) s *= 0.5;
+            double I[nsph];
+            eri_quartet<la, lb, lc, ld, RS>(bra_prims + bp.prim_start, bp.nprim, bp.abx, bp.aby, bp.abz,
+                                            ket_prims + kp.prim_start, active ? kp.nprim : 0, kp.abx, kp.aby, kp.abz,
+                                            tab, a.omega2, I);
 #pragma unroll
-                for (int l = 0; l < nd; l++) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int jb = 0; jb < nb; jb++)
-#pragma unroll
-                        for (int k = 0; k < nc; k++)
-                            v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)q[jb] * n + r[k]], v);
-                    unsafeAtomicAdd(&F[(size_t)p[i] * n + t[l]], a.fk * v);
-                }
-#pragma unroll
-            for (int jb = 0; jb < nb; jb++)
-#pragma unroll
-                for (int k = 0; k < nc; k++) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int i = 0; i < na; i++)
-#pragma unroll
-                        for (int l = 0; l < nd; l++)
-                            v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)p[i] * n + t[l]], v);
-                    unsafeAtomicAdd(&F[(size_t)q[jb] * n + r[k]], a.fk * v);
-                }
-#pragma unroll
-            for (int jb = 0; jb < nb; jb++)
-#pragma unroll
-                for (int l = 0; l < nd; l++) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int i = 0; i < na; i++)
-#pragma unroll
-                        for (int k = 0; k < nc; k++)
-                            v = fma(I[((i * nb + jb) * nc + k) * nd + l], D[(size_t)p[i] * n + r[k]], v);
-                    unsafeAtomicAdd(&F[(size_t)q[jb] * n + t[l]], a.fk * v);
-                }
+            for (int x = 0; x < nsph; x++) I[x] = active ? I[x] * s : 0.0;
+            digest_direct<la, lb, lc, ld, false, true>(I, bp, kp, active, lane, D, F, n, a.fj, a.fk, &acc);
+            pending = pending || active;
         }
+        if (pending) {
+#pragma unroll
+            for (int i = 0; i < na; i++) {
+#pragma unroll
+                for (int k = 0; k < nc; k++) {
+                    double& v = acc.kac[(i * nc + k) * NT];
+                    if (v != 0.0) unsafeAtomicAdd(&F[(size_t)p[i] * n + r[k]], v);
+                    v = 0.0;
+                }
+#pragma unroll
+                for (int l = 0; l < nd; l++) {
+                    double& v = acc.kad[(i * nd + l) * NT];
+                    if (v != 0.0) unsafeAtomicAdd(&F[(size_t)p[i] * n + t[l]], v);
+                    v = 0.0;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < nc; k++)
+#pragma unroll
+            for (int l = 0; l < nd; l++) {
+                double& v = acc.jcd[(k * nd + l) * NT];
+                if (v != 0.0) unsafeAtomicAdd(&F[(size_t)r[k] * n + t[l]], v);
+                v = 0.0;
+            }
     }
 }
 
@@ -592,7 +835,9 @@ __global__ void __launch_bounds__(VLX_BLOCK)
 // the item lands in six small dense tiles indexed by group-local AOs,
 //   J_AB, J_CD, K_AC, K_AD, K_BC, K_BD,
 // accumulated with LDS atomics and flushed to HBM once per item.  The four
-// wavefronts share the kets and split the bras.
+// wavefronts share the kets and split the bras.  MULTI: the item's integrals
+// are digested into a.ndens densities (layout and factors as in
+// fock_kernel_multi), with one set of six tiles per density in LDS.
 __device__ __forceinline__ void flush_tile(double* __restrict__ F, const int n, const double* __restrict__ tile,
                                            const int nr, const int nc, const int* __restrict__ rows,
                                            const int* __restrict__ cols, const double f)
@@ -603,7 +848,7 @@ __device__ __forceinline__ void flush_tile(double* __restrict__ F, const int n, 
     }
 }
 
-template <int la, int lb, int lc, int ld, typename R, int NW, bool RS>
+template <int la, int lb, int lc, int ld, typename R, int NW, bool RS, bool MULTI = false>
 __global__ void __launch_bounds__(NW * VLX_WAVE)
     fock_kernel_tiled(const PairInfo* __restrict__ pairs_bra, const PrimPair* __restrict__ bra_prims,
                       const PairInfo* __restrict__ pairs_ket, const PrimPair* __restrict__ ket_prims,
@@ -611,7 +856,7 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                       const R* __restrict__ boys_tab, const double* __restrict__ D0, double* __restrict__ F0,
                       const MolDesc* __restrict__ mols, const int* __restrict__ k32, const int* __restrict__ glim,
                       const int* __restrict__ grp_off, const int* __restrict__ grp_n,
-                      const int* __restrict__ grp_ao, const FockScalars a)
+                      const int* __restrict__ grp_ao, const double* __restrict__ dfac, const FockScalars a)
 {
     constexpr int na = 2 * la + 1, nb = 2 * lb + 1, nc = 2 * lc + 1, nd = 2 * ld + 1;
     constexpr int nsph = na * nb * nc * nd;
@@ -619,9 +864,9 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
     extern __shared__ double tiles[];
     stage_boys<R>(tab, boys_tab);
 
-    const bool do_j = a.fj != 0.0, do_k = a.fk != 0.0;
     const int lane = threadIdx.x & (VLX_WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / VLX_WAVE);
+    const int nds = (MULTI && a.ndens > 1) ? a.ndens : 1;
 
     for (long long it = blockIdx.x; it < a.nitems; it += gridDim.x) {
         const TileItem ti = items[it];
@@ -669,7 +914,7 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
         double* t_kbc = t_kad + nA * nD;
         double* t_kbd = t_kbc + nB * nC;
         const int ntile = nA * nB + nC * nD + nA * nC + nA * nD + nB * nC + nB * nD;
-        for (int e = threadIdx.x; e < ntile; e += blockDim.x) tiles[e] = 0.0;
+        for (int e = threadIdx.x; e < nds * ntile; e += blockDim.x) tiles[e] = 0.0;
         __syncthreads();
 
         // lane = a run of primitive pairs of one ket pair (first | count << 16):
@@ -692,13 +937,15 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
             tl[l] = kp.loc_b + l;
         }
         R dcd[nc][nd];
+        if constexpr (!MULTI) {
 #pragma unroll
-        for (int k = 0; k < nc; k++)
+            for (int k = 0; k < nc; k++)
 #pragma unroll
-            for (int l = 0; l < nd; l++) dcd[k][l] = R(D[(size_t)r[k] * n + t[l]]);
+                for (int l = 0; l < nd; l++) dcd[k][l] = R(D[(size_t)r[k] * n + t[l]]);
+        }
         // J_CD depends on the lane's ket only: small ket blocks accumulate it in
         // registers over all bras of the item (one LDS atomic per element at the end)
-        constexpr bool jcd_in_regs = nc * nd <= 9;
+        constexpr bool jcd_in_regs = !MULTI && nc * nd <= 9;
         double jcd[jcd_in_regs ? nc : 1][jcd_in_regs ? nd : 1];
 #pragma unroll
         for (int k = 0; k < (jcd_in_regs ? nc : 1); k++)
@@ -728,105 +975,122 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
             for (int i = 0; i < na; i++) p[i] = bp.ao_a + i * bp.str_a;
 #pragma unroll
             for (int jb = 0; jb < nb; jb++) q[jb] = bp.ao_b + jb * bp.str_b;
-            if (do_j) {
-#pragma unroll
-                for (int i = 0; i < na; i++)
-#pragma unroll
-                    for (int jb = 0; jb < nb; jb++) {
-                        R v = R(0);
-#pragma unroll
-                        for (int k = 0; k < nc; k++)
-#pragma unroll
-                            for (int l = 0; l < nd; l++) v = fma(I[((i * nb + jb) * nc + k) * nd + l], dcd[k][l], v);
-                        // row sums by DPP; the four rows meet in the tile through LDS atomics
-                        // (the LDS pipe has room where the VALU does not)
-                        v = row_sum(v);
-                        if ((lane & 15) == 0 && v != R(0))
-                            unsafeAtomicAdd(&t_jab[(bp.loc_a + i) * nB + bp.loc_b + jb], static_cast<double>(v));
-                    }
-                if (jcd_in_regs || active) {
-                    R vcd[nc][nd];
+            for (int dd = 0; dd < nds; dd++) {
+                const double* __restrict__ Dd = MULTI ? D + dd * a.dstride : D;
+                double* t_jab = tiles + dd * ntile;
+                double* t_jcd = t_jab + nA * nB;
+                double* t_kac = t_jcd + nC * nD;
+                double* t_kad = t_kac + nA * nC;
+                double* t_kbc = t_kad + nA * nD;
+                double* t_kbd = t_kbc + nB * nC;
+                const bool do_j = (MULTI ? dfac[2 * dd] : a.fj) != 0.0;
+                const bool do_k = (MULTI ? dfac[2 * dd + 1] : a.fk) != 0.0;
+                if constexpr (MULTI) {
 #pragma unroll
                     for (int k = 0; k < nc; k++)
 #pragma unroll
-                        for (int l = 0; l < nd; l++) vcd[k][l] = R(0);
+                        for (int l = 0; l < nd; l++) dcd[k][l] = R(Dd[(size_t)r[k] * n + t[l]]);
+                }
+                if (do_j) {
 #pragma unroll
                     for (int i = 0; i < na; i++)
 #pragma unroll
                         for (int jb = 0; jb < nb; jb++) {
-                            const R dab = R(D[(size_t)p[i] * n + q[jb]]);
+                            R v = R(0);
+#pragma unroll
+                            for (int k = 0; k < nc; k++)
+#pragma unroll
+                                for (int l = 0; l < nd; l++) v = fma(I[((i * nb + jb) * nc + k) * nd + l], dcd[k][l], v);
+                            // row sums by DPP; the four rows meet in the tile through LDS atomics
+                            // (the LDS pipe has room where the VALU does not)
+                            v = row_sum(v);
+                            if ((lane & 15) == 0 && v != R(0))
+                                unsafeAtomicAdd(&t_jab[(bp.loc_a + i) * nB + bp.loc_b + jb], static_cast<double>(v));
+                        }
+                    if (jcd_in_regs || active) {
+                        R vcd[nc][nd];
+#pragma unroll
+                        for (int k = 0; k < nc; k++)
+#pragma unroll
+                            for (int l = 0; l < nd; l++) vcd[k][l] = R(0);
+#pragma unroll
+                        for (int i = 0; i < na; i++)
+#pragma unroll
+                            for (int jb = 0; jb < nb; jb++) {
+                                const R dab = R(Dd[(size_t)p[i] * n + q[jb]]);
+#pragma unroll
+                                for (int k = 0; k < nc; k++)
+#pragma unroll
+                                    for (int l = 0; l < nd; l++)
+                                        vcd[k][l] = fma(I[((i * nb + jb) * nc + k) * nd + l], dab, vcd[k][l]);
+                            }
+                        if constexpr (jcd_in_regs) {
+                            // inactive lanes add zeros (their integrals were replaced by exact zeros)
+#pragma unroll
+                            for (int k = 0; k < nc; k++)
+#pragma unroll
+                                for (int l = 0; l < nd; l++) jcd[k][l] += static_cast<double>(vcd[k][l]);
+                        } else {
 #pragma unroll
                             for (int k = 0; k < nc; k++)
 #pragma unroll
                                 for (int l = 0; l < nd; l++)
-                                    vcd[k][l] = fma(I[((i * nb + jb) * nc + k) * nd + l], dab, vcd[k][l]);
+                                    unsafeAtomicAdd(&t_jcd[rl[k] * nD + tl[l]], static_cast<double>(vcd[k][l]));
                         }
-                    if constexpr (jcd_in_regs) {
-                        // inactive lanes add zeros (their integrals were replaced by exact zeros)
-#pragma unroll
-                        for (int k = 0; k < nc; k++)
-#pragma unroll
-                            for (int l = 0; l < nd; l++) jcd[k][l] += static_cast<double>(vcd[k][l]);
-                    } else {
-#pragma unroll
-                        for (int k = 0; k < nc; k++)
-#pragma unroll
-                            for (int l = 0; l < nd; l++)
-                                unsafeAtomicAdd(&t_jcd[rl[k] * nD + tl[l]], static_cast<double>(vcd[k][l]));
                     }
                 }
-            }
-            if (do_k && active) {
+                if (do_k && active) {
 #pragma unroll
-                for (int i = 0; i < na; i++) {
+                    for (int i = 0; i < na; i++) {
 #pragma unroll
-                    for (int k = 0; k < nc; k++) {  // K_ac += I D_bd
-                        R v = R(0);
+                        for (int k = 0; k < nc; k++) {  // K_ac += I D_bd
+                            R v = R(0);
 #pragma unroll
-                        for (int jb = 0; jb < nb; jb++)
+                            for (int jb = 0; jb < nb; jb++)
 #pragma unroll
-                            for (int l = 0; l < nd; l++)
-                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(D[(size_t)q[jb] * n + t[l]]), v);
-                        unsafeAtomicAdd(&t_kac[(bp.loc_a + i) * nC + rl[k]], static_cast<double>(v));
+                                for (int l = 0; l < nd; l++)
+                                    v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(Dd[(size_t)q[jb] * n + t[l]]), v);
+                            unsafeAtomicAdd(&t_kac[(bp.loc_a + i) * nC + rl[k]], static_cast<double>(v));
+                        }
+#pragma unroll
+                        for (int l = 0; l < nd; l++) {  // K_ad += I D_bc
+                            R v = R(0);
+#pragma unroll
+                            for (int jb = 0; jb < nb; jb++)
+#pragma unroll
+                                for (int k = 0; k < nc; k++)
+                                    v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(Dd[(size_t)q[jb] * n + r[k]]), v);
+                            unsafeAtomicAdd(&t_kad[(bp.loc_a + i) * nD + tl[l]], static_cast<double>(v));
+                        }
                     }
 #pragma unroll
-                    for (int l = 0; l < nd; l++) {  // K_ad += I D_bc
-                        R v = R(0);
+                    for (int jb = 0; jb < nb; jb++) {
 #pragma unroll
-                        for (int jb = 0; jb < nb; jb++)
+                        for (int k = 0; k < nc; k++) {  // K_bc += I D_ad
+                            R v = R(0);
 #pragma unroll
-                            for (int k = 0; k < nc; k++)
-                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(D[(size_t)q[jb] * n + r[k]]), v);
-                        unsafeAtomicAdd(&t_kad[(bp.loc_a + i) * nD + tl[l]], static_cast<double>(v));
-                    }
-                }
+                            for (int i = 0; i < na; i++)
 #pragma unroll
-                for (int jb = 0; jb < nb; jb++) {
+                                for (int l = 0; l < nd; l++)
+                                    v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(Dd[(size_t)p[i] * n + t[l]]), v);
+                            unsafeAtomicAdd(&t_kbc[(bp.loc_b + jb) * nC + rl[k]], static_cast<double>(v));
+                        }
 #pragma unroll
-                    for (int k = 0; k < nc; k++) {  // K_bc += I D_ad
-                        R v = R(0);
+                        for (int l = 0; l < nd; l++) {  // K_bd += I D_ac
+                            R v = R(0);
 #pragma unroll
-                        for (int i = 0; i < na; i++)
+                            for (int i = 0; i < na; i++)
 #pragma unroll
-                            for (int l = 0; l < nd; l++)
-                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(D[(size_t)p[i] * n + t[l]]), v);
-                        unsafeAtomicAdd(&t_kbc[(bp.loc_b + jb) * nC + rl[k]], static_cast<double>(v));
-                    }
-#pragma unroll
-                    for (int l = 0; l < nd; l++) {  // K_bd += I D_ac
-                        R v = R(0);
-#pragma unroll
-                        for (int i = 0; i < na; i++)
-#pragma unroll
-                            for (int k = 0; k < nc; k++)
-                                v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(D[(size_t)p[i] * n + r[k]]), v);
-                        unsafeAtomicAdd(&t_kbd[(bp.loc_b + jb) * nD + tl[l]], static_cast<double>(v));
+                                for (int k = 0; k < nc; k++)
+                                    v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(Dd[(size_t)p[i] * n + r[k]]), v);
+                            unsafeAtomicAdd(&t_kbd[(bp.loc_b + jb) * nD + tl[l]], static_cast<double>(v));
+                        }
                     }
                 }
             }
         }
         if constexpr (jcd_in_regs) {
-            if (do_j && lane_ok) {
+            if (a.fj != 0.0 && lane_ok) {
 #pragma unroll
                 for (int k = 0; k < nc; k++)
 #pragma unroll
@@ -839,15 +1103,25 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
         const int* rb = grp_ao + grp_off[ti.gb * 4 + lb];
         const int* rc = grp_ao + grp_off[ti.gc * 4 + lc];
         const int* rd = grp_ao + grp_off[ti.gd * 4 + ld];
-        if (do_j) {
-            flush_tile(F, n, t_jab, nA, nB, ra, rb, a.fj);
-            flush_tile(F, n, t_jcd, nC, nD, rc, rd, a.fj);
-        }
-        if (do_k) {
-            flush_tile(F, n, t_kac, nA, nC, ra, rc, a.fk);
-            flush_tile(F, n, t_kad, nA, nD, ra, rd, a.fk);
-            flush_tile(F, n, t_kbc, nB, nC, rb, rc, a.fk);
-            flush_tile(F, n, t_kbd, nB, nD, rb, rd, a.fk);
+        for (int dd = 0; dd < nds; dd++) {
+            double* __restrict__ Fd = MULTI ? F + dd * a.dstride : F;
+            const double fj = MULTI ? dfac[2 * dd] : a.fj, fk = MULTI ? dfac[2 * dd + 1] : a.fk;
+            const double* t_jab = tiles + dd * ntile;
+            const double* t_jcd = t_jab + nA * nB;
+            const double* t_kac = t_jcd + nC * nD;
+            const double* t_kad = t_kac + nA * nC;
+            const double* t_kbc = t_kad + nA * nD;
+            const double* t_kbd = t_kbc + nB * nC;
+            if (fj != 0.0) {
+                flush_tile(Fd, n, t_jab, nA, nB, ra, rb, fj);
+                flush_tile(Fd, n, t_jcd, nC, nD, rc, rd, fj);
+            }
+            if (fk != 0.0) {
+                flush_tile(Fd, n, t_kac, nA, nC, ra, rc, fk);
+                flush_tile(Fd, n, t_kad, nA, nD, ra, rd, fk);
+                flush_tile(Fd, n, t_kbc, nB, nC, rb, rc, fk);
+                flush_tile(Fd, n, t_kbd, nB, nD, rb, rd, fk);
+            }
         }
         __syncthreads();
     }
@@ -922,26 +1196,63 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
 // FP64 launcher picks the erf(omega r)/r instantiation when omega2 > 0; the
 // FP32 launcher has the Coulomb one only (range-separated builds run no FP32
 // items).
-#define VLX_TILED_LAUNCH(la, lb, lc, ld, R, NW, RS, tab)                                                \
-    hipLaunchKernelGGL((fock_kernel_tiled<la, lb, lc, ld, R, NW, RS>), dim3(grid), dim3((NW) * VLX_WAVE), \
-                       a.tile_lds_bytes, s, a.bra, a.bra_prims, a.ket, a.ket_prims, a.titems, a.kentries,  \
-                       tab, a.D, a.F, a.mols, a.k32, a.glim, a.grp_off, a.grp_n, a.grp_ao, a.sc)
+#define VLX_TILED_LAUNCH(la, lb, lc, ld, R, NW, RS, tab, MULTI)                                         \
+    hipLaunchKernelGGL((fock_kernel_tiled<la, lb, lc, ld, R, NW, RS, MULTI>), dim3(grid),                 \
+                       dim3((NW) * VLX_WAVE), a.tile_lds_bytes, s, a.bra, a.bra_prims, a.ket, a.ket_prims,   \
+                       a.titems, a.kentries, tab, a.D, a.F, a.mols, a.k32, a.glim, a.grp_off, a.grp_n,      \
+                       a.grp_ao, a.dfac, a.sc)
 #define VLX_INSTANTIATE_TILED(la, lb, lc, ld, NWB)                                                      \
     void launch_tiled_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                      \
     {                                                                                                   \
         const bool w4 = a.tiled_waves == 4 || (NWB) == 4;                                               \
         if (a.sc.omega2 > 0.0) {                                                                        \
-            if (w4) VLX_TILED_LAUNCH(la, lb, lc, ld, double, 4, true, a.boys_tab);                      \
-            else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, true, a.boys_tab);                       \
+            if (w4) VLX_TILED_LAUNCH(la, lb, lc, ld, double, 4, true, a.boys_tab, false);               \
+            else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, true, a.boys_tab, false);                \
         } else {                                                                                        \
-            if (w4) VLX_TILED_LAUNCH(la, lb, lc, ld, double, 4, false, a.boys_tab);                     \
-            else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, false, a.boys_tab);                      \
+            if (w4) VLX_TILED_LAUNCH(la, lb, lc, ld, double, 4, false, a.boys_tab, false);              \
+            else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, false, a.boys_tab, false);               \
         }                                                                                               \
     }                                                                                                   \
     void launch_tiled32_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                    \
     {                                                                                                   \
-        if (a.tiled_waves == 4 || (NWB) == 4) VLX_TILED_LAUNCH(la, lb, lc, ld, float, 4, false, a.boys_tab_f); \
-        else VLX_TILED_LAUNCH(la, lb, lc, ld, float, NWB, false, a.boys_tab_f);                         \
+        if (a.tiled_waves == 4 || (NWB) == 4) VLX_TILED_LAUNCH(la, lb, lc, ld, float, 4, false, a.boys_tab_f, false); \
+        else VLX_TILED_LAUNCH(la, lb, lc, ld, float, NWB, false, a.boys_tab_f, false);                  \
+    }                                                                                                   \
+    void launch_tiled_multi_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                \
+    {                                                                                                   \
+        if (a.sc.omega2 > 0.0) VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, true, a.boys_tab, true);   \
+        else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, false, a.boys_tab, true);                    \
+    }
+
+// multi-density form of a class's untiled kernels (work-item layout in FockScalars::bs_layout)
+#define VLX_INSTANTIATE_MULTI(la, lb, lc, ld)                                                           \
+    void launch_fock_multi_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                 \
+    {                                                                                                   \
+        if (a.sc.omega2 > 0.0)                                                                          \
+            hipLaunchKernelGGL((fock_kernel_multi<la, lb, lc, ld, true>), dim3(grid), dim3(VLX_BLOCK), 0, \
+                               s, a.bra, a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F, \
+                               a.mols, a.dfac, a.sc);                                                   \
+        else                                                                                            \
+            hipLaunchKernelGGL((fock_kernel_multi<la, lb, lc, ld, false>), dim3(grid), dim3(VLX_BLOCK), 0, \
+                               s, a.bra, a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, a.F, \
+                               a.mols, a.dfac, a.sc);                                                   \
+    }
+
+// ket-stationary kernel with LDS accumulators (NW waves per block); `grid` counts wavefronts
+#define VLX_INSTANTIATE_KSL(la, lb, lc, ld, NW)                                                         \
+    void launch_ksl_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                        \
+    {                                                                                                   \
+        constexpr int lds = ((2 * la + 1) * (2 * lc + 1) + (2 * la + 1) * (2 * ld + 1) +                 \
+                             (2 * lc + 1) * (2 * ld + 1)) * (NW) * VLX_WAVE * 8;                        \
+        const int blocks = (grid + (NW) - 1) / (NW);                                                    \
+        if (a.sc.omega2 > 0.0)                                                                          \
+            hipLaunchKernelGGL((fock_kernel_ksl<la, lb, lc, ld, true, NW>), dim3(blocks), dim3((NW) * VLX_WAVE), \
+                               lds, s, a.bra, a.bra_prims, a.ket, a.ket_prims, a.items, a.boys_tab, a.D, \
+                               a.F, a.mols, a.glim, a.sc);                                              \
+        else                                                                                            \
+            hipLaunchKernelGGL((fock_kernel_ksl<la, lb, lc, ld, false, NW>), dim3(blocks),              \
+                               dim3((NW) * VLX_WAVE), lds, s, a.bra, a.bra_prims, a.ket, a.ket_prims,   \
+                               a.items, a.boys_tab, a.D, a.F, a.mols, a.glim, a.sc);                    \
     }
 
 #define VLX_INSTANTIATE_DIAG(la, lb)                                                                    \

@@ -300,6 +300,20 @@ def tiled_waves(la, lb, lc, ld):
     return 16 if L <= 2 else (8 if L == 3 else 4)
 
 
+def ksl_waves(la, lb, lc, ld, var):
+    """Waves per block of the ket-stationary kernel with LDS accumulators, an alternative variant of
+    the bra-stationary classes, or 0 (none): (na nc + na nd + nc nd) doubles per thread must fit in
+    LDS beside the Boys table with 4 waves (one wave per SIMD, as the register use allows)."""
+    na, nc, nd = 2 * la + 1, 2 * lc + 1, 2 * ld + 1
+    nacc = na * nc + na * nd + nc * nd
+    return 4 if var == 'bs' and nacc <= KSL_MAX_SLOTS else 0
+
+
+# lane-private accumulator slots of fock_kernel_ksl with 4-wave blocks: 64 x 256 threads x 8 B =
+# 128 KiB, plus the 25.5 KB Boys table, within the 160 KB of LDS
+KSL_MAX_SLOTS = 64
+
+
 # classes with la+lb+lc+ld >= MMD_LMIN also get the matrix-core MD Fock kernel as
 # an alternative variant (the autotuner picks it per class)
 MMD_LMIN = 4
@@ -345,7 +359,10 @@ def main(outdir, lmax):
             kern = 'fock_kernel' if var == 'ks' else 'fock_kernel_bs'
             hdr = '../md_kernels.h' if mmd_eligible(la, lb, lc, ld, var) else '../eri_kernels.h'
             src = ['// generated by gen_eri.py -- do not edit', f'#include "{hdr}"', '', body, '',
-                   f'VLX_INSTANTIATE_CLASS({la}, {lb}, {lc}, {ld}, {kern})']
+                   f'VLX_INSTANTIATE_CLASS({la}, {lb}, {lc}, {ld}, {kern})',
+                   f'VLX_INSTANTIATE_MULTI({la}, {lb}, {lc}, {ld})']
+            if ksl_waves(la, lb, lc, ld, var):
+                src.append(f'VLX_INSTANTIATE_KSL({la}, {lb}, {lc}, {ld}, {ksl_waves(la, lb, lc, ld, var)})')
             if mmd_eligible(la, lb, lc, ld, var):
                 src.append(f'VLX_INSTANTIATE_MMD({la}, {lb}, {lc}, {ld})')
             if (la, lb) == (lc, ld):
@@ -361,6 +378,10 @@ def main(outdir, lmax):
     d = ['// generated by gen_eri.py -- do not edit', f'#define VLX_GEN_LMAX {lmax}']
     for (la, lb, lc, ld) in written:
         d.append(f'VLX_DECLARE_CLASS({la}, {lb}, {lc}, {ld})')
+        if variants[(la, lb, lc, ld)] != 'md':
+            d.append(f'VLX_DECLARE_MULTI({la}, {lb}, {lc}, {ld})')
+        if ksl_waves(la, lb, lc, ld, variants[(la, lb, lc, ld)]):
+            d.append(f'VLX_DECLARE_KSL({la}, {lb}, {lc}, {ld})')
         if tiled_eligible(la, lb, lc, ld) and variants[(la, lb, lc, ld)] != 'md':
             d.append(f'VLX_DECLARE_TILED({la}, {lb}, {lc}, {ld})')
         if mmd_eligible(la, lb, lc, ld, variants[(la, lb, lc, ld)]) and variants[(la, lb, lc, ld)] != 'md':
@@ -379,8 +400,12 @@ def main(outdir, lmax):
         var = variants[(la, lb, lc, ld)]
         mm = (f'&launch_fock_{la}{lb}{lc}{ld}' if var == 'md' else
               f'&launch_mmd_{la}{lb}{lc}{ld}' if mmd_eligible(la, lb, lc, ld, var) else 'nullptr')
+        # multi-density forms: the matrix-core MD kernel loops over the densities itself
+        fm = f'&launch_fock_{la}{lb}{lc}{ld}' if var == 'md' else f'&launch_fock_multi_{la}{lb}{lc}{ld}'
+        tlm = f'&launch_tiled_multi_{la}{lb}{lc}{ld}' if has_tiled else 'nullptr'
+        ksl = f'&launch_ksl_{la}{lb}{lc}{ld}' if ksl_waves(la, lb, lc, ld, var) else 'nullptr'
         d.append(f'    {{{x}, {y}, {la}, {lb}, {lc}, {ld}, {nv}, {nh}, {ks}, &launch_fock_{la}{lb}{lc}{ld}, '
-                 f'&launch_eri_{la}{lb}{lc}{ld}, {tl}, {tl32}, {mm}}},')
+                 f'&launch_eri_{la}{lb}{lc}{ld}, {tl}, {tl32}, {mm}, {fm}, {tlm}, {ksl}}},')
     d.append('};')
     d.append('static const DiagLauncher g_diag_table[] = {')
     for (la, lb) in pcs:

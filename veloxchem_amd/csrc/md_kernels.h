@@ -276,9 +276,10 @@ __device__ void quartet(const PrimPair* __restrict__ bra_prims, int bra_np, cons
 }
 
 // Fock digestion of one block in LDS (factor s included), lanes over outputs.
-template <int la, int lb, int lc, int ld>
+template <int la, int lb, int lc, int ld, bool DET = false>
 __device__ void digest(const PairInfo& bp, const PairInfo& kp, double s, const double* __restrict__ out,
-                       const double* __restrict__ D, double* __restrict__ F, const int n, const FockScalars& a)
+                       const double* __restrict__ D, double* __restrict__ F, const int n, const double fj,
+                       const double fk)
 {
     using S = Sizes<la, lb, lc, ld>;
     constexpr int na = S::nsa, nb = S::nsb, nc = S::nsc, nd = S::nsd;
@@ -288,50 +289,50 @@ __device__ void digest(const PairInfo& bp, const PairInfo& kp, double s, const d
     auto R = [&](int k) { return kp.ao_a + k * kp.str_a; };
     auto T = [&](int l) { return kp.ao_b + l * kp.str_b; };
     const int lane = threadIdx.x;
-    if (a.fj != 0.0) {
+    if (fj != 0.0) {
         for (int e = lane; e < na * nb; e += blockDim.x) {
             const int i = e / nb, j = e % nb;
             double v = 0.0;
             for (int k = 0; k < nc; k++)
                 for (int l = 0; l < nd; l++) v = fma(I(i, j, k, l), D[(size_t)R(k) * n + T(l)], v);
-            unsafeAtomicAdd(&F[(size_t)P(i) * n + Q(j)], a.fj * s * v);
+            acc_add<DET>(F, (size_t)P(i) * n + Q(j), fj * s * v);
         }
         for (int e = lane; e < nc * nd; e += blockDim.x) {
             const int k = e / nd, l = e % nd;
             double v = 0.0;
             for (int i = 0; i < na; i++)
                 for (int j = 0; j < nb; j++) v = fma(I(i, j, k, l), D[(size_t)P(i) * n + Q(j)], v);
-            unsafeAtomicAdd(&F[(size_t)R(k) * n + T(l)], a.fj * s * v);
+            acc_add<DET>(F, (size_t)R(k) * n + T(l), fj * s * v);
         }
     }
-    if (a.fk != 0.0) {
+    if (fk != 0.0) {
         for (int e = lane; e < na * nc; e += blockDim.x) {  // F'_ac += I D_bd
             const int i = e / nc, k = e % nc;
             double v = 0.0;
             for (int j = 0; j < nb; j++)
                 for (int l = 0; l < nd; l++) v = fma(I(i, j, k, l), D[(size_t)Q(j) * n + T(l)], v);
-            unsafeAtomicAdd(&F[(size_t)P(i) * n + R(k)], a.fk * s * v);
+            acc_add<DET>(F, (size_t)P(i) * n + R(k), fk * s * v);
         }
         for (int e = lane; e < na * nd; e += blockDim.x) {  // F'_ad += I D_bc
             const int i = e / nd, l = e % nd;
             double v = 0.0;
             for (int j = 0; j < nb; j++)
                 for (int k = 0; k < nc; k++) v = fma(I(i, j, k, l), D[(size_t)Q(j) * n + R(k)], v);
-            unsafeAtomicAdd(&F[(size_t)P(i) * n + T(l)], a.fk * s * v);
+            acc_add<DET>(F, (size_t)P(i) * n + T(l), fk * s * v);
         }
         for (int e = lane; e < nb * nc; e += blockDim.x) {  // F'_bc += I D_ad
             const int j = e / nc, k = e % nc;
             double v = 0.0;
             for (int i = 0; i < na; i++)
                 for (int l = 0; l < nd; l++) v = fma(I(i, j, k, l), D[(size_t)P(i) * n + T(l)], v);
-            unsafeAtomicAdd(&F[(size_t)Q(j) * n + R(k)], a.fk * s * v);
+            acc_add<DET>(F, (size_t)Q(j) * n + R(k), fk * s * v);
         }
         for (int e = lane; e < nb * nd; e += blockDim.x) {  // F'_bd += I D_ac
             const int j = e / nd, l = e % nd;
             double v = 0.0;
             for (int i = 0; i < na; i++)
                 for (int k = 0; k < nc; k++) v = fma(I(i, j, k, l), D[(size_t)P(i) * n + R(k)], v);
-            unsafeAtomicAdd(&F[(size_t)Q(j) * n + T(l)], a.fk * s * v);
+            acc_add<DET>(F, (size_t)Q(j) * n + T(l), fk * s * v);
         }
     }
 }
@@ -395,7 +396,10 @@ struct MmdSizes {
     static constexpr bool PRE_O = MX <= XC && KO * MT <= 16;
 };
 
-// Work item {j0, i0, i1}: bra i0 with kets [j0, i1), one shell quartet at a time.
+// Work item {j0, i0, i1}: bra i0 with kets [j0, i1), one shell quartet at a time.  With
+// a.ndens > 1 each block is digested into every density (density d at D0 + d * dstride, its F' at
+// F0 + d * dstride, factors dfac[2 d], dfac[2 d + 1]).  a.det: F0 holds fixed-point accumulators
+// (fx_add, two words per element) and the factors come from dfac also for one density.
 template <int la, int lb, int lc, int ld, bool RS>
 __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4)))
     fock_kernel_mmd(const PairInfo* __restrict__ bra, const PrimPair* __restrict__ bra_prims,
@@ -403,7 +407,7 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
                     const PrimPair* __restrict__ ket_prims, const double* __restrict__ ket_u,
                     const WorkItem* __restrict__ items, const double* __restrict__ boys_tab,
                     const double* __restrict__ D0, double* __restrict__ F0, const MolDesc* __restrict__ mols,
-                    const int* __restrict__ glim, const FockScalars a)
+                    const int* __restrict__ glim, const double* __restrict__ dfac, const FockScalars a)
 {
     using S = MmdSizes<la, lb, lc, ld>;
     constexpr int L = S::L;
@@ -579,7 +583,18 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
             if (bp.same) sf *= 0.5;
             if (kp.same) sf *= 0.5;
             if (a.same_class && bp.rank == j) sf *= 0.5;
-            md::digest<la, lb, lc, ld>(bp, kp, sf, outs, D, F, md.nao, a);
+            if (a.det) {  // F0: fixed-point accumulators (two words per element), factors in dfac
+                for (int dd = 0; dd < max(a.ndens, 1); dd++)
+                    md::digest<la, lb, lc, ld, true>(bp, kp, sf, outs, D + dd * a.dstride,
+                                                     F0 + 2 * (md.base + dd * a.dstride), md.nao, dfac[2 * dd],
+                                                     dfac[2 * dd + 1]);
+            } else if (a.ndens > 1) {
+                for (int dd = 0; dd < a.ndens; dd++)
+                    md::digest<la, lb, lc, ld>(bp, kp, sf, outs, D + dd * a.dstride, F + dd * a.dstride, md.nao,
+                                               dfac[2 * dd], dfac[2 * dd + 1]);
+            } else {
+                md::digest<la, lb, lc, ld>(bp, kp, sf, outs, D, F, md.nao, a.fj, a.fk);
+            }
             __syncthreads();
         }
     }
@@ -638,7 +653,7 @@ __global__ void __launch_bounds__(VLX_WAVE) diag_kernel_md(const DiagArgs a)
 #define VLX_MMD_LAUNCH(la, lb, lc, ld, RS)                                                              \
     hipLaunchKernelGGL((fock_kernel_mmd<la, lb, lc, ld, RS>), dim3(g), dim3(VLX_WAVE), 0, s, a.bra,     \
                        a.bra_prims, a.bra_u, a.ket, a.ket_prims, a.ket_u, a.items, a.boys_tab, a.D, a.F, \
-                       a.mols, a.glim, a.sc)
+                       a.mols, a.glim, a.dfac, a.sc)
 // launch_mmd_: the matrix-core kernel as an alternative variant of a class
 // whose default kernel is generated (same MD work items).
 #define VLX_INSTANTIATE_MMD(la, lb, lc, ld)                                                             \

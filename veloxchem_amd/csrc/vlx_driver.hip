@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cctype>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -165,6 +166,7 @@ struct ClassSchedule {
     bool tiled = false;
     bool singles = false; // tiled over single-primitive ket entries
     bool mmd = false;     // matrix-core MD kernel (MD work items)
+    bool ksl = false;     // ket-stationary kernel with LDS accumulators (ket-stationary work items)
     int tiled_waves = 0;  // 4: 4-wave blocks; 0: the class's wide block
     int tile_lds = 0;  // bytes of dynamic LDS (tiled)
     std::vector<TileItem> titems;
@@ -184,7 +186,7 @@ struct vlx_screener;
 struct BatchLaunch {
     int entry = -1;       // index into g_class_table
     int variant = 0, tiled_waves = 0, tile_lds = 0, stream = 0;
-    bool tiled = false, singles = false, mmd = false;
+    bool tiled = false, singles = false, mmd = false, ksl = false;
     std::vector<TileItem> titems;
     std::vector<WorkItem> items;
     DevBuf<TileItem> d_titems;
@@ -250,7 +252,8 @@ enum {
     VARIANT_MMD = 3,
     VARIANT_TILED4_P1 = 4,
     VARIANT_TILED_WIDE_P1 = 5,
-    NVARIANTS = 6
+    VARIANT_KSL = 6,  // ket-stationary items with lane-private LDS accumulators (bra-stationary classes)
+    NVARIANTS = 7
 };
 bool variant_tiled(int v)
 {
@@ -301,6 +304,14 @@ struct vlx_context {
     DevBuf<float> d_boys_f;         // the Boys table in FP32 (mixed precision)
     DevBuf<double> d_density, d_fock, d_faccum;
     DevBuf<double> d_sym, d_anti;   // symmetric and antisymmetric parts of a general density
+    // multi-density builds: (fj, fk) per digested density, the densities when general ones are
+    // split, and the F' accumulators of all of them
+    DevBuf<double> d_dfac, d_mdens, d_mfacc;
+    std::vector<double> h_dfac;
+    // deterministic mode: every build through the direct-digestion kernels with fixed-point
+    // accumulation into d_fx (two 64-bit words per F' element), the same bits on every run
+    bool deterministic = false;
+    DevBuf<unsigned long long> d_fx;
     DevBuf<unsigned long long> d_dmax;  // max |D| of the current build (bit pattern)
     double mixed_tau = 0.0;         // mixed precision: FP32 below this Schwarz x max|D| bound; 0 = FP64 only
     int last_k32 = NO_FP32;         // bin-sum threshold of the last build
@@ -622,11 +633,13 @@ const DiagLauncher* find_diag(int x)
 // src/t4c_func/T4CLocalDistributor.cpp: 2jk (4, -1), 2jkx (4, -xf),
 // j (2, 0), k (0, 1), kx (0, xf); then G = F' + F'^T.  The range-separated
 // labels j_rs, k_rs, kx_rs digest like j, k, kx on erf-attenuated integrals
-// (src/t4c_func/T4CMatrixDistributor.cpp:51-54); omega is ignored otherwise.
+// (src/t4c_func/T4CMatrixDistributor.cpp:51-54); omega is ignored otherwise.  Labels are
+// case-insensitive, as the distributors lower-case them (T4CMatrixDistributor.cpp:18).
 int label_factors(const char* label, double xf, double* fj, double* fk, bool* range_separated)
 {
     if (!label) return fail(VLX_ERR_ARG, "label is NULL");
-    const std::string s(label);
+    std::string s(label);
+    for (char& c : s) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
     *range_separated = false;
     if (s == "2jk") { *fj = 4.0; *fk = -1.0; }
     else if (s == "2jkx") { *fj = 4.0; *fk = -xf; }
@@ -707,6 +720,15 @@ __global__ void antisym_add_kernel(const double* __restrict__ f, double* __restr
     if (q < n) g[(size_t)p * n + q] += c * (f[(size_t)p * n + q] - f[(size_t)q * n + p]);
 }
 
+// F' from the fixed-point accumulators: hi 2^-32 + lo 2^-96 (hi signed)
+__global__ void fx_to_double_kernel(const unsigned long long* __restrict__ fx, double* __restrict__ f, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const long long hi = static_cast<long long>(fx[2 * i + 1]);
+        f[i] = static_cast<double>(hi) * 0x1p-32 + static_cast<double>(fx[2 * i]) * 0x1p-96;
+    }
+}
+
 // S = (D + D^T) / 2, A = (D - D^T) / 2
 __global__ void split_kernel(const double* __restrict__ d, double* __restrict__ s, double* __restrict__ a, int n)
 {
@@ -783,9 +805,11 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
         }
         if (variant_tiled(var) && !can_tile) var = VARIANT_UNTILED;
         if (var == VARIANT_MMD && !ce.mmd) var = VARIANT_UNTILED;
+        if (var == VARIANT_KSL && !ce.ksl) var = VARIANT_UNTILED;
         if (ce.variant == VLX_WAVE_MD) var = VARIANT_MMD;  // no generated kernel
         sch->variant = var;
         sch->mmd = var == VARIANT_MMD;
+        sch->ksl = var == VARIANT_KSL;
         if (variant_tiled(var)) {
             // tiled items: (bra segment prefix, chunk of <= 64 kets of a ket segment)
             const PairSet& tb = unscreened ? s->all_tiled_bra[ce.x] : s->kept_tiled_bra[ce.x];
@@ -830,7 +854,7 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
                     }
                 }
             }
-        } else if (!sch->mmd && ce.variant == VLX_KET_STATIONARY) {
+        } else if (!sch->mmd && (ce.variant == VLX_KET_STATIONARY || sch->ksl)) {
             // work items: 64 consecutive kets x a chunk of bras; the chunk shrinks
             // for classes with little work so that the launch still has enough items
             std::vector<std::pair<int, int>> ranges;
@@ -902,7 +926,7 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
             const double pcost = ce.vrr_flops + 60.0, qcost = ce.hrr_flops + 12.0 * nsph;
             std::vector<double> cost;
             cost.reserve(sch->items.size());
-            const bool ks = ce.variant == VLX_KET_STATIONARY;
+            const bool ks = ce.variant == VLX_KET_STATIONARY || sch->ksl;
             for (const auto& wi : sch->items) {
                 const int j1 = ks ? std::min(wi.j0 + 64, ket.size()) : std::min(wi.j0 + 64, wi.i1);
                 int kmax = 0;
@@ -1105,6 +1129,13 @@ int vlx_set_density_screening(vlx_context* ctx, double threshold)
     if (!(threshold >= 0.0) || !std::isfinite(threshold))
         return fail(VLX_ERR_ARG, "density screening threshold must be finite and >= 0 (0 = off)");
     ctx->dscreen_tau = threshold;
+    return VLX_OK;
+}
+
+int vlx_set_deterministic(vlx_context* ctx, int enable)
+{
+    if (!ctx) return fail(VLX_ERR_ARG, "context is NULL");
+    ctx->deterministic = enable != 0;
     return VLX_OK;
 }
 
@@ -1539,6 +1570,7 @@ int vlx_fock_autotune(vlx_context* ctx, vlx_screener* scr, const double* d_densi
 {
     if (!ctx || !scr || !d_density) return fail(VLX_ERR_ARG, "NULL argument to vlx_fock_autotune");
     if (scr->ctx != ctx) return fail(VLX_ERR_ARG, "screener belongs to another context");
+    if (ctx->deterministic) return fail(VLX_ERR_ARG, "no autotuning in the deterministic mode (one kernel family)");
     if (repeats < 1) repeats = 1;
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t nn = (size_t)scr->nao * scr->nao;
@@ -1707,9 +1739,11 @@ int vlx_fock_class_kernel(vlx_context* ctx, vlx_screener* scr, int ithreshold, i
                           sp->tiled_waves == 4 ? 4 : wide);
         } else {
             const char* k = sp->mmd                             ? "fock_kernel_mmd"
+                            : sp->ksl                           ? "fock_kernel_ksl"
                             : (ce.variant == VLX_KET_STATIONARY) ? "fock_kernel"
                                                                 : "fock_kernel_bs";
-            std::snprintf(name, len, "%s<%d,%d,%d,%d,false>", k, la, lb, lc, ld);
+            if (sp->ksl) std::snprintf(name, len, "%s<%d,%d,%d,%d,false,4>", k, la, lb, lc, ld);
+            else std::snprintf(name, len, "%s<%d,%d,%d,%d,false>", k, la, lb, lc, ld);
         }
         return VLX_OK;
     }
@@ -1754,14 +1788,22 @@ namespace {
 // Enqueues the class kernels of one build that accumulate F' (nao x nao) from D: on `stream`, or,
 // when it is null, on the side stream each class schedule was given.  solo: 1 only the classes
 // that run alone (stream -1), 0 only the others, -1 all.  No synchronisation.
+// ndens > 1: the multi-density kernels digest each quartet into ndens densities, D and F' of
+// density c at d_density + c nao^2 and F + c nao^2, factors (fj, fk) of density c at dfac[2 c]
+// (device, and on the host in ctx->h_dfac); fj and fk are then unused.  det: the same kernels with
+// fixed-point accumulation, F being the accumulator array (two words per element); the schedules
+// must be untiled.
 int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::unique_ptr<ClassSchedule>>& list,
                     const double* d_density, double* F, double fj, double fk, double omega2, int ithreshold,
-                    hipStream_t stream, int solo = -1, bool allow_timing = true, const int* glim = nullptr)
+                    hipStream_t stream, int solo = -1, bool allow_timing = true, const int* glim = nullptr,
+                    int ndens = 1, const double* dfac = nullptr, bool det = false)
 {
+    const bool multi = ndens > 1 || det;
     const bool timing = ctx->timing && allow_timing;
     const bool unscreened = ithreshold < 0;
+    const long long nn = (long long)scr->nao * scr->nao;
     for (const auto& schp : list) {
-        if (fj == 0.0 && fk == 0.0) break;  // nothing to digest: G = 0
+        if (!multi && fj == 0.0 && fk == 0.0) break;  // nothing to digest: G = 0
         const ClassSchedule& sch = *schp;
         if (solo >= 0 && (sch.stream < 0) != (solo == 1)) continue;
         const auto& ce = g_class_table[sch.entry];
@@ -1800,6 +1842,11 @@ int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::
         a.sc.fj = fj;
         a.sc.fk = fk;
         a.sc.omega2 = omega2;
+        a.sc.ndens = ndens;
+        a.sc.dstride = nn;
+        a.sc.det = det ? 1 : 0;
+        a.sc.bs_layout = (ce.variant == VLX_BRA_STATIONARY && !sch.ksl) ? 1 : 0;
+        a.dfac = dfac;
         const int grid = sch.tiled ? static_cast<int>(std::min<long long>(a.sc.nitems, ctx->tiled_grid))
                          : sch.mmd ? static_cast<int>(std::min<long long>(a.sc.nitems, ctx->mmd_grid))
                                    : static_cast<int>(untiled_waves(a.sc.nitems, ctx->untiled_wave_cap) / VLX_WAVES_PER_BLOCK);
@@ -1816,7 +1863,54 @@ int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::
             }
             HIP_TRY(hipEventRecord(pend.start, st));
         }
-        if (sch.tiled) {
+        if (multi) {
+            if (det && sch.tiled) return fail(VLX_ERR_ARG, "deterministic builds run the untiled kernels only");
+            // Kernels whose single-density form keeps accumulators the multi-density form cannot
+            // hold run once per density instead (the integrals are recomputed): ks-lds (one density
+            // of LDS slots), the ket-stationary register accumulators (the multi-density direct
+            // digestion measured 4.8x slower than four launches) and the tiled classes above L = 2
+            // or over single primitives (k tile sets cut the occupancy: 1.1-1.9x slower).  MD,
+            // bra-stationary and L <= 2 tiled classes share the integrals (0.5-0.95x of k launches,
+            // def2-SVP and def2-TZVP water clusters).
+            const int L = ce.la + ce.lb + ce.lc + ce.ld;
+            const bool per_density =
+                !det && (sch.ksl || (!sch.tiled && !sch.mmd && ce.variant == VLX_KET_STATIONARY) ||
+                         (sch.tiled && (sch.singles || L > 2)));
+            if (per_density) {
+                const int waves = static_cast<int>(untiled_waves(a.sc.nitems, ctx->untiled_wave_cap));
+                for (int c = 0; c < ndens; c++) {
+                    FockArgs a1 = a;
+                    a1.sc.ndens = 1;
+                    a1.sc.fj = ctx->h_dfac[2 * c];
+                    a1.sc.fk = ctx->h_dfac[2 * c + 1];
+                    a1.D = d_density + c * nn;
+                    a1.F = F + c * nn;
+                    if (a1.sc.fj == 0.0 && a1.sc.fk == 0.0) continue;
+                    if (sch.ksl) ce.ksl(a1, waves, st);
+                    else if (sch.tiled) {
+                        if (!sch.titems.empty()) ce.tiled(a1, grid, st);
+                    } else ce.fock(a1, grid, st);
+                }
+            } else if (sch.tiled) {
+                // one set of six tiles per density in LDS: as many densities per launch as fit
+                // beside the FP64 Boys table; the integrals are recomputed per group of densities
+                const int lds_max = 160 * 1024 - static_cast<int>(BOYS_TAB_SIZE * sizeof(double)) - 1024;
+                const int chunk = std::max(1, std::min(ndens, lds_max / std::max(1, sch.tile_lds)));
+                for (int c0 = 0; c0 < ndens && !sch.titems.empty(); c0 += chunk) {
+                    FockArgs am = a;
+                    am.sc.ndens = std::min(chunk, ndens - c0);
+                    am.D = d_density + c0 * nn;
+                    am.F = F + c0 * nn;
+                    am.dfac = dfac + 2 * c0;
+                    am.tile_lds_bytes = am.sc.ndens * sch.tile_lds;
+                    ce.tiled_multi(am, grid, st);
+                }
+            } else if (sch.mmd) {
+                ce.mmd(a, grid, st);
+            } else {
+                ce.fock_multi(a, grid, st);
+            }
+        } else if (sch.tiled) {
             if (!sch.titems.empty()) ce.tiled(a, grid, st);
             if (!sch.titems32.empty()) {
                 FockArgs a32 = a;
@@ -1827,6 +1921,8 @@ int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::
             }
         } else if (sch.mmd) {
             ce.mmd(a, grid, st);
+        } else if (sch.ksl) {
+            ce.ksl(a, static_cast<int>(untiled_waves(a.sc.nitems, ctx->untiled_wave_cap)), st);
         } else {
             ce.fock(a, grid, st);
         }
@@ -1849,6 +1945,56 @@ int join_side_streams(vlx_context* ctx)
     return VLX_OK;
 }
 
+// One pass over the integrals for m densities packed at dens (nao x nao each) with factors h_fac
+// (fj, fk of density c at h_fac[2 c]): F' of density c into facc + c nao^2.  FP64 with the Schwarz
+// selection.  m = 1 runs the single-density kernels; in the context's deterministic mode every class
+// runs the direct-digestion kernels (schedules of the untiled variant) with fixed-point accumulation,
+// converted to FP64 at the end.
+int multi_pass(vlx_context* ctx, vlx_screener* scr, const double* dens, int m, const double* h_fac, double omega2,
+               int ithreshold, double* facc)
+{
+    const size_t nn = (size_t)scr->nao * scr->nao;
+    const bool det = ctx->deterministic;
+    ctx->h_dfac.assign(h_fac, h_fac + 2 * (size_t)m);
+    HIP_TRY(ctx->d_dfac.alloc(2 * (size_t)m));
+    HIP_TRY(hipMemcpyAsync(ctx->d_dfac.ptr, ctx->h_dfac.data(), 2 * (size_t)m * sizeof(double), hipMemcpyHostToDevice,
+                           ctx->stream));
+    double* acc = facc;
+    if (det) {
+        HIP_TRY(ctx->d_fx.alloc(2 * (size_t)m * nn));
+        acc = reinterpret_cast<double*>(ctx->d_fx.ptr);
+    }
+    const size_t nacc = (det ? 2 : 1) * (size_t)m * nn;
+    zero_kernel<<<std::min<size_t>((nacc + 255) / 256, 4096), 256, 0, ctx->stream>>>(acc, nacc);
+    HIP_TRY(hipGetLastError());
+    std::vector<std::unique_ptr<ClassSchedule>>* list = nullptr;
+    const int saved_variant = ctx->force_variant;
+    if (det) ctx->force_variant = VARIANT_UNTILED;
+    int rc = get_schedules(scr, ithreshold, NO_FP32, &list, true);
+    ctx->force_variant = saved_variant;
+    if (rc) return rc;
+    const bool single = m == 1 && !det;
+    const double fj = single ? h_fac[0] : 0.0, fk = single ? h_fac[1] : 0.0;
+    const int nd = single ? 1 : m;
+    const double* dfac = single ? nullptr : ctx->d_dfac.ptr;
+    rc = enqueue_classes(ctx, scr, *list, dens, acc, fj, fk, omega2, ithreshold, nullptr, 1, true, nullptr, nd, dfac,
+                         det);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+    for (int k = 0; k < NSIDE; k++) HIP_TRY(hipStreamWaitEvent(ctx->side[k], ctx->ev_start, 0));
+    rc = enqueue_classes(ctx, scr, *list, dens, acc, fj, fk, omega2, ithreshold, nullptr, 0, true, nullptr, nd, dfac,
+                         det);
+    const int rj = join_side_streams(ctx);
+    if (rc) return rc;
+    if (rj) return rj;
+    if (det) {
+        fx_to_double_kernel<<<std::min<size_t>((m * nn + 255) / 256, 4096), 256, 0, ctx->stream>>>(ctx->d_fx.ptr, facc,
+                                                                                                  m * nn);
+        HIP_TRY(hipGetLastError());
+    }
+    return VLX_OK;
+}
+
 // scratch = nullptr: F' and the max|D| slot live in context buffers (allocated on first use) and
 // missing schedules are built.  Otherwise F' is `scratch` (nao x nao doubles) and max|D| the
 // slot `scratch_dmax`, the schedules must exist already and nothing is allocated.
@@ -1863,6 +2009,19 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
     const size_t nn = (size_t)n * n;
     if (!no_alloc) HIP_TRY(ctx->d_faccum.alloc(nn));
     double* facc = no_alloc ? scratch : ctx->d_faccum.ptr;
+    dim3 sg((n + 255) / 256, n);
+    if (ctx->deterministic) {
+        // FP64, Schwarz selection only (the mixed-precision and density-weighted modes do not apply)
+        if (no_alloc) return fail(VLX_ERR_ARG, "the no-allocation build has no deterministic mode");
+        const double fac[2] = {fj, fk};
+        rc = multi_pass(ctx, scr, d_density, 1, fac, omega2, ithreshold, facc);
+        if (rc) return rc;
+        ctx->last_k32 = NO_FP32;
+        if (anti_coef == 0.0) symmetrize_kernel<<<sg, 256, 0, ctx->stream>>>(facc, d_fock_out, n);
+        else antisym_add_kernel<<<sg, 256, 0, ctx->stream>>>(facc, d_fock_out, n, anti_coef);
+        HIP_TRY(hipGetLastError());
+        return VLX_OK;
+    }
     zero_kernel<<<std::min<size_t>((nn + 255) / 256, 4096), 256, 0, ctx->stream>>>(facc, nn);
     HIP_TRY(hipGetLastError());
     // mixed precision: FP32 where Schwarz bound x max|D| < tau, i.e. for bin sums
@@ -1916,7 +2075,6 @@ int fock_build(vlx_context* ctx, vlx_screener* scr, const double* d_density, dou
     const int rj = join_side_streams(ctx);
     if (rc) return rc;
     if (rj) return rj;
-    dim3 sg((n + 255) / 256, n);
     if (anti_coef == 0.0) symmetrize_kernel<<<sg, 256, 0, ctx->stream>>>(facc, d_fock_out, n);
     else antisym_add_kernel<<<sg, 256, 0, ctx->stream>>>(facc, d_fock_out, n, anti_coef);
     HIP_TRY(hipGetLastError());
@@ -2116,6 +2274,7 @@ int build_batch_plan(vlx_context* ctx, BatchPlan& P)
                 L->tiled = sch.tiled;
                 L->singles = sch.singles;
                 L->mmd = sch.mmd;
+                L->ksl = sch.ksl;
                 L->tiled_waves = sch.tiled_waves;
             }
             L->tile_lds = std::max(L->tile_lds, sch.tile_lds);
@@ -2137,7 +2296,7 @@ int build_batch_plan(vlx_context* ctx, BatchPlan& P)
                         L->titems.push_back(ti);
                     }
             } else {
-                const bool ks = !sch.mmd && ce.variant == VLX_KET_STATIONARY;  // i1 is a bra index
+                const bool ks = !sch.mmd && (ce.variant == VLX_KET_STATIONARY || sch.ksl);  // i1 is a bra index
                 for (WorkItem wi : sch.items) {
                     wi.j0 += poff[KET][y];
                     wi.i0 += poff[BRA][x];
@@ -2166,7 +2325,7 @@ int build_batch_plan(vlx_context* ctx, BatchPlan& P)
             // the wavefronts' contiguous ranges of equal estimated cost, over the whole batch
             const int nsph = (2 * ce.la + 1) * (2 * ce.lb + 1) * (2 * ce.lc + 1) * (2 * ce.ld + 1);
             const double pcost = ce.vrr_flops + 60.0, qcost = ce.hrr_flops + 12.0 * nsph;
-            const bool ks = ce.variant == VLX_KET_STATIONARY;
+            const bool ks = ce.variant == VLX_KET_STATIONARY || L.ksl;
             std::vector<double> cost;
             cost.reserve(L.items.size());
             for (const auto& wi : L.items) {
@@ -2263,6 +2422,8 @@ int record_batch(vlx_context* ctx, BatchPlan& P)
             }
         } else if (L.mmd) {
             ce.mmd(a, static_cast<int>(std::min<long long>(a.sc.nitems, ctx->mmd_grid)), s);
+        } else if (L.ksl) {
+            ce.ksl(a, static_cast<int>(untiled_waves(a.sc.nitems, ctx->untiled_wave_cap)), s);
         } else {
             ce.fock(a, static_cast<int>(untiled_waves(a.sc.nitems, ctx->untiled_wave_cap) / VLX_WAVES_PER_BLOCK), s);
         }
@@ -2306,6 +2467,7 @@ int vlx_fock_compute_batch_device(vlx_context* ctx, int nmol, vlx_screener* cons
         return fail(VLX_ERR_ARG, "NULL argument to vlx_fock_compute_batch_device");
     if (nmol == 0) return VLX_OK;
     if (nmol > 65535) return fail(VLX_ERR_ARG, "%d molecules in one batch (at most 65535)", nmol);
+    if (ctx->deterministic) return fail(VLX_ERR_ARG, "batched builds have no deterministic mode");
     double fj = 0.0, fk = 0.0, omega2 = 0.0;
     for (int m = 0; m < nmol; m++) {
         const int rc = parse_build(ctx, scrs[m], d_densities[m], d_focks[m], label, exchange_factor, omega, ithreshold,
@@ -2447,6 +2609,113 @@ int vlx_fock_compute_general(vlx_context* ctx, vlx_screener* scr, const double* 
                                              ctx->d_fock.ptr);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(fock_out, ctx->d_fock.ptr, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VLX_OK;
+}
+
+// Multi-density build (src/t4c_func/FockDriver.cpp:173-229, CT4CMatricesDistributor): one pass over
+// the integrals digests every density with its own label; the integrals are erf(omega r)/r only
+// when every label is range-separated (CT4CMatricesDistributor::need_omega), else 1/r for all, the
+// *_rs labels then digesting like their Coulomb forms.  A general density contributes two
+// digested densities: its symmetric part with the label's (fj, fk) and its antisymmetric part with
+// (0, fk), G = F'_S + F'_S^T + F'_A - F'_A^T.  FP64 with the Schwarz selection (the context's
+// mixed-precision and density-weighted modes do not apply).
+int vlx_fock_compute_multi_device(vlx_context* ctx, vlx_screener* scr, int nmat, const double* d_densities,
+                                  const int* general, const char* const* labels, double exchange_factor, double omega,
+                                  int ithreshold, double* d_focks_out)
+{
+    if (!ctx || !scr || !d_densities || !labels || !d_focks_out || nmat < 1)
+        return fail(VLX_ERR_ARG, "bad argument to vlx_fock_compute_multi_device");
+    if (scr->ctx != ctx) return fail(VLX_ERR_ARG, "screener belongs to another context");
+    if (ithreshold > 30) return fail(VLX_ERR_ARG, "ithreshold %d outside the accepted range (< 0 or 0..30)", ithreshold);
+    std::vector<double> fj(nmat), fk(nmat);
+    bool all_rs = true;
+    for (int i = 0; i < nmat; i++) {
+        bool rs = false;
+        int rc = label_factors(labels[i], exchange_factor, &fj[i], &fk[i], &rs);
+        if (rc) return rc;
+        all_rs = all_rs && rs;
+    }
+    if (all_rs && !std::isfinite(omega)) return fail(VLX_ERR_ARG, "omega must be finite");
+    const double omega2 = all_rs ? omega * omega : 0.0;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int n = scr->nao;
+    const size_t nn = (size_t)n * n;
+    dim3 sg((n + 255) / 256, n);
+    // erf(0 r)/r = 0: every output is zero
+    if (all_rs && omega2 == 0.0) {
+        zero_kernel<<<std::min<size_t>((nmat * nn + 255) / 256, 4096), 256, 0, ctx->stream>>>(d_focks_out, nmat * nn);
+        HIP_TRY(hipGetLastError());
+        return VLX_OK;
+    }
+    // digested densities: c_sym[i] (and c_anti[i] for a general density with exchange)
+    std::vector<int> c_sym(nmat), c_anti(nmat, -1);
+    bool any_general = false;
+    int m = 0;
+    ctx->h_dfac.clear();
+    for (int i = 0; i < nmat; i++) {
+        c_sym[i] = m++;
+        ctx->h_dfac.push_back(fj[i]);
+        ctx->h_dfac.push_back(fk[i]);
+        if (general && general[i] && fk[i] != 0.0) {
+            c_anti[i] = m++;
+            ctx->h_dfac.push_back(0.0);
+            ctx->h_dfac.push_back(fk[i]);
+        }
+        any_general = any_general || (general && general[i]);
+    }
+    const double* dens = d_densities;
+    if (any_general) {
+        HIP_TRY(ctx->d_mdens.alloc((size_t)m * nn));
+        HIP_TRY(ctx->d_sym.alloc(nn));
+        for (int i = 0; i < nmat; i++) {
+            const double* di = d_densities + (size_t)i * nn;
+            double* ds = ctx->d_mdens.ptr + (size_t)c_sym[i] * nn;
+            if (general && general[i]) {
+                double* da = c_anti[i] >= 0 ? ctx->d_mdens.ptr + (size_t)c_anti[i] * nn : ctx->d_sym.ptr;
+                split_kernel<<<sg, 256, 0, ctx->stream>>>(di, ds, da, n);
+                HIP_TRY(hipGetLastError());
+            } else {
+                HIP_TRY(hipMemcpyAsync(ds, di, nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            }
+        }
+        dens = ctx->d_mdens.ptr;
+    }
+    HIP_TRY(ctx->d_mfacc.alloc((size_t)m * nn));
+    double* facc = ctx->d_mfacc.ptr;
+    const std::vector<double> fac = ctx->h_dfac;
+    int rc = multi_pass(ctx, scr, dens, m, fac.data(), omega2, ithreshold, facc);
+    if (rc) return rc;
+    for (int i = 0; i < nmat; i++) {
+        double* g = d_focks_out + (size_t)i * nn;
+        symmetrize_kernel<<<sg, 256, 0, ctx->stream>>>(facc + (size_t)c_sym[i] * nn, g, n);
+        if (c_anti[i] >= 0) antisym_add_kernel<<<sg, 256, 0, ctx->stream>>>(facc + (size_t)c_anti[i] * nn, g, n, 1.0);
+        HIP_TRY(hipGetLastError());
+    }
+    return VLX_OK;
+}
+
+int vlx_fock_compute_multi(vlx_context* ctx, vlx_screener* scr, int nmat, const double* densities, const int* general,
+                           const char* const* labels, double exchange_factor, double omega, int ithreshold,
+                           double* focks_out)
+{
+    if (!ctx || !scr || !densities || !labels || !focks_out || nmat < 1)
+        return fail(VLX_ERR_ARG, "bad argument to vlx_fock_compute_multi");
+    const int n = scr->nao;
+    const size_t nn = (size_t)n * n;
+    for (int i = 0; i < nmat; i++) {
+        if (general && general[i]) continue;
+        int rc = check_symmetric_host(densities + i * nn, n);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(ctx->d_density.alloc(nmat * nn));
+    HIP_TRY(ctx->d_fock.alloc(nmat * nn));
+    HIP_TRY(hipMemcpyAsync(ctx->d_density.ptr, densities, nmat * nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int rc = vlx_fock_compute_multi_device(ctx, scr, nmat, ctx->d_density.ptr, general, labels, exchange_factor, omega,
+                                           ithreshold, ctx->d_fock.ptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(focks_out, ctx->d_fock.ptr, nmat * nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return VLX_OK;
 }

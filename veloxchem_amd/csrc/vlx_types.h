@@ -80,6 +80,11 @@ struct FockScalars {
     double fj;                     // coefficient of the Coulomb terms (0: none)
     double fk;                     // coefficient of the exchange terms (0: none)
     double omega2;                 // omega^2 of the erf(omega r)/r operator (range-separated labels), 0: 1/r
+    int ndens;                     // densities digested per quartet (multi-density kernels; 0 or 1: one)
+    int det;                       // 1: deterministic fixed-point accumulation (multi-density and MD kernels)
+    int bs_layout;                 // multi-density kernel: 1 bra-stationary work items, 0 ket-stationary
+    int pad_sc;
+    long long dstride;             // elements between consecutive densities (and F' accumulators)
 };
 
 struct FockArgs {
@@ -104,6 +109,7 @@ struct FockArgs {
     const MolDesc* mols;           // per molecule: offset and size of its D and F'
     const int* glim;               // density-weighted screening: ngroups x ngroups largest bin sums per
                                    // pair of atom groups (from max|D| of the block), or nullptr (off)
+    const double* dfac;            // multi-density kernels: (fj, fk) of each density, 2 x ndens doubles
     const int* k32;                // per molecule: FP32 bin-sum threshold on the device, or nullptr
                                    // (tiled items: the FP64 kernel runs the item's bras with
                                    // bin + bin_lo < k32, the FP32 kernel the others)
@@ -158,6 +164,9 @@ struct ClassLaunchers {
     FockLaunchFn tiled;   // LDS-tiled Fock kernel, or nullptr
     FockLaunchFn tiled32; // the tiled kernel with FP32 recursions (mixed precision), or nullptr
     FockLaunchFn mmd;     // matrix-core McMurchie-Davidson Fock kernel (MD work items), or nullptr
+    FockLaunchFn fock_multi;   // multi-density form of `fock` (same work items)
+    FockLaunchFn tiled_multi;  // multi-density form of `tiled` (FP64), or nullptr
+    FockLaunchFn ksl;          // ket-stationary kernel with LDS accumulators (grid in wavefronts), or nullptr
 };
 
 struct DiagLauncher {
@@ -171,6 +180,10 @@ struct DiagLauncher {
     void launch_eri_##la##lb##lc##ld(const EriArgs& a, hipStream_t s);
 #define VLX_DECLARE_TILED(la, lb, lc, ld)                                          \
     void launch_tiled_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s); \
-    void launch_tiled32_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);
+    void launch_tiled32_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s); \
+    void launch_tiled_multi_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);
+#define VLX_DECLARE_MULTI(la, lb, lc, ld) \
+    void launch_fock_multi_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);
+#define VLX_DECLARE_KSL(la, lb, lc, ld) void launch_ksl_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);
 #define VLX_DECLARE_DIAG(la, lb) void launch_diag_##la##lb(const DiagArgs& a, hipStream_t s);
 #define VLX_DECLARE_MMD(la, lb, lc, ld) void launch_mmd_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);

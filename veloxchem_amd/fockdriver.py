@@ -6,7 +6,8 @@ Mirrors the Python facade src/pymodule/fockdriver.py:37-135 and the pybind11
 * ``compute(screener, D, label, exchange_factor, omega, ithreshold)`` ->
   ``Matrix`` (same angular layout and type as ``D``), the call
   scfdriver.py:1955-1957 makes with ``'2jk', 1.0, 0.0, thresh_int``;
-* ``compute(screener, Ds: Matrices, labels: list, ...)`` -> ``Matrices``;
+* ``compute(screener, Ds: Matrices, labels: list, ...)`` -> ``Matrices``,
+  one pass over the integrals for all densities (vlx_fock_compute_multi);
 * ``_compute_fock_omp(screener, D, ...)`` and the unscreened
   ``_compute_fock_omp(basis, molecule, D, label, xf, omega)``;
 * ``compute_eri(molecule, basis, eri_thresh)`` -> N^4 ndarray;
@@ -84,7 +85,7 @@ class FockDriver:
         self._block_size_factor = int(factor)
         _native.check(_native.lib().vlx_set_block_size_factor(_native.context(self._device), int(factor)))
 
-    VARIANTS = ('untiled', 'tiled4', 'tiled-wide', 'mmd', 'tiled4-p1', 'tiled-wide-p1')
+    VARIANTS = ('untiled', 'tiled4', 'tiled-wide', 'mmd', 'tiled4-p1', 'tiled-wide-p1', 'ks-lds')
 
     def _autotune(self, screener, density, ithreshold=12, repeats=2):
         """Times each angular class under every kernel variant on this density and keeps
@@ -223,32 +224,63 @@ class FockDriver:
             keys = density.keys()
             if len(keys) != len(label):
                 raise ValueError('number of densities and labels differ')
-            # one set of integrals serves every density: erf(omega r)/r only when every label is
-            # range-separated, else 1/r for all, the *_rs labels digesting like their Coulomb forms
-            # (CT4CMatricesDistributor::need_omega, src/t4c_func/T4CMatricesDistributor.cpp:55-66)
-            all_rs = all(str(lab).endswith('_rs') for lab in label)
-            out = Matrices()
-            for key, lab in zip(keys, label):
-                if not all_rs and str(lab).endswith('_rs'):
-                    lab = lab[:-3]
-                out.add(self._one(screener, rank, nodes, density.matrix(key), lab, xf, omega, ithreshold), key)
-            return out
+            return self._multi(screener, rank, nodes, density, keys, list(label), xf, omega, ithreshold)
         return self._one(screener, rank, nodes, density, label, xf, omega, ithreshold)
 
-    def _one(self, screener, rank, nodes, density, label, xf, omega, ithreshold):
+    def _check_call(self, screener, rank, nodes, ithreshold):
         if screener.is_restricted:
             raise ValueError('FockDriver: screeners of partition_atom / partition_atom_pair feed the geometric-'
                              'derivative drivers; the Fock build takes a partition() screener')
+        if ithreshold > 30:
+            raise ValueError(f'FockDriver: ithreshold {ithreshold} outside 0..30')
+        if nodes != 1 or rank != 0:
+            raise NotImplementedError('FockDriver: one process per GPU; rank/nodes partitioning is not built')
+
+    def _multi(self, screener, rank, nodes, mats, keys, labels, xf, omega, ithreshold):
+        """Matrices overload (src/t4c_func/FockDriver.cpp:173-229): one pass over the integrals
+        digests every density with its own label (vlx_fock_compute_multi).  The integrals are
+        erf(omega r)/r only when every label is range-separated, else 1/r for all, the *_rs labels
+        digesting like their Coulomb forms (CT4CMatricesDistributor::need_omega,
+        src/t4c_func/T4CMatricesDistributor.cpp:55-66)."""
+        self._check_call(screener, rank, nodes, ithreshold)
+        n = screener.number_of_ao()
+        dens, general, mats_in = [], [], []
+        for key, lab in zip(keys, labels):
+            if not isinstance(lab, str):
+                raise TypeError('label must be a string')
+            dm, mat = _dense(mats.matrix(key))
+            if dm.shape != (n, n):
+                raise ValueError(f'FockDriver: density has shape {dm.shape}, basis has {n} AOs')
+            dens.append(dm)
+            mats_in.append(mat)
+            general.append(1 if mat is not None and mat.get_type() == mat_t.general else 0)
+        k = len(dens)
+        d = np.ascontiguousarray(np.stack(dens).astype(np.float64, copy=False))
+        g = np.zeros_like(d)
+        flags = np.ascontiguousarray(np.array(general, dtype=np.int32))
+        labs = (ctypes.c_char_p * k)(*[lab.encode() for lab in labels])
+        _native.check(_native.lib().vlx_fock_compute_multi(
+            _native.context(self._device), screener.handle, k, d.ctypes.data_as(_native._c_double_p),
+            flags.ctypes.data_as(_native._c_int_p), labs, ctypes.c_double(xf), ctypes.c_double(omega),
+            int(ithreshold), g.ctypes.data_as(_native._c_double_p)))
+        out = Matrices()
+        for i, (key, mat) in enumerate(zip(keys, mats_in)):
+            if mat is not None:
+                res = Matrix(mat._block_dims, mat.get_type())
+                res._values = g[i].copy()
+            else:
+                res = g[i].copy()
+            out.add(res, key)
+        return out
+
+    def _one(self, screener, rank, nodes, density, label, xf, omega, ithreshold):
+        self._check_call(screener, rank, nodes, ithreshold)
         dm, mat = _dense(density)
         n = screener.number_of_ao()
         if dm.shape != (n, n):
             raise ValueError(f'FockDriver: density has shape {dm.shape}, basis has {n} AOs')
         if not isinstance(label, str):
             raise TypeError('label must be a string')
-        if ithreshold > 30:
-            raise ValueError(f'FockDriver: ithreshold {ithreshold} outside 0..30')
-        if nodes != 1 or rank != 0:
-            raise NotImplementedError('FockDriver: one process per GPU; rank/nodes partitioning is not built')
         dm = np.ascontiguousarray(dm)
         g = np.zeros((n, n))
         general = mat is not None and mat.get_type() == mat_t.general

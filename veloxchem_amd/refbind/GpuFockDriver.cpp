@@ -230,15 +230,42 @@ auto CGpuFockDriver::compute(const CT4CScreener& screener, const CMatrices& dens
 {
     const auto keys = densities.keys();
     if (keys.size() != labels.size()) throw std::runtime_error("CGpuFockDriver: one label per density");
-    // one set of integrals serves every density: erf(omega r)/r only when every label is
-    // range-separated, else 1/r for all, the *_rs labels then digesting like their Coulomb forms
-    // (CT4CMatricesDistributor::need_omega, src/t4c_func/T4CMatricesDistributor.cpp:55-66)
-    auto is_rs = [](const std::string& l) { return l.size() > 3 && l.compare(l.size() - 3, 3, "_rs") == 0; };
-    const bool all_rs = std::all_of(labels.begin(), labels.end(), is_rs);
+    if (keys.empty()) return CMatrices(densities);
+    vlx_screener* s = _screener_for(screener);
+    // one pass over the integrals digests every density (vlx_fock_compute_multi); erf(omega r)/r only
+    // when every label is range-separated (CT4CMatricesDistributor::need_omega,
+    // src/t4c_func/T4CMatricesDistributor.cpp:55-66), as the library applies it
+    const size_t k = keys.size();
+    size_t n = 0;
+    std::vector<double> dens, g;
+    std::vector<int> general(k);
+    std::vector<const char*> labs(k);
+    for (size_t i = 0; i < k; i++) {
+        const CMatrix* d = densities.matrix(keys[i]);
+        const auto full = d->full_matrix();
+        if (i == 0) {
+            n = full.number_of_rows();
+            dens.resize(k * n * n);
+        }
+        if (full.number_of_rows() != n || full.number_of_columns() != n)
+            throw std::runtime_error("CGpuFockDriver: densities of different sizes");
+        std::copy(full.data(), full.data() + n * n, dens.begin() + i * n * n);
+        general[i] = d->get_type() == mat_t::symmetric ? 0 : 1;
+        labs[i] = labels[i].c_str();
+    }
+    g.resize(k * n * n);
+    check(vlx_fock_compute_multi(_ctx, s, static_cast<int>(k), dens.data(), general.data(), labs.data(),
+                                 exchange_factor, omega, ithreshold, g.data()));
     CMatrices focks(densities);
-    for (size_t i = 0; i < keys.size(); i++) {
-        const std::string lab = (all_rs || !is_rs(labels[i])) ? labels[i] : labels[i].substr(0, labels[i].size() - 3);
-        *focks.matrix(keys[i]) = compute(screener, *densities.matrix(keys[i]), lab, exchange_factor, omega, ithreshold);
+    for (size_t i = 0; i < k; i++) {
+        CMatrix* fock = focks.matrix(keys[i]);
+        fock->zero();
+        for (const auto& key : fock->angular_pairs()) {
+            auto* sub = fock->sub_matrix(key);
+            const auto dims = sub->get_dimensions();
+            for (size_t r = 0; r < dims[2]; r++)
+                for (size_t c = 0; c < dims[3]; c++) sub->at({r, c}) = g[i * n * n + (r + dims[0]) * n + c + dims[1]];
+        }
     }
     return focks;
 }

@@ -142,14 +142,19 @@ class ScfRestrictedDriver:
         self.density_method = 'purification'
         # fock_2e(screener, D, thresh_int) -> G on D's device; default: the GPU build
         self.fock_2e = None
-        # 'fp64', or 'mixed': FP32 recursions where Schwarz bound x max|D| < mixed_threshold
+        # 'fp64', or 'mixed': FP32 recursions where Schwarz bound x max|D| < the smaller of
+        # mixed_threshold and mixed_gradient_factor x the last gradient norm (the FP32 errors of a
+        # build at threshold tau left a gradient floor of about 1.4 tau on a 2752-AO cluster)
         self.precision = 'fp64'
         self.mixed_threshold = 1.0e-8
+        self.mixed_gradient_factor = 0.05
         # incremental (difference-density) Fock builds: G_n = G_{n-1} + G(D_n - D_{n-1}) with the
         # screening threshold tightened by max|D_n - D_{n-1}|; a full build every full_fock_every
         # iterations, when the ERI threshold changes, and to confirm convergence
         self.incremental_fock = False
         self.full_fock_every = 8
+        # per-quartet error bound of an incremental build, relative to conv_thresh (see _comp_diis)
+        self.incremental_drop_factor = 1.0e-6
         # density-weighted screening of the builds (a fast mode, no reference counterpart): quartets
         # whose Schwarz bound x the largest |D| (|dD| in incremental builds) of the blocks they read
         # is below min(density_screening, 1e-6 x the last gradient norm) are skipped; 0 = off.  The
@@ -157,6 +162,10 @@ class ScfRestrictedDriver:
         # keeps converging; the converged density is confirmed by one unscreened full build, which
         # gives the reported energy.
         self.density_screening = 0.0
+        # deterministic reduction mode of the GPU builds (fixed-point integer accumulation): the same
+        # bits on every run; FP64 with Schwarz selection only, so precision and density_screening
+        # do not apply while it is on
+        self.deterministic = False
         self.density_screening_gradient_factor = 1.0e-6
         self.screener_factory = None
         # starting density: None for the reference's guesses (SAD, L2-DIIS valence stage), or an
@@ -211,6 +220,7 @@ class ScfRestrictedDriver:
             self.device = torch.device('cuda', dev)
             _native.set_precision(self.precision, self.mixed_threshold if self.precision == 'mixed' else 0.0, dev)
             _native.set_density_screening(self.density_screening, dev)
+            _native.set_deterministic(self.deterministic, dev)
         elif self.device is None:
             self.device = torch.device('cpu')
         t0 = time.perf_counter()
@@ -222,6 +232,7 @@ class ScfRestrictedDriver:
             if isinstance(self.fock_2e, _DeviceFock):
                 _native.set_precision('fp64', 0.0, self.fock_2e.device)
                 _native.set_density_screening(0.0, self.fock_2e.device)
+                _native.set_deterministic(False, self.fock_2e.device)
         self.timings['total'] = time.perf_counter() - t0
         res['timings'] = dict(self.timings)
         res['num_iter'] = self._num_iter
@@ -309,25 +320,46 @@ class ScfRestrictedDriver:
         fmax = 1.0 / math.sqrt(self.ovl_thresh)
         g_prev = d_prev = None
         full_thresh, since_full, confirm = None, 0, False
+        # set when the fast modes' error floor is found above conv_thresh (below): the rest of the
+        # SCF runs full, unscreened FP64 builds (else it would alternate between drifting
+        # incremental builds and failed confirmations)
+        full_only = False
+        best_grad, best_it = math.inf, 0
         for it in range(max_iter + 1):
             self._num_iter = it
             thresh = self.eri_thresh if e_grad is None else self._get_dyn_threshold(e_grad)
             thresh_int = int(-math.log10(thresh))
             t = time.perf_counter()
+            incremental = (self.incremental_fock and g_prev is not None and not confirm and not full_only and
+                           thresh_int == full_thresh and since_full < self.full_fock_every)
+            if self.precision == 'mixed' and isinstance(self.fock_2e, _DeviceFock):
+                # FP32 in full builds only: a difference-density build splits by max|dD|, so nearly all
+                # of it would run in FP32, and those errors would add up over the incremental builds
+                # (a gradient floor of 1.4e-6 on the 64-water def2-TZVP cluster)
+                tau_m = 0.0 if (full_only or incremental) else self.mixed_threshold
+                if e_grad is not None:
+                    tau_m = min(tau_m, self.mixed_gradient_factor * e_grad)
+                _native.set_precision('mixed' if tau_m > 0.0 else 'fp64', tau_m, self.fock_2e.device)
             if self.density_screening > 0.0 and isinstance(self.fock_2e, _DeviceFock) and not confirm:
-                tau_d = self.density_screening
+                tau_d = 0.0 if full_only else self.density_screening
                 if e_grad is not None:
                     tau_d = min(tau_d, self.density_screening_gradient_factor * e_grad)
                 _native.set_density_screening(tau_d, self.fock_2e.device)
-            incremental = (self.incremental_fock and g_prev is not None and not confirm and
-                           thresh_int == full_thresh and since_full < self.full_fock_every)
             build_thresh = thresh_int
             if incremental:
-                # quartets with Schwarz bound x max|dD| below what a full build drops are skipped:
-                # bin sums > thresh_int + ceil(log10 max|dD|)
+                # a quartet of bin sum s is bounded by 10^(1 - s) max|dD| once s > build_thresh.  The
+                # errors of skipped quartets add up over the incremental builds (unlike a full build's,
+                # which are the same every iteration), and their sum in the gradient norm measured
+                # about 1e5 times the per-quartet bound (64-water def2-TZVP), so the bound is kept at
+                # incremental_drop_factor x conv_thresh: s <= 1 + log10(max|dD| / (factor conv)), at
+                # most the full build's thresh_int
                 dd = (den - d_prev).contiguous()
                 dmax = float(dd.abs().max())
-                build_thresh = thresh_int + math.ceil(math.log10(dmax)) if dmax > 0.0 else -1
+                if dmax > 0.0:
+                    need = math.ceil(1.0 + math.log10(dmax / (self.incremental_drop_factor * conv_thresh)))
+                    build_thresh = min(thresh_int, max(need, 0))
+                else:
+                    build_thresh = -1
                 G = g_prev + self.fock_2e(screener, dd, build_thresh) if build_thresh >= 0 else g_prev.clone()
                 since_full += 1
             elif confirm and self.density_screening > 0.0 and isinstance(self.fock_2e, _DeviceFock):
@@ -357,11 +389,26 @@ class ScfRestrictedDriver:
             self._scf_energy = e_scf
             if it > 0 and e_grad < conv_thresh:
                 self.timings['linear_algebra'] += time.perf_counter() - t
-                if incremental or (self.density_screening > 0.0 and not confirm):
+                if incremental or (self.density_screening > 0.0 and not confirm and not full_only):
                     confirm = True  # the same density once more with a full (unscreened) build
                     continue
                 self._is_converged = True
                 break
+            # a confirming full build that found the SCF not converged after all, or a gradient that
+            # has not halved in four iterations while incremental builds, density screening or FP32
+            # recursions ran, means that their error floor is above conv_thresh: the rest of the SCF
+            # runs full, unscreened FP64 builds, and the DIIS history, whose Fock matrices carry
+            # those errors, starts afresh
+            fast = self.incremental_fock or self.density_screening > 0.0 or self.precision == 'mixed'
+            if e_grad < 0.5 * best_grad:
+                best_grad, best_it = e_grad, it
+            elif it - best_it >= 4 and fast and not full_only:
+                confirm = True
+            if confirm and not full_only:
+                full_only = True
+                focks.clear()
+                errs.clear()
+                bmat = np.zeros((0, 0))
             confirm = False
             if e_grad < self.diis_thresh:
                 if len(focks) == self.max_err_vecs:
