@@ -157,10 +157,11 @@ class ScfRestrictedDriver:
         self.incremental_drop_factor = 1.0e-6
         # density-weighted screening of the builds (a fast mode, no reference counterpart): quartets
         # whose Schwarz bound x the largest |D| (|dD| in incremental builds) of the blocks they read
-        # is below min(density_screening, 1e-6 x the last gradient norm) are skipped; 0 = off.  The
-        # skipped terms leave an error in G, so the threshold follows the gradient down and the SCF
-        # keeps converging; the converged density is confirmed by one unscreened full build, which
-        # gives the reported energy.
+        # is below min(density_screening, 1e-6 x the last gradient norm) are skipped (in incremental
+        # builds also below incremental_drop_factor x conv_thresh, as their errors add up); 0 = off.
+        # The skipped terms leave an error in G, so the threshold follows the gradient down and the
+        # SCF keeps converging; the converged density is confirmed by one unscreened full build,
+        # which gives the reported energy.
         self.density_screening = 0.0
         # deterministic reduction mode of the GPU builds (fixed-point integer accumulation): the same
         # bits on every run; FP64 with Schwarz selection only, so precision and density_screening
@@ -344,6 +345,10 @@ class ScfRestrictedDriver:
                 tau_d = 0.0 if full_only else self.density_screening
                 if e_grad is not None:
                     tau_d = min(tau_d, self.density_screening_gradient_factor * e_grad)
+                if incremental:
+                    # skipped difference-density terms add up over the incremental builds like the
+                    # threshold shift's (below): the same per-quartet budget
+                    tau_d = min(tau_d, self.incremental_drop_factor * conv_thresh)
                 _native.set_density_screening(tau_d, self.fock_2e.device)
             build_thresh = thresh_int
             if incremental:
