@@ -837,7 +837,10 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
 // accumulated with LDS atomics and flushed to HBM once per item.  The four
 // wavefronts share the kets and split the bras.  MULTI: the item's integrals
 // are digested into a.ndens densities (layout and factors as in
-// fock_kernel_multi), with one set of six tiles per density in LDS.
+// fock_kernel_multi), with one set of six tiles per density in LDS.  DLDS: the
+// six D blocks the item reads (D_AB, D_CD, D_AC, D_AD, D_BC, D_BD, shapes of the
+// F' tiles) are staged in LDS after the F' tiles once per item, and every lane
+// reads D from there instead of gathering it from HBM (single density only).
 __device__ __forceinline__ void flush_tile(double* __restrict__ F, const int n, const double* __restrict__ tile,
                                            const int nr, const int nc, const int* __restrict__ rows,
                                            const int* __restrict__ cols, const double f)
@@ -848,7 +851,7 @@ __device__ __forceinline__ void flush_tile(double* __restrict__ F, const int n, 
     }
 }
 
-template <int la, int lb, int lc, int ld, typename R, int NW, bool RS, bool MULTI = false>
+template <int la, int lb, int lc, int ld, typename R, int NW, bool RS, bool MULTI = false, bool DLDS = false>
 __global__ void __launch_bounds__(NW * VLX_WAVE)
     fock_kernel_tiled(const PairInfo* __restrict__ pairs_bra, const PrimPair* __restrict__ bra_prims,
                       const PairInfo* __restrict__ pairs_ket, const PrimPair* __restrict__ ket_prims,
@@ -915,6 +918,29 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
         double* t_kbd = t_kbc + nB * nC;
         const int ntile = nA * nB + nC * nD + nA * nC + nA * nD + nB * nC + nB * nD;
         for (int e = threadIdx.x; e < nds * ntile; e += blockDim.x) tiles[e] = 0.0;
+        const int* ra = grp_ao + grp_off[ti.ga * 4 + la];
+        const int* rb = grp_ao + grp_off[ti.gb * 4 + lb];
+        const int* rc = grp_ao + grp_off[ti.gc * 4 + lc];
+        const int* rd = grp_ao + grp_off[ti.gd * 4 + ld];
+        // D blocks in LDS (DLDS), laid out like the F' tiles: AB, CD, AC, AD, BC, BD
+        double* d_ab = tiles + ntile;
+        double* d_cd = d_ab + nA * nB;
+        double* d_ac = d_cd + nC * nD;
+        double* d_ad = d_ac + nA * nC;
+        double* d_bc = d_ad + nA * nD;
+        double* d_bd = d_bc + nB * nC;
+        if constexpr (DLDS) {
+            auto stage = [&](double* dst, const int* rows, const int nr, const int* cols, const int ncol) {
+                for (int x = threadIdx.x; x < nr * ncol; x += blockDim.x)
+                    dst[x] = D[(size_t)rows[x / ncol] * n + cols[x % ncol]];
+            };
+            stage(d_ab, ra, nA, rb, nB);
+            stage(d_cd, rc, nC, rd, nD);
+            stage(d_ac, ra, nA, rc, nC);
+            stage(d_ad, ra, nA, rd, nD);
+            stage(d_bc, rb, nB, rc, nC);
+            stage(d_bd, rb, nB, rd, nD);
+        }
         __syncthreads();
 
         // lane = a run of primitive pairs of one ket pair (first | count << 16):
@@ -941,7 +967,8 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
 #pragma unroll
             for (int k = 0; k < nc; k++)
 #pragma unroll
-                for (int l = 0; l < nd; l++) dcd[k][l] = R(D[(size_t)r[k] * n + t[l]]);
+                for (int l = 0; l < nd; l++)
+                    dcd[k][l] = DLDS ? R(d_cd[rl[k] * nD + tl[l]]) : R(D[(size_t)r[k] * n + t[l]]);
         }
         // J_CD depends on the lane's ket only: small ket blocks accumulate it in
         // registers over all bras of the item (one LDS atomic per element at the end)
@@ -1017,7 +1044,8 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                         for (int i = 0; i < na; i++)
 #pragma unroll
                             for (int jb = 0; jb < nb; jb++) {
-                                const R dab = R(Dd[(size_t)p[i] * n + q[jb]]);
+                                const R dab = DLDS ? R(d_ab[(bp.loc_a + i) * nB + bp.loc_b + jb])
+                                               : R(Dd[(size_t)p[i] * n + q[jb]]);
 #pragma unroll
                                 for (int k = 0; k < nc; k++)
 #pragma unroll
@@ -1049,7 +1077,9 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                             for (int jb = 0; jb < nb; jb++)
 #pragma unroll
                                 for (int l = 0; l < nd; l++)
-                                    v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(Dd[(size_t)q[jb] * n + t[l]]), v);
+                                    v = fma(I[((i * nb + jb) * nc + k) * nd + l],
+                                            DLDS ? R(d_bd[(bp.loc_b + jb) * nD + tl[l]]) : R(Dd[(size_t)q[jb] * n + t[l]]),
+                                            v);
                             unsafeAtomicAdd(&t_kac[(bp.loc_a + i) * nC + rl[k]], static_cast<double>(v));
                         }
 #pragma unroll
@@ -1059,7 +1089,9 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                             for (int jb = 0; jb < nb; jb++)
 #pragma unroll
                                 for (int k = 0; k < nc; k++)
-                                    v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(Dd[(size_t)q[jb] * n + r[k]]), v);
+                                    v = fma(I[((i * nb + jb) * nc + k) * nd + l],
+                                            DLDS ? R(d_bc[(bp.loc_b + jb) * nC + rl[k]]) : R(Dd[(size_t)q[jb] * n + r[k]]),
+                                            v);
                             unsafeAtomicAdd(&t_kad[(bp.loc_a + i) * nD + tl[l]], static_cast<double>(v));
                         }
                     }
@@ -1072,7 +1104,9 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                             for (int i = 0; i < na; i++)
 #pragma unroll
                                 for (int l = 0; l < nd; l++)
-                                    v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(Dd[(size_t)p[i] * n + t[l]]), v);
+                                    v = fma(I[((i * nb + jb) * nc + k) * nd + l],
+                                            DLDS ? R(d_ad[(bp.loc_a + i) * nD + tl[l]]) : R(Dd[(size_t)p[i] * n + t[l]]),
+                                            v);
                             unsafeAtomicAdd(&t_kbc[(bp.loc_b + jb) * nC + rl[k]], static_cast<double>(v));
                         }
 #pragma unroll
@@ -1082,7 +1116,9 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                             for (int i = 0; i < na; i++)
 #pragma unroll
                                 for (int k = 0; k < nc; k++)
-                                    v = fma(I[((i * nb + jb) * nc + k) * nd + l], R(Dd[(size_t)p[i] * n + r[k]]), v);
+                                    v = fma(I[((i * nb + jb) * nc + k) * nd + l],
+                                            DLDS ? R(d_ac[(bp.loc_a + i) * nC + rl[k]]) : R(Dd[(size_t)p[i] * n + r[k]]),
+                                            v);
                             unsafeAtomicAdd(&t_kbd[(bp.loc_b + jb) * nD + tl[l]], static_cast<double>(v));
                         }
                     }
@@ -1099,10 +1135,6 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
             }
         }
         __syncthreads();
-        const int* ra = grp_ao + grp_off[ti.ga * 4 + la];
-        const int* rb = grp_ao + grp_off[ti.gb * 4 + lb];
-        const int* rc = grp_ao + grp_off[ti.gc * 4 + lc];
-        const int* rd = grp_ao + grp_off[ti.gd * 4 + ld];
         for (int dd = 0; dd < nds; dd++) {
             double* __restrict__ Fd = MULTI ? F + dd * a.dstride : F;
             const double fj = MULTI ? dfac[2 * dd] : a.fj, fk = MULTI ? dfac[2 * dd + 1] : a.fk;
@@ -1196,8 +1228,8 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
 // FP64 launcher picks the erf(omega r)/r instantiation when omega2 > 0; the
 // FP32 launcher has the Coulomb one only (range-separated builds run no FP32
 // items).
-#define VLX_TILED_LAUNCH(la, lb, lc, ld, R, NW, RS, tab, MULTI)                                         \
-    hipLaunchKernelGGL((fock_kernel_tiled<la, lb, lc, ld, R, NW, RS, MULTI>), dim3(grid),                 \
+#define VLX_TILED_LAUNCH(la, lb, lc, ld, R, NW, RS, tab, MULTI, DLDS)                                   \
+    hipLaunchKernelGGL((fock_kernel_tiled<la, lb, lc, ld, R, NW, RS, MULTI, DLDS>), dim3(grid),           \
                        dim3((NW) * VLX_WAVE), a.tile_lds_bytes, s, a.bra, a.bra_prims, a.ket, a.ket_prims,   \
                        a.titems, a.kentries, tab, a.D, a.F, a.mols, a.k32, a.glim, a.grp_off, a.grp_n,      \
                        a.grp_ao, a.dfac, a.sc)
@@ -1206,22 +1238,27 @@ __global__ void __launch_bounds__(VLX_BLOCK) diag_kernel(const DiagArgs a)
     {                                                                                                   \
         const bool w4 = a.tiled_waves == 4 || (NWB) == 4;                                               \
         if (a.sc.omega2 > 0.0) {                                                                        \
-            if (w4) VLX_TILED_LAUNCH(la, lb, lc, ld, double, 4, true, a.boys_tab, false);               \
-            else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, true, a.boys_tab, false);                \
+            if (w4) VLX_TILED_LAUNCH(la, lb, lc, ld, double, 4, true, a.boys_tab, false, false);               \
+            else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, true, a.boys_tab, false, false);                \
         } else {                                                                                        \
-            if (w4) VLX_TILED_LAUNCH(la, lb, lc, ld, double, 4, false, a.boys_tab, false);              \
-            else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, false, a.boys_tab, false);               \
+            if (w4) VLX_TILED_LAUNCH(la, lb, lc, ld, double, 4, false, a.boys_tab, false, false);              \
+            else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, false, a.boys_tab, false, false);               \
         }                                                                                               \
     }                                                                                                   \
     void launch_tiled32_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                    \
     {                                                                                                   \
-        if (a.tiled_waves == 4 || (NWB) == 4) VLX_TILED_LAUNCH(la, lb, lc, ld, float, 4, false, a.boys_tab_f, false); \
-        else VLX_TILED_LAUNCH(la, lb, lc, ld, float, NWB, false, a.boys_tab_f, false);                  \
+        if (a.tiled_waves == 4 || (NWB) == 4) VLX_TILED_LAUNCH(la, lb, lc, ld, float, 4, false, a.boys_tab_f, false, false); \
+        else VLX_TILED_LAUNCH(la, lb, lc, ld, float, NWB, false, a.boys_tab_f, false, false);                  \
     }                                                                                                   \
     void launch_tiled_multi_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                \
     {                                                                                                   \
-        if (a.sc.omega2 > 0.0) VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, true, a.boys_tab, true);   \
-        else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, false, a.boys_tab, true);                    \
+        if (a.sc.omega2 > 0.0) VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, true, a.boys_tab, true, false);   \
+        else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, false, a.boys_tab, true, false);                    \
+    }                                                                                                   \
+    void launch_tiled_dlds_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s)                 \
+    {                                                                                                   \
+        if (a.sc.omega2 > 0.0) VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, true, a.boys_tab, false, true);   \
+        else VLX_TILED_LAUNCH(la, lb, lc, ld, double, NWB, false, a.boys_tab, false, true);                    \
     }
 
 // multi-density form of a class's untiled kernels (work-item layout in FockScalars::bs_layout)

@@ -404,8 +404,9 @@ def main(outdir, lmax):
         fm = f'&launch_fock_{la}{lb}{lc}{ld}' if var == 'md' else f'&launch_fock_multi_{la}{lb}{lc}{ld}'
         tlm = f'&launch_tiled_multi_{la}{lb}{lc}{ld}' if has_tiled else 'nullptr'
         ksl = f'&launch_ksl_{la}{lb}{lc}{ld}' if ksl_waves(la, lb, lc, ld, var) else 'nullptr'
+        tld = f'&launch_tiled_dlds_{la}{lb}{lc}{ld}' if has_tiled else 'nullptr'
         d.append(f'    {{{x}, {y}, {la}, {lb}, {lc}, {ld}, {nv}, {nh}, {ks}, &launch_fock_{la}{lb}{lc}{ld}, '
-                 f'&launch_eri_{la}{lb}{lc}{ld}, {tl}, {tl32}, {mm}, {fm}, {tlm}, {ksl}}},')
+                 f'&launch_eri_{la}{lb}{lc}{ld}, {tl}, {tl32}, {mm}, {fm}, {tlm}, {ksl}, {tld}}},')
     d.append('};')
     d.append('static const DiagLauncher g_diag_table[] = {')
     for (la, lb) in pcs:

@@ -167,6 +167,7 @@ struct ClassSchedule {
     bool singles = false; // tiled over single-primitive ket entries
     bool mmd = false;     // matrix-core MD kernel (MD work items)
     bool ksl = false;     // ket-stationary kernel with LDS accumulators (ket-stationary work items)
+    bool dlds = false;    // tiled, the item's D blocks staged in LDS (twice the tile LDS)
     int tiled_waves = 0;  // 4: 4-wave blocks; 0: the class's wide block
     int tile_lds = 0;  // bytes of dynamic LDS (tiled)
     std::vector<TileItem> titems;
@@ -253,11 +254,13 @@ enum {
     VARIANT_TILED4_P1 = 4,
     VARIANT_TILED_WIDE_P1 = 5,
     VARIANT_KSL = 6,  // ket-stationary items with lane-private LDS accumulators (bra-stationary classes)
-    NVARIANTS = 7
+    VARIANT_TILED_DLDS = 7,  // tiled wide block with the item's D blocks staged in LDS
+    NVARIANTS = 8
 };
 bool variant_tiled(int v)
 {
-    return v == VARIANT_TILED4 || v == VARIANT_TILED_WIDE || v == VARIANT_TILED4_P1 || v == VARIANT_TILED_WIDE_P1;
+    return v == VARIANT_TILED4 || v == VARIANT_TILED_WIDE || v == VARIANT_TILED4_P1 || v == VARIANT_TILED_WIDE_P1 ||
+           v == VARIANT_TILED_DLDS;
 }
 
 struct vlx_context {
@@ -806,10 +809,15 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
         if (variant_tiled(var) && !can_tile) var = VARIANT_UNTILED;
         if (var == VARIANT_MMD && !ce.mmd) var = VARIANT_UNTILED;
         if (var == VARIANT_KSL && !ce.ksl) var = VARIANT_UNTILED;
+        // D tiles double the LDS of the F' tiles: beside the FP64 Boys table, or the plain wide tiles
+        if (var == VARIANT_TILED_DLDS &&
+            (!ce.tiled_dlds || 2 * tile_lds > 160 * 1024 - static_cast<int>(BOYS_TAB_SIZE * sizeof(double)) - 1024))
+            var = VARIANT_TILED_WIDE;
         if (ce.variant == VLX_WAVE_MD) var = VARIANT_MMD;  // no generated kernel
         sch->variant = var;
         sch->mmd = var == VARIANT_MMD;
         sch->ksl = var == VARIANT_KSL;
+        sch->dlds = var == VARIANT_TILED_DLDS;
         if (variant_tiled(var)) {
             // tiled items: (bra segment prefix, chunk of <= 64 kets of a ket segment)
             const PairSet& tb = unscreened ? s->all_tiled_bra[ce.x] : s->kept_tiled_bra[ce.x];
@@ -1735,8 +1743,9 @@ int vlx_fock_class_kernel(vlx_context* ctx, vlx_screener* scr, int ithreshold, i
         if (sp->tiled) {
             const int L = la + lb + lc + ld;
             const int wide = (L <= 2) ? 16 : (L == 3 ? 8 : 4);
-            std::snprintf(name, len, "fock_kernel_tiled<%d,%d,%d,%d,double,%d,false>", la, lb, lc, ld,
-                          sp->tiled_waves == 4 ? 4 : wide);
+            // template arguments as the profiler prints them: <..., R, NW, RS, MULTI, DLDS>
+            std::snprintf(name, len, "fock_kernel_tiled<%d,%d,%d,%d,double,%d,false,false,%s>", la, lb, lc, ld,
+                          sp->tiled_waves == 4 ? 4 : wide, sp->dlds ? "true" : "false");
         } else {
             const char* k = sp->mmd                             ? "fock_kernel_mmd"
                             : sp->ksl                           ? "fock_kernel_ksl"
@@ -1911,7 +1920,15 @@ int enqueue_classes(vlx_context* ctx, vlx_screener* scr, const std::vector<std::
                 ce.fock_multi(a, grid, st);
             }
         } else if (sch.tiled) {
-            if (!sch.titems.empty()) ce.tiled(a, grid, st);
+            if (!sch.titems.empty()) {
+                if (sch.dlds) {
+                    FockArgs ad = a;
+                    ad.tile_lds_bytes = 2 * sch.tile_lds;
+                    ce.tiled_dlds(ad, grid, st);
+                } else {
+                    ce.tiled(a, grid, st);
+                }
+            }
             if (!sch.titems32.empty()) {
                 FockArgs a32 = a;
                 a32.titems = sch.d_titems32.ptr;

@@ -167,6 +167,7 @@ struct ClassLaunchers {
     FockLaunchFn fock_multi;   // multi-density form of `fock` (same work items)
     FockLaunchFn tiled_multi;  // multi-density form of `tiled` (FP64), or nullptr
     FockLaunchFn ksl;          // ket-stationary kernel with LDS accumulators (grid in wavefronts), or nullptr
+    FockLaunchFn tiled_dlds;   // `tiled` (FP64, wide block) with the item's D blocks staged in LDS, or nullptr
 };
 
 struct DiagLauncher {
@@ -181,7 +182,8 @@ struct DiagLauncher {
 #define VLX_DECLARE_TILED(la, lb, lc, ld)                                          \
     void launch_tiled_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s); \
     void launch_tiled32_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s); \
-    void launch_tiled_multi_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);
+    void launch_tiled_multi_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s); \
+    void launch_tiled_dlds_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);
 #define VLX_DECLARE_MULTI(la, lb, lc, ld) \
     void launch_fock_multi_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);
 #define VLX_DECLARE_KSL(la, lb, lc, ld) void launch_ksl_##la##lb##lc##ld(const FockArgs& a, int grid, hipStream_t s);
