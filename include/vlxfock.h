@@ -270,7 +270,8 @@ int vlx_fock_schedule_stats(vlx_screener* scr, int ithreshold, int max_records, 
  * la+lb+lc+ld >= 4, 4 and 5 = variants 1 and 2 with one ket primitive per lane
  * instead of runs of up to three, 6 = ket-stationary work items with lane-private
  * LDS accumulators for the bra-stationary classes, 7 = variant 2 with the item's
- * D blocks staged in LDS), `repeats` builds each on a
+ * D blocks staged in LDS, 8 = variant 2 with chunks spanning two ket segments),
+ * `repeats` builds each on a
  * single stream, and keeps
  * the fastest variant per class for later builds with this screener (and as
  * the default of screeners created later on this context).

@@ -897,7 +897,9 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
         // groups, so the item has one bin-sum limit; bras past it (sorted by bin) are cut off
         int ilim = a.ithreshold;
         if (glim != nullptr) {
-            ilim = min(ilim, quartet_bin_limit(glim, a.ngroups, ti.ga, ti.gb, ti.gc, ti.gd));
+            int ql = quartet_bin_limit(glim, a.ngroups, ti.ga, ti.gb, ti.gc, ti.gd);
+            if (ti.gd2 >= 0) ql = max(ql, quartet_bin_limit(glim, a.ngroups, ti.ga, ti.gb, ti.gc, ti.gd2));
+            ilim = min(ilim, ql);
             const int bl = ilim - ti.bin_lo;  // a bra needs bin <= bl for one quartet with the item's kets
             int lo = b0, hi = b1;
             while (lo < hi) {
@@ -909,37 +911,38 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
             if (b0 >= b1) continue;  // block-uniform
         }
         const int nA = grp_n[ti.ga * 4 + la], nB = grp_n[ti.gb * 4 + lb];
-        const int nC = grp_n[ti.gc * 4 + lc], nD = grp_n[ti.gd * 4 + ld];
-        double* t_jab = tiles;
-        double* t_jcd = t_jab + nA * nB;
-        double* t_kac = t_jcd + nC * nD;
-        double* t_kad = t_kac + nA * nC;
-        double* t_kbc = t_kad + nA * nD;
-        double* t_kbd = t_kbc + nB * nC;
-        const int ntile = nA * nB + nC * nD + nA * nC + nA * nD + nB * nC + nB * nD;
+        const int nC = grp_n[ti.gc * 4 + lc];
+        // D groups: gd, and gd2 for the entries [jm, j1) of a chunk spanning two ket segments
+        const bool two = ti.gd2 >= 0;
+        const int nD0 = grp_n[ti.gd * 4 + ld], nD1 = two ? grp_n[ti.gd2 * 4 + ld] : 0;
+        // tiles of one density: J_AB, K_AC, K_BC, then per D group J_CD, K_AD, K_BD
+        const int nshared = nA * nB + nA * nC + nB * nC;
+        const int nseg0 = (nC + nA + nB) * nD0;
+        const int ntile = nshared + nseg0 + (nC + nA + nB) * nD1;
         for (int e = threadIdx.x; e < nds * ntile; e += blockDim.x) tiles[e] = 0.0;
         const int* ra = grp_ao + grp_off[ti.ga * 4 + la];
         const int* rb = grp_ao + grp_off[ti.gb * 4 + lb];
         const int* rc = grp_ao + grp_off[ti.gc * 4 + lc];
-        const int* rd = grp_ao + grp_off[ti.gd * 4 + ld];
-        // D blocks in LDS (DLDS), laid out like the F' tiles: AB, CD, AC, AD, BC, BD
-        double* d_ab = tiles + ntile;
-        double* d_cd = d_ab + nA * nB;
-        double* d_ac = d_cd + nC * nD;
-        double* d_ad = d_ac + nA * nC;
-        double* d_bc = d_ad + nA * nD;
-        double* d_bd = d_bc + nB * nC;
+        const int* rd0 = grp_ao + grp_off[ti.gd * 4 + ld];
+        const int* rd1 = two ? grp_ao + grp_off[ti.gd2 * 4 + ld] : rd0;
+        // D blocks in LDS (DLDS) after the F' tiles, in the same layout
+        double* dtiles = tiles + ntile;
         if constexpr (DLDS) {
             auto stage = [&](double* dst, const int* rows, const int nr, const int* cols, const int ncol) {
                 for (int x = threadIdx.x; x < nr * ncol; x += blockDim.x)
                     dst[x] = D[(size_t)rows[x / ncol] * n + cols[x % ncol]];
             };
-            stage(d_ab, ra, nA, rb, nB);
-            stage(d_cd, rc, nC, rd, nD);
-            stage(d_ac, ra, nA, rc, nC);
-            stage(d_ad, ra, nA, rd, nD);
-            stage(d_bc, rb, nB, rc, nC);
-            stage(d_bd, rb, nB, rd, nD);
+            stage(dtiles, ra, nA, rb, nB);
+            stage(dtiles + nA * nB, ra, nA, rc, nC);
+            stage(dtiles + nA * nB + nA * nC, rb, nB, rc, nC);
+            for (int sgi = 0; sgi < (two ? 2 : 1); sgi++) {
+                const int nDs = sgi ? nD1 : nD0;
+                const int* rds = sgi ? rd1 : rd0;
+                double* base = dtiles + nshared + (sgi ? nseg0 : 0);
+                stage(base, rc, nC, rds, nDs);
+                stage(base + nC * nDs, ra, nA, rds, nDs);
+                stage(base + nC * nDs + nA * nDs, rb, nB, rds, nDs);
+            }
         }
         __syncthreads();
 
@@ -948,9 +951,20 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
         // step after the VRR is linear)
         const int e = ti.j0 + lane;
         const bool lane_ok = e < ti.j1;
-        const int2 ke = kentries[lane_ok ? e : ti.j0];
+        const int esel = lane_ok ? e : ti.j0;
+        const int2 ke = kentries[esel];
         const int kq0 = ke.y & 0xffff, kqn = ke.y >> 16;
         const PairInfo kp = pairs_ket[ke.x];
+        // the lane's D group: the tiles J_CD, K_AD, K_BD (and D_CD, D_AD, D_BD) of its segment
+        const int sg = (two && esel >= ti.jm) ? 1 : 0;
+        const int nD = sg ? nD1 : nD0;
+        const int dofs = nshared + (sg ? nseg0 : 0);
+        const double* d_ab = dtiles;
+        const double* d_ac = d_ab + nA * nB;
+        const double* d_bc = d_ac + nA * nC;
+        const double* d_cd = dtiles + dofs;
+        const double* d_ad = d_cd + nC * nD;
+        const double* d_bd = d_ad + nA * nD;
         int r[nc], t[nd], rl[nc], tl[nd];
 #pragma unroll
         for (int k = 0; k < nc; k++) {
@@ -1005,11 +1019,11 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
             for (int dd = 0; dd < nds; dd++) {
                 const double* __restrict__ Dd = MULTI ? D + dd * a.dstride : D;
                 double* t_jab = tiles + dd * ntile;
-                double* t_jcd = t_jab + nA * nB;
-                double* t_kac = t_jcd + nC * nD;
-                double* t_kad = t_kac + nA * nC;
-                double* t_kbc = t_kad + nA * nD;
-                double* t_kbd = t_kbc + nB * nC;
+                double* t_kac = t_jab + nA * nB;
+                double* t_kbc = t_kac + nA * nC;
+                double* t_jcd = tiles + dd * ntile + dofs;
+                double* t_kad = t_jcd + nC * nD;
+                double* t_kbd = t_kad + nA * nD;
                 const bool do_j = (MULTI ? dfac[2 * dd] : a.fj) != 0.0;
                 const bool do_k = (MULTI ? dfac[2 * dd + 1] : a.fk) != 0.0;
                 if constexpr (MULTI) {
@@ -1131,7 +1145,7 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
                 for (int k = 0; k < nc; k++)
 #pragma unroll
                     for (int l = 0; l < nd; l++)
-                        if (jcd[k][l] != 0.0) unsafeAtomicAdd(&t_jcd[rl[k] * nD + tl[l]], jcd[k][l]);
+                        if (jcd[k][l] != 0.0) unsafeAtomicAdd(&tiles[dofs + rl[k] * nD + tl[l]], jcd[k][l]);
             }
         }
         __syncthreads();
@@ -1139,20 +1153,24 @@ __global__ void __launch_bounds__(NW * VLX_WAVE)
             double* __restrict__ Fd = MULTI ? F + dd * a.dstride : F;
             const double fj = MULTI ? dfac[2 * dd] : a.fj, fk = MULTI ? dfac[2 * dd + 1] : a.fk;
             const double* t_jab = tiles + dd * ntile;
-            const double* t_jcd = t_jab + nA * nB;
-            const double* t_kac = t_jcd + nC * nD;
-            const double* t_kad = t_kac + nA * nC;
-            const double* t_kbc = t_kad + nA * nD;
-            const double* t_kbd = t_kbc + nB * nC;
-            if (fj != 0.0) {
-                flush_tile(Fd, n, t_jab, nA, nB, ra, rb, fj);
-                flush_tile(Fd, n, t_jcd, nC, nD, rc, rd, fj);
-            }
+            const double* t_kac = t_jab + nA * nB;
+            const double* t_kbc = t_kac + nA * nC;
+            if (fj != 0.0) flush_tile(Fd, n, t_jab, nA, nB, ra, rb, fj);
             if (fk != 0.0) {
                 flush_tile(Fd, n, t_kac, nA, nC, ra, rc, fk);
-                flush_tile(Fd, n, t_kad, nA, nD, ra, rd, fk);
                 flush_tile(Fd, n, t_kbc, nB, nC, rb, rc, fk);
-                flush_tile(Fd, n, t_kbd, nB, nD, rb, rd, fk);
+            }
+            for (int sgi = 0; sgi < (two ? 2 : 1); sgi++) {
+                const int nDs = sgi ? nD1 : nD0;
+                const int* rds = sgi ? rd1 : rd0;
+                const double* t_jcd = tiles + dd * ntile + nshared + (sgi ? nseg0 : 0);
+                const double* t_kad = t_jcd + nC * nDs;
+                const double* t_kbd = t_kad + nA * nDs;
+                if (fj != 0.0) flush_tile(Fd, n, t_jcd, nC, nDs, rc, rds, fj);
+                if (fk != 0.0) {
+                    flush_tile(Fd, n, t_kad, nA, nDs, ra, rds, fk);
+                    flush_tile(Fd, n, t_kbd, nB, nDs, rb, rds, fk);
+                }
             }
         }
         __syncthreads();

@@ -255,12 +255,13 @@ enum {
     VARIANT_TILED_WIDE_P1 = 5,
     VARIANT_KSL = 6,  // ket-stationary items with lane-private LDS accumulators (bra-stationary classes)
     VARIANT_TILED_DLDS = 7,  // tiled wide block with the item's D blocks staged in LDS
-    NVARIANTS = 8
+    VARIANT_TILED_2SEG = 8,  // tiled wide block, chunks spanning two ket segments of one C group
+    NVARIANTS = 9
 };
 bool variant_tiled(int v)
 {
     return v == VARIANT_TILED4 || v == VARIANT_TILED_WIDE || v == VARIANT_TILED4_P1 || v == VARIANT_TILED_WIDE_P1 ||
-           v == VARIANT_TILED_DLDS;
+           v == VARIANT_TILED_DLDS || v == VARIANT_TILED_2SEG;
 }
 
 struct vlx_context {
@@ -813,6 +814,10 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
         if (var == VARIANT_TILED_DLDS &&
             (!ce.tiled_dlds || 2 * tile_lds > 160 * 1024 - static_cast<int>(BOYS_TAB_SIZE * sizeof(double)) - 1024))
             var = VARIANT_TILED_WIDE;
+        // two D groups per chunk: the J_CD, K_AD and K_BD tiles twice
+        const int tile_lds2 = 8 * (gm[ce.la] * gm[ce.lb] + gm[ce.la] * gm[ce.lc] + gm[ce.lb] * gm[ce.lc] +
+                                   2 * (gm[ce.lc] + gm[ce.la] + gm[ce.lb]) * gm[ce.ld]);
+        if (var == VARIANT_TILED_2SEG && tile_lds2 > s->ctx->max_tile_lds) var = VARIANT_TILED_WIDE;
         if (ce.variant == VLX_WAVE_MD) var = VARIANT_MMD;  // no generated kernel
         sch->variant = var;
         sch->mmd = var == VARIANT_MMD;
@@ -826,11 +831,36 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
             sch->singles = var == VARIANT_TILED4_P1 || var == VARIANT_TILED_WIDE_P1;
             const PairSet::Entries& en = sch->singles ? tk.singles : tk.runs;
             sch->tiled_waves = (var == VARIANT_TILED4 || var == VARIANT_TILED4_P1) ? 4 : 0;
-            sch->tile_lds = tile_lds;
+            sch->tile_lds = var == VARIANT_TILED_2SEG ? tile_lds2 : tile_lds;
+            // chunks of <= 64 lane entries of one ket segment; with VARIANT_TILED_2SEG the last, partly
+            // filled chunk of a segment takes the first entries of the next segment when both have the
+            // same C group (their entry ranges are adjacent): {j0, j1, first segment, gd2, jm}
+            std::vector<std::array<int, 5>> chunks;
+            int open = -1;  // a one-segment chunk with room left
             for (size_t sk = 0; sk < tk.segs.size(); sk++) {
+                int j = en.seg_e0[sk];
+                const int e1 = en.seg_e1[sk];
+                if (var == VARIANT_TILED_2SEG && open >= 0 && tk.segs[chunks[open][2]].ga == tk.segs[sk].ga &&
+                    chunks[open][1] == j) {
+                    auto& c = chunks[open];
+                    const int take = std::min(64 - (c[1] - c[0]), e1 - j);
+                    c[3] = tk.segs[sk].gb;
+                    c[4] = j;
+                    c[1] = j + take;
+                    j += take;
+                }
+                open = -1;
+                for (; j < e1; j += 64) {
+                    const int j1 = std::min(e1, j + 64);
+                    chunks.push_back({j, j1, static_cast<int>(sk), -1, j1});
+                    if (j1 - j < 64) open = static_cast<int>(chunks.size()) - 1;
+                }
+            }
+            for (const auto& ch : chunks) {
+                const size_t sk = static_cast<size_t>(ch[2]);
                 const auto& ks = tk.segs[sk];
-                for (int j0 = en.seg_e0[sk]; j0 < en.seg_e1[sk]; j0 += 64) {
-                    const int j1 = std::min(en.seg_e1[sk], j0 + 64);
+                const int j0 = ch[0], j1 = ch[1], gd2 = ch[3], jm = ch[4];
+                {
                     int bin_lo = NBINS, kmin = en.kentries[j0].x;
                     for (int j = j0; j < j1; j++) {
                         bin_lo = std::min(bin_lo, std::max(tk.pairs[en.kentries[j].x].bin, 0));
@@ -838,7 +868,7 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
                     }
                     const int lim = unscreened ? NBINS - 1 : ithr - bin_lo;
                     if (lim < 0) continue;
-                    // same class: bra segments at or after the ket segment (segment order is
+                    // same class: bra segments at or after the (first) ket segment (segment order is
                     // identical in both lists); the diagonal segment is masked by rank
                     for (size_t sb = same ? sk : 0; sb < tb.segs.size(); sb++) {
                         const auto& bs = tb.segs[sb];
@@ -856,9 +886,9 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
                             im = (b < 0) ? i0 : std::min(i1, bs.start + bs.cnt_le[std::min(b, NBINS - 1)]);
                             im = std::max(im, i0);
                         }
-                        if (im > i0) sch->titems.push_back({i0, im, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb, 0, bin_lo, 0, 0});
+                        if (im > i0) sch->titems.push_back({i0, im, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb, 0, bin_lo, gd2, jm});
                         if (i1 > im)
-                            sch->titems32.push_back({im, i1, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb, 0, bin_lo, 0, 0});
+                            sch->titems32.push_back({im, i1, j0, j1, bs.ga, bs.gb, ks.ga, ks.gb, 0, bin_lo, gd2, jm});
                     }
                 }
             }
@@ -2309,6 +2339,8 @@ int build_batch_plan(vlx_context* ctx, BatchPlan& P)
                         ti.gb += goff[m];
                         ti.gc += goff[m];
                         ti.gd += goff[m];
+                        if (ti.gd2 >= 0) ti.gd2 += goff[m];
+                        ti.jm += eo;
                         ti.mol = m;
                         L->titems.push_back(ti);
                     }

@@ -47,10 +47,12 @@ struct alignas(16) PairInfo {
 // primitive entries [j0, j1) (at most 64, one per lane) of atom groups (gc, gd),
 // of molecule `mol` of the launch (MolDesc).  bin_lo is the smallest Schwarz
 // bin of the item's kets (the precision split on the device, FockArgs::k32).
+// A chunk may span two ket segments with the same C group: entries [jm, j1)
+// then belong to D group gd2 (gd2 = -1 and jm = j1 for one segment).
 struct TileItem {
     int i0, i1, j0, j1;
     int ga, gb, gc, gd;
-    int mol, bin_lo, pad0, pad1;
+    int mol, bin_lo, gd2, jm;
 };
 
 // One unit of Fock work: kets [j0, j0 + 64) of the ket list (one per lane,

@@ -85,7 +85,8 @@ class FockDriver:
         self._block_size_factor = int(factor)
         _native.check(_native.lib().vlx_set_block_size_factor(_native.context(self._device), int(factor)))
 
-    VARIANTS = ('untiled', 'tiled4', 'tiled-wide', 'mmd', 'tiled4-p1', 'tiled-wide-p1', 'ks-lds', 'tiled-dlds')
+    VARIANTS = ('untiled', 'tiled4', 'tiled-wide', 'mmd', 'tiled4-p1', 'tiled-wide-p1', 'ks-lds', 'tiled-dlds',
+                'tiled-2seg')
 
     def _autotune(self, screener, density, ithreshold=12, repeats=2):
         """Times each angular class under every kernel variant on this density and keeps
