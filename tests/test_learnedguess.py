@@ -107,8 +107,12 @@ def test_trained_guess_cuts_scf_iterations():
     print(f'{len(X)} pair samples', flush=True)
     guess = LearnedGuess(max_batch=2048, seed=1)
     loss = guess.train(X, T, S, epochs=60, batch=1024, lr=2e-3, log=lambda m: None)
-    print(f'final training loss {loss:.3e}', flush=True)
-    assert loss < 0.1 * guess.history[0]
+    # the network learns the residual over the per-class mean blocks, normalised by one scale: its
+    # loss must fall below that of the mean blocks alone (a zero residual)
+    resid = (T - guess.baseline(X)) / guess.scale
+    base = np.mean([np.mean(resid[k].reshape(MAX_BF, MAX_BF)[:ni, :nj] ** 2) for k, (ni, nj) in enumerate(S)])
+    print(f'final training loss {loss:.3e}, mean blocks alone {base:.3e}', flush=True)
+    assert loss < 0.95 * base
     cut = []
     for seed in (31000, 31001):
         mol = water_cluster(8, 10.0 * (8 / 32.0) ** (1 / 3), seed)
