@@ -373,6 +373,46 @@ constexpr HermDecode make_herm_decode()
 }
 static __device__ constexpr HermDecode g_herm = make_herm_decode();
 
+// The step of the R recursion that produces Hermite index idx (t, u, v), as one packed int:
+// s1 | s2 << 9 | axis << 18 | c << 20.  Along the first axis with a nonzero index (x for t > 0,
+// else y, else z; k the index along it), R^n[idx] = PQ_axis R^{n+1}[s1] + c R^{n+1}[s2] with
+// s1 the index one lower along the axis, s2 two lower and c = k - 1 (s2 = 0 and c = 0 for k = 1;
+// idx 0 takes the seed).  Indices stay below 2^9 (nherm(VLX_MAX_LTOT) = 455).
+struct HermRecur {
+    int e[nherm(VLX_MAX_LTOT)];
+};
+constexpr HermRecur make_herm_recur()
+{
+    HermRecur r{};
+    auto lin = [](int t, int u, int v) {
+        const int n = t + u + v, s = u + v;
+        return n * (n + 1) * (n + 2) / 6 + s * (s + 1) / 2 + v;
+    };
+    int i = 0;
+    for (int n = 0; n <= VLX_MAX_LTOT; n++)
+        for (int s = 0; s <= n; s++)
+            for (int v = 0; v <= s; v++) {
+                const int t = n - s, u = s - v;
+                int s1 = 0, s2 = 0, axis = 0, c = 0;
+                if (t > 0) {
+                    s1 = lin(t - 1, u, v);
+                    if (t > 1) s2 = lin(t - 2, u, v), c = t - 1;
+                } else if (u > 0) {
+                    axis = 1;
+                    s1 = lin(t, u - 1, v);
+                    if (u > 1) s2 = lin(t, u - 2, v), c = u - 1;
+                } else if (v > 0) {
+                    axis = 2;
+                    s1 = lin(t, u, v - 1);
+                    if (v > 1) s2 = lin(t, u, v - 2), c = v - 1;
+                }
+                r.e[i++] = s1 | (s2 << 9) | (axis << 18) | (c << 20);
+            }
+    return r;
+}
+static __device__ constexpr HermRecur g_hrec = make_herm_recur();
+static_assert(nherm(VLX_MAX_LTOT) <= 512, "R recursion table packs indices in 9 bits");
+
 __device__ __forceinline__ int hidx_packed(int tuv)
 {
     return hidx(tuv & 0xff, (tuv >> 8) & 0xff, tuv >> 16);
@@ -417,6 +457,16 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
     __shared__ double outs[S::nsab * S::nscd];
     const int lane = threadIdx.x;
     const int lr = lane & 15, lk = lane >> 4;
+    static_assert(16 * (S::MX + S::XC) <= nherm(VLX_MAX_LTOT) && 4 * S::KX <= nherm(VLX_MAX_LTOT),
+                  "X-product rows and columns index the Hermite decode table");
+    // packed Hermite (t, u, v) of this lane's X-product columns q = 4 kq + lk (0 beyond nhcd): the
+    // same for every quartet, kept in registers when the column loop is unrolled
+    int tqv[S::PRE_X ? S::KX : 1];
+#pragma unroll
+    for (int kq = 0; kq < (S::PRE_X ? S::KX : 1); kq++) {
+        const int q = 4 * kq + lk;
+        tqv[kq] = (S::PRE_X && q < S::nhcd) ? g_herm.e[q] : 0;
+    }
     for (long long it = blockIdx.x; it < a.nitems; it += gridDim.x) {
         const WorkItem wi = items[it];
         const MolDesc md = mols[wi.mol];
@@ -484,22 +534,12 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
                             __syncthreads();
                             const int nn = nherm(L - n);
                             for (int idx = lane; idx < nn; idx += VLX_WAVE) {
-                                const int tuv = g_herm.e[idx];
-                                const int t = tuv & 0xff, u = (tuv >> 8) & 0xff, v = tuv >> 16;
-                                double val;
-                                if (t > 0) {
-                                    val = pqx * prev[hidx(t - 1, u, v)];
-                                    if (t > 1) val = fma(double(t - 1), prev[hidx(t - 2, u, v)], val);
-                                } else if (u > 0) {
-                                    val = pqy * prev[hidx(t, u - 1, v)];
-                                    if (u > 1) val = fma(double(u - 1), prev[hidx(t, u - 2, v)], val);
-                                } else if (v > 0) {
-                                    val = pqz * prev[hidx(t, u, v - 1)];
-                                    if (v > 1) val = fma(double(v - 1), prev[hidx(t, u, v - 2)], val);
-                                } else {
-                                    val = seeds[n];
-                                }
-                                cur[idx] = val;
+                                // one branch-free step from the packed table (idx 0: the seed)
+                                const int rr = g_hrec.e[idx];
+                                const int ax = (rr >> 18) & 3;
+                                const double pq = ax == 0 ? pqx : (ax == 1 ? pqy : pqz);
+                                const double val = fma(double(rr >> 20), prev[(rr >> 9) & 511], pq * prev[rr & 511]);
+                                cur[idx] = idx == 0 ? seeds[n] : val;
                             }
                         }
                         __syncthreads();
@@ -513,26 +553,34 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
 #pragma unroll
                             for (int i = 0; i < S::XC; i++) {
                                 x[i] = vlx_d4{0.0, 0.0, 0.0, 0.0};
-                                const int p = 16 * (pc + i) + lr;
+                                const int p = 16 * (pc + i) + lr;  // < 16 (MX + XC) <= HA + 16, inside g_herm
                                 prow_ok[i] = p < S::nhab;
-                                prow[i] = prow_ok[i] ? g_herm.e[p] : 0;
+                                const int tp = g_herm.e[p];
+                                prow[i] = prow_ok[i] ? tp : 0;
                             }
+                            // branch-free steps: a masked row (p >= nhab) or column (q >= nhcd) has its
+                            // Hermite index zeroed, so it reads an in-range R entry, and selects an
+                            // exact zero; the LDS reads can then be issued ahead of the products
 #pragma unroll(S::PRE_X ? S::KX : 1)
                             for (int kq = 0; kq < S::KX; kq++) {
                                 const int q = 4 * kq + lk;
                                 const bool qok = q < S::nhcd;
-                                const int tq = qok ? g_herm.e[q] : 0;
+                                int tq;
+                                if constexpr (S::PRE_X) {
+                                    tq = tqv[kq];
+                                } else {
+                                    const int t0 = g_herm.e[q];
+                                    tq = qok ? t0 : 0;
+                                }
                                 const bool neg = ((tq & 0xff) + ((tq >> 8) & 0xff) + (tq >> 16)) & 1;
                                 double bv;
                                 if constexpr (S::PRE_X) bv = ucv[kq];
                                 else bv = uc[q * S::SC];
 #pragma unroll
                                 for (int i = 0; i < S::XC; i++) {
-                                    double av = 0.0;
-                                    if (qok && prow_ok[i]) {
-                                        const double r0 = cur[hidx_packed(prow[i] + tq)];
-                                        av = neg ? -r0 : r0;
-                                    }
+                                    const double r0 = cur[hidx_packed(prow[i] + tq)];
+                                    const double sr = neg ? -r0 : r0;
+                                    const double av = (qok && prow_ok[i]) ? sr : 0.0;
                                     x[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, x[i], 0, 0, 0);
                                 }
                             }
