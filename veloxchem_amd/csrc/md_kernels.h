@@ -561,7 +561,7 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
                             // branch-free steps: a masked row (p >= nhab) or column (q >= nhcd) has its
                             // Hermite index zeroed, so it reads an in-range R entry, and selects an
                             // exact zero; the LDS reads can then be issued ahead of the products
-#pragma unroll(S::PRE_X ? S::KX : 1)
+#pragma unroll(S::PRE_X ? S::KX : 4)
                             for (int kq = 0; kq < S::KX; kq++) {
                                 const int q = 4 * kq + lk;
                                 const bool qok = q < S::nhcd;
@@ -578,6 +578,9 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
                                 else bv = uc[q * S::SC];
 #pragma unroll
                                 for (int i = 0; i < S::XC; i++) {
+                                    // a last chunk past the MX row tiles (odd MX) has nothing to compute:
+                                    // wave-uniform skip; its X rows are never read by the next product
+                                    if (S::MX % S::XC != 0 && pc + i >= S::MX) continue;
                                     const double r0 = cur[hidx_packed(prow[i] + tq)];
                                     const double sr = neg ? -r0 : r0;
                                     const double av = (qok && prow_ok[i]) ? sr : 0.0;
@@ -602,7 +605,8 @@ __global__ void __launch_bounds__(VLX_WAVE) __attribute__((amdgpu_waves_per_eu(4
                                         acc[ms] = __builtin_amdgcn_mfma_f64_16x16x4f64(uav[kp2][ms], bv, acc[ms], 0, 0, 0);
                                 }
                             } else {
-#pragma unroll 1
+                                // unrolled so that the U_AB loads of later steps issue ahead of the products
+#pragma unroll 4
                                 for (int kp2 = k0; kp2 < k1; kp2++) {
                                     const int pr = 4 * kp2 + lk;
                                     const double bv = xt[(pr - 16 * pc) * 16 + lr];
