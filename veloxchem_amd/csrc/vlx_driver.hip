@@ -293,6 +293,9 @@ struct vlx_context {
     // largest grid (one wavefront per block) of a matrix-core MD launch (VLX_MMD_GRID): on the 64-water
     // def2-TZVP build 16384 / 32768 / 65536 measured 0.775 / 0.786 / 0.793 builds/s
     int mmd_grid = 65536;
+    // kets per matrix-core MD work item (VLX_MMD_KETS): on the 64-water def2-TZVP build the MD classes
+    // took 333.2 / 327.1 / 332.1 / 342.9 ms with 1 / 2 / 4 / 8 kets
+    int mmd_kets = 2;
     // untiled items reordered so each wave's contiguous range has equal estimated cost
     // (VLX_BALANCE_ITEMS=0: the plain build order)
     int balance_items = 1;
@@ -922,7 +925,7 @@ int get_schedules(vlx_screener* s, int ithr, int k32, std::vector<std::unique_pt
         } else {
             // work items: one bra x a run of the ket prefix its bins allow
             // (64 kets, one per lane; or a few kets for the matrix-core MD kernel)
-            const int kstep = sch->mmd ? 4 : 64;
+            const int kstep = sch->mmd ? s->ctx->mmd_kets : 64;
             for (int i = 0; i < bra.size(); i++) {
                 int nk;
                 if (unscreened) {
@@ -1086,6 +1089,7 @@ int vlx_context_create(int device, vlx_context** out)
     if (const char* e = std::getenv("VLX_LAUNCH_ORDER")) ctx->launch_by_cost = std::atoi(e) != 0;
     if (const char* e = std::getenv("VLX_TILED_GRID")) ctx->tiled_grid = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("VLX_MMD_GRID")) ctx->mmd_grid = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("VLX_MMD_KETS")) ctx->mmd_kets = std::max(1, std::min(64, std::atoi(e)));
     // dynamic LDS of a tiled block: the Boys table (static) and the tiles share the 160 KB of a CU
     if (const char* e = std::getenv("VLX_MAX_TILE_LDS"))
         ctx->max_tile_lds = std::max(16 * 1024, std::min(160 * 1024 - (int)(BOYS_TAB_SIZE * sizeof(double)), std::atoi(e)));
