@@ -274,7 +274,7 @@ def multi_density(scr, dm, ithr, counts=(2, 4, 8), steps=3, seed=7):
                         _native.check(lib.vlx_context_set_timing(ctx, 0))
                         per[name] = {'%d%d%d%d' % c: v[0] for c, v in _native.class_times(0).items()}
                 finally:
-                    _native.check(lib.vlx_context_set_streams(ctx, 4))
+                    _native.check(lib.vlx_context_set_streams(ctx, _native.default_streams()))
                 class_rows = {c: {'ms_one_density': round(per['one'][c], 3), 'ms_pass': round(per['multi'].get(c, 0.0), 3),
                                   'pass_vs_separate': round(per['multi'].get(c, 0.0) / max(k * per['one'][c], 1e-9), 3)}
                               for c in per['one']}
@@ -358,7 +358,7 @@ def load_variants(scr, path):
 def workload_builds(nmol, box, seed, basis, ithr, steps, warmup, taus, autotune_repeats, workload_key,
                     save_tune=None, load_tune=None, single_stream=True):
     """FP64 builds of one seeded water cluster after its own autotuner pass: rate, per-class kernel
-    times and the roofline of the dominant class kernel (timed region, four streams), the same
+    times and the roofline of the dominant class kernel (timed region, three streams), the same
     kernel alone on one stream, and the mixed sweep with its error against FP64."""
     from veloxchem_amd import FockDriver, _native
     mol, bas, scr, dm, t_screen = build_workload(nmol, box, seed, basis)
@@ -389,7 +389,7 @@ def workload_builds(nmol, box, seed, basis, ithr, steps, warmup, taus, autotune_
         try:
             ms1, times1, _, _ = measure_builds(scr, dm, ithr, 1, 1, [])
         finally:
-            _native.check(lib.vlx_context_set_streams(ctx, max(1, min(4, int(os.environ.get('VLX_STREAMS', 4))))))
+            _native.check(lib.vlx_context_set_streams(ctx, _native.default_streams()))
         rf1 = roofline(scr, ithr, times1, ms1, workload_key)
     out = {'workload': f'{nmol}-water cluster (seed {seed}, {box} A box), {basis}, {n} AOs, 2jk FP64 build on the '
                        f'SAD density, ithreshold {ithr}',

@@ -229,7 +229,7 @@ int vlx_copy_to_host(vlx_context* ctx, void* dst, const void* src, unsigned long
 int vlx_context_synchronize(vlx_context* ctx);
 
 /* Side streams the class kernels of later builds fan out to (1..4; the
- * VLX_STREAMS environment variable sets the initial value, default 4).  No
+ * VLX_STREAMS environment variable sets the initial value, default 3).  No
  * reference counterpart: one stream serialises the class kernels, for timing
  * each in isolation. */
 int vlx_context_set_streams(vlx_context* ctx, int nstreams);

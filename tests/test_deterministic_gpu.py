@@ -51,7 +51,7 @@ def test_deterministic_build_is_bitwise_repeatable_and_matches_oracle():
         try:
             runs.append(np.asarray(drv.compute(scr, D, '2jkx', 0.3, 0.0, 12)))
         finally:
-            _native.check(lib.vlx_context_set_streams(ctx, 4))
+            _native.check(lib.vlx_context_set_streams(ctx, _native.default_streams()))
         rng = np.random.default_rng(3)
         Dg = D + 0.05 * rng.standard_normal((n, n))
         dm = make_matrix(bas, mat_t.general)

@@ -306,7 +306,7 @@ def test_every_variant_matches_oracle():
                     assert reference_equal(gt, tz[2]), name
     finally:
         _native.check(_native.lib().vlx_context_set_variant(ctx, -1))
-        _native.check(_native.lib().vlx_context_set_streams(ctx, 4))
+        _native.check(_native.lib().vlx_context_set_streams(ctx, _native.default_streams()))
 
 
 def test_batch_matches_single_builds():

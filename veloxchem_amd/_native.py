@@ -126,6 +126,11 @@ def check(rc):
         raise NativeError(msg)
 
 
+def default_streams():
+    """The side-stream count a new context starts with (VLX_STREAMS, default 3, within 1..4)."""
+    return max(1, min(4, int(os.environ.get('VLX_STREAMS', 3))))
+
+
 def context(device=0):
     """Per-device context (stream + Boys table), created on first use."""
     if device not in _contexts:

@@ -266,7 +266,9 @@ bool variant_tiled(int v)
 
 struct vlx_context {
     int device = 0;
-    int nstreams = NSIDE;           // side streams in use (VLX_STREAMS, 1..NSIDE)
+    // side streams in use (VLX_STREAMS, 1..NSIDE): three measured faster than four on the 64-water
+    // def2-TZVP build (0.852 against 0.815 builds/s) and the 32-water def2-SVP build (45.7 against 43.8)
+    int nstreams = 3;
     int max_chunk = BRA_CHUNK;      // bras per work item upper bound (VLX_BRA_CHUNK)
     int min_items = 16384;          // work items a class launch aims for (VLX_MIN_ITEMS)
     int use_tiled = 1;              // LDS-tiled kernels where built (VLX_TILED)
